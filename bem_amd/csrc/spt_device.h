@@ -1,0 +1,154 @@
+// Device-side maths for the MI355X tracing path: complete elliptic
+// integrals of the complementary parameter, m = 0 ring kernels, block
+// reductions. FP64 throughout; explicit fma() where fusion is wanted (the
+// library is built with -ffp-contract=off so every other expression
+// rounds like the NumPy reference).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "ellip_coeffs.h"
+
+namespace spt {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kTwoPi = 6.28318530717958647692;
+constexpr double kInv4Pi2 = 1.0 / (4.0 * kPi * kPi);
+constexpr double kInv2Pi2 = 1.0 / (2.0 * kPi * kPi);
+constexpr double kInvPi = 1.0 / kPi;
+
+__constant__ double c_PK[11] = SPT_ELLIP_PK;
+__constant__ double c_QK[11] = SPT_ELLIP_QK;
+__constant__ double c_PE[11] = SPT_ELLIP_PE;
+__constant__ double c_QE[11] = SPT_ELLIP_QE;
+
+__device__ __forceinline__ double horner11(const double* c, double x) {
+  double s = c[10];
+#pragma unroll
+  for (int i = 9; i >= 0; --i) s = fma(s, x, c[i]);
+  return s;
+}
+
+// K(1-p), E(1-p) for p in [0, 1]: degree-10 P/Q pairs in the logarithmic
+// form of DLMF 19.12.1-2 (coefficients fitted by gen_ellip_coeffs.py).
+__device__ __forceinline__ void ellip_ke_c(double p, double& K, double& E) {
+  const double lp = log(p);
+  K = fma(-lp, horner11(c_QK, p), horner11(c_PK, p));
+  E = fma(-lp * p, horner11(c_QE, p), horner11(c_PE, p));
+}
+
+__device__ __forceinline__ double ellip_k_c(double p) {
+  const double lp = log(p);
+  return fma(-lp, horner11(c_QK, p), horner11(c_PK, p));
+}
+
+// 1/x to ~1 ulp: hardware reciprocal estimate plus two Newton steps.
+__device__ __forceinline__ double rcp64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// 1/sqrt(x) to ~1 ulp: hardware estimate plus two Newton steps.
+__device__ __forceinline__ double rsqrt64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double h = 0.5 * x;
+  double e = fma(-h * y, y, 0.5);
+  y = fma(y, e, y);
+  e = fma(-h * y, y, 0.5);
+  return fma(y, e, y);
+}
+
+// Near-axis Taylor terms of the ring field (reference
+// bem/axisymmetric.py:262-264), full constants included.
+__device__ __noinline__ void ring_field_axis(double rho, double z, double a, double& er, double& ez) {
+  const double ad2 = a * a + z * z;
+  er = rho * (2 * z * z - a * a) / (8 * kPi * pow(ad2, 2.5));
+  ez = z / (4 * kPi * pow(ad2, 1.5)) + 3 * rho * rho * z * (3 * a * a - 2 * z * z) / (16 * kPi * pow(ad2, 3.5));
+}
+
+// Weighted ring-field accumulation for one source ring (radius a, axial
+// offset zz = z_point - z_ring, weight w = charge). The main-branch terms
+// are accumulated without their per-point constants:
+//   er_main += w (K - (a^2 - rho^2 + zz^2) E / d2) / sqrt(D)   [x 1/(4 pi^2 rho) later]
+//   ez_main += w zz E / (sqrt(D) d2)                           [x 1/(2 pi^2) later]
+// Near-axis nodes (rho^2 < 1e-10 (a^2 + zz^2)) go to er_ax/ez_ax with
+// their constants (reference bem/axisymmetric.py:248-265).
+struct RingAcc {
+  double er_main, ez_main, er_ax, ez_ax;
+  __device__ void zero() { er_main = ez_main = er_ax = ez_ax = 0.0; }
+};
+
+__device__ __forceinline__ void ring_accum(double rho, double rho2, double zz, double a, double w, RingAcc& acc) {
+  const double zz2 = zz * zz;
+  const double a2 = a * a;
+  if (rho2 < 1e-10 * (a2 + zz2)) {
+    double er, ez;
+    ring_field_axis(rho, zz, a, er, ez);
+    acc.er_ax = fma(w, er, acc.er_ax);
+    acc.ez_ax = fma(w, ez, acc.ez_ax);
+    return;
+  }
+  const double sp = rho + a, sm = rho - a;
+  const double D = fma(sp, sp, zz2);
+  const double d2 = fma(sm, sm, zz2);
+  const double rD = rsqrt64(D);
+  const double p = d2 * (rD * rD);
+  double K, E;
+  ellip_ke_c(fmin(p, 1.0), K, E);
+  const double inv_d2 = rcp64(d2);
+  const double c = (a2 - rho2) + zz2;
+  const double wr = w * rD;
+  acc.er_main = fma(wr, fma(-c * E, inv_d2, K), acc.er_main);
+  acc.ez_main = fma(wr * zz, E * inv_d2, acc.ez_main);
+}
+
+__device__ __forceinline__ void ring_finish(const RingAcc& acc, double rho, double& er, double& ez) {
+  er = (rho > 0.0 ? acc.er_main * kInv4Pi2 / rho : 0.0) + acc.er_ax;
+  ez = acc.ez_main * kInv2Pi2 + acc.ez_ax;
+}
+
+// Potential of a unit-total-charge ring, G = 1/(4 pi r) (reference
+// bem/axisymmetric.py:221-229).
+__device__ __forceinline__ double ring_potential(double rho, double z, double a) {
+  const double sp = rho + a, sm = rho - a;
+  const double D = sp * sp + z * z;
+  const double p = (sm * sm + z * z) / D;
+  return ellip_k_c(fmin(p, 1.0)) / (2.0 * kPi * kPi * sqrt(D));
+}
+
+// ---- block reductions (deterministic: fixed tree over thread index) ----
+template <int T>
+__device__ double block_sum(double v, double* red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = T / 2; s > 0; s >>= 1) {
+    if (tid < s) red[tid] = red[tid] + red[tid + s];
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+template <int T>
+__device__ double block_min(double v, double* red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = T / 2; s > 0; s >>= 1) {
+    if (tid < s) red[tid] = fmin(red[tid], red[tid + s]);
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+}  // namespace spt

@@ -1,0 +1,259 @@
+"""Write the golden fixtures under tests/golden/ from the reference itself.
+
+Run in a container that has the reference checkout (path in SPT_REFERENCE,
+default /root/reference); nothing here runs in the test suite. The
+reference needs openPMD-beamphysics only for its ParticleGroup container,
+which is not installed, so a minimal data-only stand-in is registered under
+that module name before the import.
+
+Only functions that no closed form, library routine or invariant pins are
+captured here:
+
+* bem_sphere.npz  -- reference HemisphericalTipBEMField solve (sigma) and
+  evaluate_axisymmetric_field / geometry.field at far, near, on-surface and
+  on-axis points;
+* bem_post80.npz  -- sigma and field for a coarse (80-segment) post profile,
+  solved by the reference's AxisymmetricBEMSolution.solve;
+* bem_post400.npz -- sigma for the 400-segment post profile of the
+  benchmark workload;
+* run_*.npz       -- full tracker runs: per-group attempts/accepted, final
+  state and screen events in the reference's own recording order.
+"""
+import os
+import sys
+import time
+import types
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = os.environ.get("SPT_REFERENCE", "/root/reference")
+sys.path.insert(0, ROOT)
+sys.path.insert(0, REF)
+
+
+class _ParticleGroup:
+    """Data-only stand-in for beamphysics.ParticleGroup (attributes used by
+    the reference tracker: x..pz, t, weight, status, id, species, mass,
+    species_charge, len)."""
+
+    def __init__(self, data):
+        self._d = dict(data)
+
+    def __getattr__(self, k):
+        d = self.__dict__["_d"]
+        if k in d:
+            return d[k]
+        raise AttributeError(k)
+
+    @property
+    def mass(self):
+        return 510998.95069
+
+    @property
+    def species_charge(self):
+        return -1.602176634e-19
+
+    def __len__(self):
+        return len(self._d["x"])
+
+
+_stub = types.ModuleType("beamphysics")
+_stub.ParticleGroup = _ParticleGroup
+sys.modules["beamphysics"] = _stub
+
+from specific_particle_tracer import SpecificParticleTracer, FlatCathode  # noqa: E402
+from specific_particle_tracer import batched_rk45  # noqa: E402
+from specific_particle_tracer import geometry as ref_geometry  # noqa: E402
+from specific_particle_tracer.bem import axisymmetric as ref_axi  # noqa: E402
+from specific_particle_tracer.bem import fields as ref_bem_fields  # noqa: E402
+from specific_particle_tracer.bem import geometry as ref_bem_geometry  # noqa: E402
+from specific_particle_tracer.fields import GunField  # noqa: E402
+
+from bem_amd import emission  # noqa: E402
+from bem_amd.bem.mesh import post_profile  # noqa: E402
+
+
+def field_points(profile, scale, rng, n_far=60, n_near=60):
+    """Exterior sample points: far, near (within a segment length), exactly
+    on nodes and on segment midpoints, on the axis, and below z = 0."""
+    prof = np.asarray(profile)
+    seg = np.diff(prof, axis=0)
+    L = np.linalg.norm(seg, axis=1)
+    nrm = np.stack([-seg[:, 1], seg[:, 0]], axis=1) / L[:, None]
+    pts = []
+    # far: spherical shell 1.5..20 scale, upper half
+    r = scale * (1.5 + 18.5 * rng.random(n_far))
+    th = np.arccos(rng.random(n_far))
+    ph = 2 * np.pi * rng.random(n_far)
+    pts.append(np.stack([r * np.sin(th) * np.cos(ph), r * np.sin(th) * np.sin(ph), r * np.cos(th)], -1))
+    # near: along outward normals of random segments at 1e-4..2 segment lengths
+    j = rng.integers(0, len(seg), n_near)
+    u = rng.random(n_near)
+    dist = L[j] * 10 ** rng.uniform(-4, 0.3, n_near)
+    rz = prof[j] + u[:, None] * seg[j] + dist[:, None] * nrm[j]
+    ph = 2 * np.pi * rng.random(n_near)
+    pts.append(np.stack([rz[:, 0] * np.cos(ph), rz[:, 0] * np.sin(ph), rz[:, 1]], -1))
+    # exactly on the surface: nodes and midpoints
+    k = rng.integers(1, len(prof) - 1, 8)
+    on = np.concatenate([prof[k], 0.5 * (prof[k] + prof[k + 1])])
+    pts.append(np.stack([on[:, 0], np.zeros(len(on)), on[:, 1]], -1))
+    # axis
+    za = scale * np.array([1.2, 2.0, 5.0, 30.0]) + prof[0, 1]
+    pts.append(np.stack([np.zeros(4), np.zeros(4), za], -1))
+    return np.concatenate(pts, 0)
+
+
+def capture_sphere():
+    Ez, R = -1e8, 50e-9
+    t0 = time.time()
+    fld = ref_bem_fields.HemisphericalTipBEMField(Ez, R)
+    sol = fld.solution
+    rng = np.random.default_rng(1)
+    pts = field_points(sol.profile, R, rng)
+    pert = sol.field(pts)
+    geom = ref_bem_geometry.HemisphericalTipBEMGeometry(Ez, R)
+    gpts = np.concatenate([pts, np.stack([np.zeros(5), 3e-9 * np.ones(5), R * np.array([5, 12, 15, 19, 25])], -1)])
+    gfield = geom.field(gpts)
+    np.savez(os.path.join(HERE, "bem_sphere.npz"), Ez=Ez, R=R, profile=sol.profile, sigma=sol.sigma,
+             points=pts, pert=pert, gpoints=gpts, gfield=gfield, z_handoff=geom.z_handoff)
+    print(f"bem_sphere: {len(sol.profile)} nodes, {len(pts)} points, {time.time() - t0:.1f} s", flush=True)
+
+
+def capture_post(n_seg, with_field):
+    Ez, R, H = -1e8, 1e-6, 10e-6
+    prof = post_profile(R, H, n_seg)
+    t0 = time.time()
+    sol = ref_axi.AxisymmetricBEMSolution.solve(prof, dirichlet_fn=lambda rho, z: Ez * z)
+    out = dict(Ez=Ez, R=R, H=H, profile=prof, sigma=sol.sigma)
+    if with_field:
+        rng = np.random.default_rng(2)
+        pts = field_points(prof, R, rng)
+        out.update(points=pts, pert=sol.field(pts))
+    np.savez(os.path.join(HERE, f"bem_post{n_seg}.npz"), **out)
+    print(f"bem_post{n_seg}: {len(prof)} nodes, {time.time() - t0:.1f} s", flush=True)
+
+
+class _PostGeometry(ref_geometry.Geometry):
+    """The post-on-plane field assembled from reference pieces: the
+    reference's AxisymmetricBEMSolution.field on the mirror-doubled post
+    profile, plus the uniform field, masked outside the capsule and z >= 0,
+    blended to the reference GunField with the reference's smoothstep."""
+
+    def __init__(self, E_gun, R, H, profile, sigma, z_handoff, xp=np):
+        self.E_gun, self.R, self.H, self.z_handoff = E_gun, R, H, z_handoff
+        self.profile, self.sigma = profile, sigma
+        self.sol = ref_axi.AxisymmetricBEMSolution(profile, sigma)
+        self.flat = GunField(E_gun)
+
+    def _r(self, p):
+        zc = np.clip(p[..., 2], -self.H, self.H)
+        return np.sqrt(p[..., 0] ** 2 + p[..., 1] ** 2 + (p[..., 2] - zc) ** 2)
+
+    def field(self, p):
+        valid = (self._r(p) >= self.R * (1 - 1e-6)) & (p[..., 2] >= 0)
+        near = np.zeros_like(p)
+        near[..., 2] = self.E_gun
+        near = np.where(valid[..., None], near + self.sol.field(p), 0.0)
+        return ref_bem_geometry._blend_with_flat_fallback(near, self.flat.evaluate(p), p[..., 2],
+                                                          0.5 * self.z_handoff, self.z_handoff, np)
+
+    def image_force(self, p, q, a, pr):
+        return np.zeros_like(p)
+
+    def kill_mask(self, p):
+        rk = self.R * (1 - 1e-6)
+        return (self._r(p) ** 2 <= rk * rk) | (p[..., 2] <= 0.0)
+
+    def worker_args(self):
+        return ("golden_post", self.E_gun, self.R, self.H, self.profile, self.sigma, self.z_handoff)
+
+    @classmethod
+    def _from_worker_args(cls, args, xp=np):
+        return cls(*args, xp=xp)
+
+
+ref_geometry._WORKER_REGISTRY["golden_post"] = _PostGeometry
+
+
+def traced_run(name, src, n_emit, geometry, screens, **kw):
+    """Run the reference tracker, capturing per-group attempt/accept counts
+    (by wrapping _attempt_step) and the integrator's final state."""
+    counts = {}
+    orig_attempt = batched_rk45._attempt_step
+    orig_integrate = batched_rk45.integrate
+    final = {}
+
+    def attempt(accel_fn, pos, vel, t, h, tb, td, gidx, *a):
+        out = orig_attempt(accel_fn, pos, vel, t, h, tb, td, gidx, *a)
+        acc = out[3]
+        for g, ok in zip(gidx.tolist(), acc.tolist()):
+            c = counts.setdefault(g, [0, 0])
+            c[0] += 1
+            c[1] += int(ok)
+        return out
+
+    def integrate(*a, **k):
+        pos, vel = orig_integrate(*a, **k)
+        final["pos"], final["vel"] = pos, vel
+        return pos, vel
+
+    batched_rk45._attempt_step = attempt
+    batched_rk45.integrate = integrate
+    try:
+        pg = _ParticleGroup(dict(src, weight=np.full(len(src["x"]), 1.602176634e-19), status=np.ones(len(src["x"])),
+                                 species="electron"))
+        t0 = time.time()
+        tr = SpecificParticleTracer(pg, n_emit, geometry, screens, **kw)
+        res, _ = tr.run()
+        dt = time.time() - t0
+    finally:
+        batched_rk45._attempt_step = orig_attempt
+        batched_rk45.integrate = orig_integrate
+    G = tr.n_groups
+    out = dict(attempts=np.array([counts[g][0] for g in range(G)]),
+               accepted=np.array([counts[g][1] for g in range(G)]),
+               pos=final["pos"], vel=final["vel"], t_max=tr.t_max, screens=np.array(screens, dtype=float),
+               rtol=tr.rtol, atol=tr.atol, n_emit=n_emit)
+    for k, v in src.items():
+        out["in_" + k] = np.asarray(v)
+    for si, s in enumerate(res):
+        for f in ("x", "y", "z", "px", "py", "pz", "t", "id"):
+            out[f"s{si}_{f}"] = np.asarray(getattr(s, f))
+    np.savez(os.path.join(HERE, f"run_{name}.npz"), **out)
+    print(f"run_{name}: groups {G}, attempts {out['attempts'].tolist()[:8]}..., "
+          f"events {[len(s) for s in res]}, {dt:.1f} s", flush=True)
+
+
+def main(which):
+    if "sphere" in which:
+        capture_sphere()
+    if "post80" in which:
+        capture_post(80, True)
+    if "post400" in which:
+        capture_post(400, False)
+    if "c1" in which:
+        src = emission.flat_gaussian_source(64, seed=0)
+        traced_run("c1", src, 64, FlatCathode(-1e7, z0=3e-9), [100e-6], rtol=1e-8, atol=1e-12, t_max=1e-9)
+    if "c1small" in which:
+        # 4 groups of 8, flat cathode with image, two screens, late births
+        src = emission.flat_gaussian_source(32, seed=3)
+        src["t"] = np.repeat([0.0, 2e-14, 0.0, 5e-14], 8) + np.tile(np.linspace(0, 3e-14, 8), 4)
+        traced_run("c1small", src, 8, FlatCathode(-1e8, z0=3e-9), [0.2e-6, 1e-6], rtol=1e-8, atol=1e-12,
+                   t_max=1.5e-12)
+    if "tipbem" in which:
+        R = 1e-6
+        src = emission.cap_source(8, R, 0.0, seed=4)
+        geo = ref_bem_geometry.HemisphericalTipBEMGeometry(-1e8, R)
+        traced_run("tipbem", src, 4, geo, [3e-6, 8e-6], rtol=1e-9, atol=1e-12)
+    if "post" in which:
+        d = np.load(os.path.join(HERE, "bem_post80.npz"))
+        R, H = float(d["R"]), float(d["H"])
+        src = emission.cap_source(8, R, H, seed=5)
+        geo = _PostGeometry(-1e8, R, H, d["profile"], d["sigma"], 20 * (R + H))
+        traced_run("post80", src, 4, geo, [12e-6, 20e-6], rtol=1e-9, atol=1e-12)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:] or ["sphere", "post80", "c1", "c1small", "tipbem", "post", "post400"])
