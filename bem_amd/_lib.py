@@ -1,0 +1,182 @@
+"""ctypes binding of libspt_hip.so (declarations in include/spt.h).
+
+Device buffers are torch tensors on the context's device; only their
+data_ptr() crosses the boundary. Errors come back as negative codes and
+are raised here as the reference's exception types: ValueError for bad
+input, RuntimeError otherwise (no convergence, HIP failures).
+
+There is no CPU fallback: if the library or a GPU is missing, every entry
+point raises.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libspt_hip.so")
+
+SPT_OK, SPT_EINVAL, SPT_ENOMEM, SPT_EHIP, SPT_ENOBEM, SPT_ESTEPCAP = 0, -1, -2, -3, -4, -5
+SPT_MAX_SCREENS = 16
+GEOM_FLAT, GEOM_TIP, GEOM_BEM_POST = 0, 1, 2
+IMAGE_NONE, IMAGE_PLANE, IMAGE_TIP3 = 0, 1, 2
+STATUS_DONE, STATUS_STEPCAP = 0, 1
+
+EXPORTS = (
+    "spt_create", "spt_destroy", "spt_last_error", "spt_set_stream", "spt_sync", "spt_device_cus",
+    "spt_bem_upload", "spt_bem_assemble", "spt_bem_solve", "spt_bem_pert_field", "spt_geometry_field",
+    "spt_pair_forces", "spt_kill_mask", "spt_trace",
+)
+
+
+class Geometry_t(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32), ("image", ctypes.c_int32),
+        ("E_gun", ctypes.c_double), ("R", ctypes.c_double), ("H", ctypes.c_double),
+        ("z0", ctypes.c_double), ("z_handoff", ctypes.c_double),
+        ("has_kill_z", ctypes.c_int32), ("has_z_max", ctypes.c_int32),
+        ("kill_z_below", ctypes.c_double), ("z_max", ctypes.c_double),
+    ]
+
+
+class Event_t(ctypes.Structure):
+    _fields_ = [
+        ("group", ctypes.c_int32), ("particle", ctypes.c_int32), ("screen", ctypes.c_int32),
+        ("interval", ctypes.c_int32), ("batch", ctypes.c_int64), ("t", ctypes.c_double),
+        ("pos", ctypes.c_double * 3), ("vel", ctypes.c_double * 3),
+    ]
+
+
+EVENT_DTYPE = np.dtype([
+    ("group", np.int32), ("particle", np.int32), ("screen", np.int32), ("interval", np.int32),
+    ("batch", np.int64), ("t", np.float64), ("pos", np.float64, 3), ("vel", np.float64, 3),
+])
+assert EVENT_DTYPE.itemsize == ctypes.sizeof(Event_t) == 80
+
+_p = ctypes.c_void_p
+
+
+class TraceArgs_t(ctypes.Structure):
+    _fields_ = [
+        ("geom", Geometry_t),
+        ("n_groups", ctypes.c_int32), ("block_threads", ctypes.c_int32), ("n_particles", ctypes.c_int64),
+        ("group_offsets", _p), ("group_order", _p),
+        ("pos", _p), ("vel", _p), ("t_birth", _p), ("charge", _p), ("mass", _p), ("t_death", _p),
+        ("q_over_m", ctypes.c_double), ("plummer_radius", ctypes.c_double), ("t_max", ctypes.c_double),
+        ("rtol", ctypes.c_double), ("atol", ctypes.c_double),
+        ("safety", ctypes.c_double), ("min_factor", ctypes.c_double), ("max_factor", ctypes.c_double),
+        ("initial_step_fraction", ctypes.c_double),
+        ("max_steps", ctypes.c_int64),
+        ("n_screens", ctypes.c_int32), ("pad0", ctypes.c_int32),
+        ("screens", ctypes.c_double * SPT_MAX_SCREENS),
+        ("attempts", _p), ("accepted", _p), ("active_stages", _p), ("margin", _p), ("status", _p),
+        ("events", _p), ("event_capacity", ctypes.c_int64), ("event_count", _p),
+    ]
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load():
+    """Load (once) and return the ctypes library handle; raises if absent."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build() (hipcc, gfx950)")
+        lib = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        sig = {
+            "spt_create": (ctypes.c_int, [ctypes.c_int, P(_p)]),
+            "spt_destroy": (None, [_p]),
+            "spt_last_error": (ctypes.c_char_p, [_p]),
+            "spt_set_stream": (ctypes.c_int, [_p, _p]),
+            "spt_sync": (ctypes.c_int, [_p]),
+            "spt_device_cus": (ctypes.c_int, [_p, P(ctypes.c_int)]),
+            "spt_bem_upload": (ctypes.c_int, [_p, _p, ctypes.c_int32, _p, ctypes.c_int32]),
+            "spt_bem_assemble": (ctypes.c_int, [_p, _p, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _p]),
+            "spt_bem_solve": (ctypes.c_int, [_p, _p, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_int32, _p]),
+            "spt_bem_pert_field": (ctypes.c_int, [_p, _p, ctypes.c_int64, _p]),
+            "spt_geometry_field": (ctypes.c_int, [_p, P(Geometry_t), _p, ctypes.c_int64, _p]),
+            "spt_pair_forces": (ctypes.c_int, [_p, P(Geometry_t), _p, _p, _p, _p, ctypes.c_int32,
+                                               ctypes.c_double, _p, _p]),
+            "spt_kill_mask": (ctypes.c_int, [_p, P(Geometry_t), _p, ctypes.c_int64, _p]),
+            "spt_trace": (ctypes.c_int, [_p, P(TraceArgs_t)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+        return lib
+
+
+class Context:
+    """One library context on one device (reference: the per-device cache
+    of backend.array_cache_key, backend.py:49-51)."""
+
+    _by_device = {}
+
+    def __init__(self, device=0):
+        import torch
+
+        if not torch.cuda.is_available():
+            raise RuntimeError("backend='mi355x' needs a visible AMD GPU (torch.cuda.is_available() is False)")
+        self.lib = load()
+        self.device = int(device)
+        self.torch_device = torch.device("cuda", self.device)
+        h = _p()
+        rc = self.lib.spt_create(self.device, ctypes.byref(h))
+        if rc != SPT_OK:
+            raise RuntimeError(f"spt_create failed ({rc})")
+        self.h = h
+        self.bem_key = None
+
+    @classmethod
+    def get(cls, device=0):
+        ctx = cls._by_device.get(device)
+        if ctx is None:
+            ctx = cls._by_device[device] = Context(device)
+        return ctx
+
+    def check(self, rc, what):
+        if rc == SPT_OK:
+            return
+        msg = self.lib.spt_last_error(self.h).decode(errors="replace")
+        if rc in (SPT_EINVAL, SPT_ENOBEM):
+            raise ValueError(f"{what}: {msg}")
+        raise RuntimeError(f"{what} failed ({rc}): {msg}")
+
+    def bind_stream(self):
+        import torch
+
+        s = torch.cuda.current_stream(self.torch_device)
+        self.check(self.lib.spt_set_stream(self.h, _p(s.cuda_stream)), "spt_set_stream")
+
+    def sync(self):
+        self.check(self.lib.spt_sync(self.h), "spt_sync")
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.lib.spt_destroy(self.h)
+        except Exception:
+            pass
+
+
+def ptr(t):
+    """data_ptr of a contiguous tensor (or NULL for None)."""
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise ValueError("device buffers must be contiguous")
+    return _p(t.data_ptr())
+
+
+def host_ptr(a):
+    a = np.ascontiguousarray(a)
+    return a, a.ctypes.data_as(_p)

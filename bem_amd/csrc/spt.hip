@@ -1,0 +1,1336 @@
+// libspt_hip: MI355X (gfx950) kernels and C ABI for the
+// specific_particle_tracer tracing path. Declarations: include/spt.h.
+//
+// Layout in HBM
+//   BEM node table   double4[n_seg * n_sub * 4] = (rho_q, z_q, area_q*sigma_q, 0),
+//                    segment-major so a lane walks one segment contiguously.
+//   segment table    double[n_seg * 12] (p0, tangent, |t|^2, length, unit, normal).
+//   particle state   caller's (n, 3) pos/vel, CSR group offsets.
+//   stage buffer     library scratch double[7 stages][6 comps][n] (coalesced
+//                    across the threads of a group).
+//
+// Execution
+//   spt_trace launches one persistent kernel. Each workgroup pulls a group
+//   index from a device queue and integrates that group to t_max with its
+//   own t and h: stage positions in LDS, external field by a block-
+//   cooperative BEM gather (or closed form), in-group pair forces from LDS,
+//   RMS error norm by a fixed-order block reduction, accept/reject, birth
+//   checkpoints, kill and screen crossings, all on device.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+
+#include "../../include/spt.h"
+#include "spt_device.h"
+
+using namespace spt;
+
+// Gauss-Legendre nodes/weights exactly as numpy.polynomial.legendre.leggauss
+// returns them (the reference's rules, bem/axisymmetric.py:94,98).
+constexpr double c_gx4[4] = {-0.8611363115940526, -0.33998104358485626, 0.33998104358485626, 0.8611363115940526};
+constexpr double c_gw4[4] = {0.3478548451374537, 0.6521451548625462, 0.6521451548625462, 0.3478548451374537};
+constexpr double c_gx8[8] = {-0.9602898564975362, -0.7966664774136267, -0.525532409916329, -0.18343464249564978,
+                                0.18343464249564978, 0.525532409916329, 0.7966664774136267, 0.9602898564975362};
+constexpr double c_gw8[8] = {0.10122853629037669, 0.22238103445337434, 0.31370664587788705, 0.36268378337836177,
+                                0.36268378337836177, 0.31370664587788705, 0.22238103445337434, 0.10122853629037669};
+
+// Dormand-Prince 5(4) tableau (Dormand & Prince 1980), the pair the
+// reference integrates with (batched_rk45.py:30-50).
+__constant__ double c_A[6][5] = {
+    {0, 0, 0, 0, 0},
+    {1.0 / 5, 0, 0, 0, 0},
+    {3.0 / 40, 9.0 / 40, 0, 0, 0},
+    {44.0 / 45, -56.0 / 15, 32.0 / 9, 0, 0},
+    {19372.0 / 6561, -25360.0 / 2187, 64448.0 / 6561, -212.0 / 729, 0},
+    {9017.0 / 3168, -355.0 / 33, 46732.0 / 5247, 49.0 / 176, -5103.0 / 18656},
+};
+__constant__ double c_B[6] = {35.0 / 384, 0.0, 500.0 / 1113, 125.0 / 192, -2187.0 / 6784, 11.0 / 84};
+__constant__ double c_E[7] = {71.0 / 57600, 0.0, -71.0 / 16695, 71.0 / 1920, -17253.0 / 339200, 22.0 / 525, -1.0 / 40};
+
+constexpr double kCoulomb = 8987551785.9720688;  // 1/(4 pi eps0), eps0 = 8.8541878188e-12
+constexpr double kLight = 299792458.0;
+
+// ---------------------------------------------------------------- tables
+enum { S_P0R = 0, S_P0Z, S_ABR, S_ABZ, S_ABAB, S_L, S_UR, S_UZ, S_NR, S_NZ, SEG_STRIDE = 12 };
+
+struct BemView {
+  int n_seg;
+  int nq;                 // nodes per segment (n_sub * 4)
+  const double4* nodes;   // n_seg * nq
+  const double* segs;     // n_seg * SEG_STRIDE
+  const double* sigma;    // n_seg + 1
+};
+
+// Segment geometry (reference bem/axisymmetric.py:329-338,490-498).
+__global__ void seg_table_kernel(const double* prof, int n_seg, double* segs) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_seg) return;
+  const double p0r = prof[2 * j], p0z = prof[2 * j + 1];
+  const double tr = prof[2 * j + 2] - p0r, tz = prof[2 * j + 3] - p0z;
+  const double L = sqrt(tr * tr + tz * tz);
+  const double nn = sqrt((-tz) * (-tz) + tr * tr);
+  double* g = segs + (size_t)j * SEG_STRIDE;
+  g[S_P0R] = p0r;
+  g[S_P0Z] = p0z;
+  g[S_ABR] = tr;
+  g[S_ABZ] = tz;
+  g[S_ABAB] = tr * tr + tz * tz;
+  g[S_L] = L;
+  g[S_UR] = tr / L;
+  g[S_UZ] = tz / L;
+  g[S_NR] = -tz / nn;
+  g[S_NZ] = tr / nn;
+  g[10] = 0.0;
+  g[11] = 0.0;
+}
+
+// Fixed quadrature node table (reference bem/axisymmetric.py:305-326):
+// weight = area_q * sigma_q with area_q = 2 pi rho_q (L / n_sub) w_g / 2.
+__global__ void node_table_kernel(const double* prof, const double* sigma, int n_seg, int n_sub, double4* nodes) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int per = n_sub * 4;
+  if (k >= n_seg * per) return;
+  const int j = k / per, r = k - j * per, sub = r / 4, gq = r - sub * 4;
+  const double p0r = prof[2 * j], p0z = prof[2 * j + 1];
+  const double p1r = prof[2 * j + 2], p1z = prof[2 * j + 3];
+  const double dr = p1r - p0r, dz = p1z - p0z;
+  const double L = sqrt(dr * dr + dz * dz);
+  const double ta = (double)sub / (double)n_sub, tb = (double)(sub + 1) / (double)n_sub;
+  const double t = ta + 0.5 * (tb - ta) * (c_gx4[gq] + 1.0);
+  const double rho = p0r + t * (p1r - p0r);
+  const double z = p0z + t * (p1z - p0z);
+  const double seg_len = (L / n_sub) * 0.5 * c_gw4[gq];
+  const double area = 2.0 * kPi * rho * seg_len;
+  const double sq = sigma[j] + t * (sigma[j + 1] - sigma[j]);
+  nodes[k] = make_double4(rho, z, area * sq, 0.0);
+}
+
+// ------------------------------------------------------- BEM field gather
+// Near segment (closer than its own length): s = s0 + d sinh(v), Gauss-8 in
+// intervals of width <= 0.5 over [asinh(-s0/d), asinh((L-s0)/d)], density
+// linear between the segment's nodal values (reference
+// bem/axisymmetric.py:571-600).
+__device__ __noinline__ void near_segment(double re, double re2, double ze, const double* g, double sig0, double sig1,
+                                          double proj, double dist, RingAcc& acc) {
+  const double L = g[S_L];
+  const double d = fmax(dist, 1e-15 * L);
+  const double s0 = proj;
+  const double lo = asinh(-s0 / d);
+  const double hi = asinh((L - s0) / d);
+  const int count = max(1, (int)ceil((hi - lo) / 0.5));
+  const double step = (hi - lo) / count;
+  const double half = 0.5 * (hi - lo) / count;
+  const double p0r = g[S_P0R], p0z = g[S_P0Z], ur = g[S_UR], uz = g[S_UZ];
+  for (int k = 0; k < count; ++k) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const double v = lo + (k + 0.5 * (c_gx8[q] + 1.0)) * step;
+      const double sh = sinh(v), ch = cosh(v);
+      const double s = s0 + d * sh;
+      const double sr = p0r + s * ur;
+      const double sz = p0z + s * uz;
+      const double dens = sig0 + (s / L) * (sig1 - sig0);
+      const double w = dens * (2 * kPi * sr) * d * ch * half * c_gw8[q];
+      ring_accum(re, re2, ze - sz, sr, w, acc);
+    }
+  }
+}
+
+template <int T, int NMAX>
+struct BemShared {
+  double re[NMAX], ze[NMAX], rho[NMAX], cph[NMAX], sph[NMAX];
+  union {
+    struct {
+      double bd2[T];
+      int bseg[T];
+    } a;
+    double acc[4][T];
+  } u;
+};
+
+// Perturbation field -grad S[sigma] at m <= NMAX points held in LDS, by all
+// T threads of the block: S = T/m threads ("slices") per point, slice s
+// taking segments s, s+S, ... Closest segment first (first index wins
+// ties, reference bem/axisymmetric.py:341-389) to place exactly-on-surface
+// points 1e-4 L off the surface (554-557), then per segment either its far
+// Gauss nodes or its near sinh rule. Partial sums are combined in slice
+// order, so results are deterministic. m must be block-uniform.
+template <int T, int NMAX>
+__device__ void bem_pert_block(const BemView& B, int m, const double* px, const double* py, const double* pz,
+                               double* ox, double* oy, double* oz, BemShared<T, NMAX>& sh) {
+  if (m <= 0) return;
+  const int tid = threadIdx.x;
+  const int S = m >= T ? 1 : T / m;
+  const int p = tid / S, s = tid - p * S;
+  const bool on = p < m;
+  if (on) {
+    const double x = px[p], y = py[p], z = pz[p];
+    const double rho = hypot(x, y);
+    double bd2 = INFINITY;
+    int bseg = 0;
+    for (int j = s; j < B.n_seg; j += S) {
+      const double* g = B.segs + (size_t)j * SEG_STRIDE;
+      const double ax = g[S_ABR], ay = g[S_ABZ];
+      double t = ((rho - g[S_P0R]) * ax + (z - g[S_P0Z]) * ay) / g[S_ABAB];
+      t = fmin(fmax(t, 0.0), 1.0);
+      const double dr = rho - (g[S_P0R] + t * ax);
+      const double dz = z - (g[S_P0Z] + t * ay);
+      const double d2 = dr * dr + dz * dz;
+      if (d2 < bd2) {
+        bd2 = d2;
+        bseg = j;
+      }
+    }
+    sh.u.a.bd2[tid] = bd2;
+    sh.u.a.bseg[tid] = bseg;
+  }
+  __syncthreads();
+  if (on && s == 0) {
+    double bd2 = sh.u.a.bd2[tid];
+    int bseg = sh.u.a.bseg[tid];
+    for (int k = 1; k < S; ++k) {
+      const double d = sh.u.a.bd2[tid + k];
+      const int sg = sh.u.a.bseg[tid + k];
+      if (d < bd2 || (d == bd2 && sg < bseg)) {
+        bd2 = d;
+        bseg = sg;
+      }
+    }
+    const double x = px[p], y = py[p], z = pz[p];
+    const double rho = hypot(x, y);
+    const double* g = B.segs + (size_t)bseg * SEG_STRIDE;
+    const double L = g[S_L];
+    const double thr = 1e-9 * L;
+    const double off = bd2 < thr * thr ? 1e-4 * L : 0.0;
+    double re = rho + off * g[S_NR];
+    if (rho == 0.0) re = 0.0;
+    sh.re[p] = re;
+    sh.ze[p] = z + off * g[S_NZ];
+    sh.rho[p] = rho;
+    const double phi = atan2(y, x);
+    sh.cph[p] = cos(phi);
+    sh.sph[p] = sin(phi);
+  }
+  __syncthreads();
+  RingAcc acc;
+  acc.zero();
+  if (on) {
+    const double re = sh.re[p], ze = sh.ze[p], re2 = re * re;
+    for (int j = s; j < B.n_seg; j += S) {
+      const double* g = B.segs + (size_t)j * SEG_STRIDE;
+      const double L = g[S_L], ur = g[S_UR], uz = g[S_UZ];
+      const double dr = re - g[S_P0R], dz = ze - g[S_P0Z];
+      const double along = dr * ur + dz * uz;
+      const double proj = fmin(fmax(along, 0.0), L);
+      const double qr = dr - proj * ur, qz = dz - proj * uz;
+      const double dist = sqrt(qr * qr + qz * qz);
+      if (dist < L) {
+        near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, acc);
+      } else {
+        const double4* nd = B.nodes + (size_t)j * B.nq;
+#pragma unroll 1
+        for (int q = 0; q < B.nq; ++q) {
+          const double4 v = nd[q];
+          ring_accum(re, re2, ze - v.y, v.x, v.z, acc);
+        }
+      }
+    }
+  }
+  __syncthreads();  // bd2/bseg reads done before acc (aliased) is written
+  sh.u.acc[0][tid] = acc.er_main;
+  sh.u.acc[1][tid] = acc.ez_main;
+  sh.u.acc[2][tid] = acc.er_ax;
+  sh.u.acc[3][tid] = acc.ez_ax;
+  __syncthreads();
+  if (on && s == 0) {
+    RingAcc tot = {sh.u.acc[0][tid], sh.u.acc[1][tid], sh.u.acc[2][tid], sh.u.acc[3][tid]};
+    for (int k = 1; k < S; ++k) {
+      tot.er_main += sh.u.acc[0][tid + k];
+      tot.ez_main += sh.u.acc[1][tid + k];
+      tot.er_ax += sh.u.acc[2][tid + k];
+      tot.ez_ax += sh.u.acc[3][tid + k];
+    }
+    double er, ez;
+    ring_finish(tot, sh.re[p], er, ez);
+    const bool axis = sh.rho[p] == 0.0;
+    ox[p] = axis ? 0.0 : er * sh.cph[p];
+    oy[p] = axis ? 0.0 : er * sh.sph[p];
+    oz[p] = ez;
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------ geometry helpers
+// Smoothstep weight: 1 at z <= z_lo, 0 at z >= z_hi (reference
+// bem/image_charge.py:328-342, used by bem/geometry.py:44-55).
+__device__ __forceinline__ double blend_weight(double z, double z_lo, double z_hi) {
+  double t = (z_hi - z) / (z_hi - z_lo);
+  t = fmin(fmax(t, 0.0), 1.0);
+  return t * t * (3.0 - 2.0 * t);
+}
+
+// |p - nearest point of the axis segment [0, H]|^2: the capsule-interior
+// measure of the post (H = 0: squared distance from the origin, as the
+// reference hemisphere's sum(position**2)).
+__device__ __forceinline__ double post_r2(double x, double y, double z, double H) {
+  if (H == 0.0) return x * x + y * y + z * z;
+  const double zc = fmin(fmax(z, -H), H);
+  const double dz = z - zc;
+  return x * x + y * y + dz * dz;
+}
+
+// Does this point need the BEM perturbation? (valid region and w > 0)
+__device__ __forceinline__ bool bem_needed(const spt_geometry& G, double x, double y, double z) {
+  const double r = sqrt(post_r2(x, y, z, G.H));
+  const bool valid = (r >= G.R * (1.0 - 1e-6)) && (z >= 0.0);
+  return valid && blend_weight(z, 0.5 * G.z_handoff, G.z_handoff) > 0.0;
+}
+
+// Full external field at one point; (qx, qy, qz) is the BEM perturbation
+// when bem_needed() held, else unused.
+__device__ void geometry_field_point(const spt_geometry& G, double x, double y, double z, double qx, double qy,
+                                     double qz, double& Ex, double& Ey, double& Ez) {
+  if (G.kind == SPT_GEOM_FLAT) {  // GunField (reference fields.py:43-59)
+    Ex = 0.0;
+    Ey = 0.0;
+    Ez = z >= 0.0 ? G.E_gun : 0.0;
+    return;
+  }
+  if (G.kind == SPT_GEOM_TIP) {  // reference fields.py:119-160
+    const double R = G.R;
+    const double r = sqrt(x * x + y * y + z * z);
+    const bool valid = (r >= R * (1.0 - 1e-6)) && (z >= 0.0);
+    const double rs = valid ? r : R;
+    const double r3 = rs * rs * rs;
+    const double r5 = r3 * rs * rs;
+    const double R3 = R * R * R;
+    if (!valid) {
+      Ex = Ey = Ez = 0.0;
+      return;
+    }
+    Ex = 3.0 * G.E_gun * R3 * x * z / r5;
+    Ey = 3.0 * G.E_gun * R3 * y * z / r5;
+    Ez = G.E_gun * (1.0 - R3 / r3 + 3.0 * R3 * z * z / r5);
+    return;
+  }
+  // BEM post / hemisphere: reference bem/fields.py:200-211, bem/geometry.py:319-324
+  const double r = sqrt(post_r2(x, y, z, G.H));
+  const bool valid = (r >= G.R * (1.0 - 1e-6)) && (z >= 0.0);
+  const double w = blend_weight(z, 0.5 * G.z_handoff, G.z_handoff);
+  double nx = 0.0, ny = 0.0, nz = 0.0;
+  if (valid && w > 0.0) {
+    nx = 0.0 + qx;
+    ny = 0.0 + qy;
+    nz = G.E_gun + qz;
+  }
+  const double fz = z >= 0.0 ? G.E_gun : 0.0;
+  Ex = w * nx + (1.0 - w) * 0.0;
+  Ey = w * ny + (1.0 - w) * 0.0;
+  Ez = w * nz + (1.0 - w) * fz;
+}
+
+// Geometry.kill_mask, optionally OR z >= z_max (reference geometry.py:152-155,
+// 238-248,304-322; bem/geometry.py:429-436).
+__device__ __forceinline__ bool kill_point(const spt_geometry& G, double x, double y, double z) {
+  bool k = false;
+  if (G.kind != SPT_GEOM_FLAT) {
+    const double rk = G.R * (1.0 - 1e-6);
+    k = post_r2(x, y, z, G.kind == SPT_GEOM_BEM_POST ? G.H : 0.0) <= rk * rk;
+  }
+  if (G.has_kill_z) k = k || (z <= G.kill_z_below);
+  if (G.has_z_max) k = k || (z >= G.z_max);
+  return k;
+}
+
+// One Plummer-softened pair term (reference forces.py:16-29).
+__device__ __forceinline__ void pair_term(double xi, double yi, double zi, double qi, double xj, double yj, double zj,
+                                          double qj, double a2, double& fx, double& fy, double& fz) {
+  const double dx = xi - xj, dy = yi - yj, dz = zi - zj;
+  const double r2 = dx * dx + dy * dy + dz * dz + a2;
+  const double r3 = r2 * sqrt(r2);
+  const double c = kCoulomb * qi * qj / r3;
+  fx += c * dx;
+  fy += c * dy;
+  fz += c * dz;
+}
+
+// In-group Coulomb and image forces on particle i from LDS arrays
+// (reference forces.py:32-191). Inactive partners contribute nothing.
+__device__ void pair_forces_i(const spt_geometry& G, int i, int n, const double* sx, const double* sy,
+                              const double* sz, const double* sq, const int* act, double a2, double* Fc, double* Fi) {
+  Fc[0] = Fc[1] = Fc[2] = 0.0;
+  Fi[0] = Fi[1] = Fi[2] = 0.0;
+  if (!act[i]) return;
+  const double xi = sx[i], yi = sy[i], zi = sz[i], qi = sq[i];
+  for (int j = 0; j < n; ++j) {
+    if (!act[j]) continue;
+    pair_term(xi, yi, zi, qi, sx[j], sy[j], sz[j], sq[j], a2, Fc[0], Fc[1], Fc[2]);
+  }
+  if (G.image == SPT_IMAGE_PLANE) {
+    for (int j = 0; j < n; ++j) {
+      if (!act[j]) continue;
+      pair_term(xi, yi, zi, qi, sx[j], sy[j], -2.0 * G.z0 - sz[j], -sq[j], a2, Fi[0], Fi[1], Fi[2]);
+    }
+  } else if (G.image == SPT_IMAGE_TIP3) {
+    const double a = G.R - G.z0;
+    double f1[3] = {0, 0, 0}, f2[3] = {0, 0, 0}, f3[3] = {0, 0, 0};
+    for (int j = 0; j < n; ++j) {
+      if (!act[j]) continue;
+      const double xj = sx[j], yj = sy[j], zj = sz[j], qj = sq[j];
+      double d = sqrt(xj * xj + yj * yj + zj * zj);
+      d = d > 0.0 ? d : a;
+      const double sc = (a * a) / (d * d);
+      const double six = xj * sc, siy = yj * sc, siz = zj * sc;
+      const double siq = -qj * (a / d);
+      pair_term(xi, yi, zi, qi, six, siy, siz, siq, a2, f1[0], f1[1], f1[2]);
+      pair_term(xi, yi, zi, qi, xj, yj, -2.0 * G.z0 - zj, -qj, a2, f2[0], f2[1], f2[2]);
+      pair_term(xi, yi, zi, qi, six, siy, -2.0 * G.z0 - siz, -siq, a2, f3[0], f3[1], f3[2]);
+    }
+    for (int c = 0; c < 3; ++c) Fi[c] = f1[c] + f2[c] + f3[c];
+  }
+}
+
+// ----------------------------------------------------------- screens
+__device__ __forceinline__ double sgn(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : (x == 0.0 ? 0.0 : x)); }
+
+__device__ __forceinline__ void hbasis(double s, double& h00, double& h10, double& h01, double& h11) {
+  const double s2 = s * s, s3 = s2 * s;
+  h00 = 2 * s3 - 3 * s2 + 1;
+  h10 = s3 - 2 * s2 + s;
+  h01 = -2 * s3 + 3 * s2;
+  h11 = s3 - s2;
+}
+
+__device__ __forceinline__ double hermite_z(double p0, double v0, double p1, double v1, double dt, double s) {
+  double h00, h10, h01, h11;
+  hbasis(s, h00, h10, h01, h11);
+  return h00 * p0 + h10 * dt * v0 + h01 * p1 + h11 * dt * v1;
+}
+
+struct ScreenParams {
+  int n;
+  double z[SPT_MAX_SCREENS];
+};
+
+// Screen crossings of one particle over one accepted step: cubic Hermite
+// z(s), split at its real extrema, half-open (t_old, t_new] sign test on
+// moving particles, 60-step bisection (reference screens.py:43-168).
+__device__ __noinline__ void screens_particle(const ScreenParams& SP, spt_event* events, int64_t cap, unsigned long long* count,
+                                 int group, int particle, int64_t batch, double t0, double t1, const double* p0,
+                                 const double* v0, const double* p1, const double* v1) {
+  if (SP.n == 0) return;
+  const double dt = t1 - t0;
+  const bool moving = p0[0] != p1[0] || p0[1] != p1[1] || p0[2] != p1[2] || v0[0] != v1[0] || v0[1] != v1[1] ||
+                      v0[2] != v1[2];
+  if (!moving || !(dt > 0.0)) return;
+  const double z0 = p0[2], z1 = p1[2];
+  const double u0 = dt * v0[2], u1 = dt * v1[2];
+  double A = 3 * (2 * (z0 - z1) + u0 + u1);
+  double B = 2 * (3 * (z1 - z0) - 2 * u0 - u1);
+  double C = u0;
+  double sc = fmax(fmax(fabs(A), fabs(B)), fabs(C));
+  sc = sc > 0.0 ? sc : 1.0;
+  A = A / sc;
+  B = B / sc;
+  C = C / sc;
+  const double disc = B * B - 4 * A * C;
+  const double q = -.5 * (B + (B >= 0.0 ? 1.0 : -1.0) * sqrt(fmax(disc, 0.0)));
+  double r0 = q / (A != 0.0 ? A : 1.0);
+  const double r1 = C / (q != 0.0 ? q : 1.0);
+  const double lin = -C / (B != 0.0 ? B : 1.0);
+  if (A == 0.0) r0 = lin;
+  const bool ok0 = (disc >= 0.0) && (A != 0.0 || B != 0.0) && (r0 > 0.0) && (r0 < 1.0);
+  const bool ok1 = (disc >= 0.0) && (A != 0.0) && (q != 0.0) && (r1 > 0.0) && (r1 < 1.0);
+  const double ca = ok0 ? r0 : 1.0, cb = ok1 ? r1 : 1.0;
+  const double cuts[4] = {0.0, fmin(ca, cb), fmax(ca, cb), 1.0};
+  for (int si = 0; si < SP.n; ++si) {
+    const double zs = SP.z[si];
+    for (int iv = 0; iv < 3; ++iv) {
+      const double left = cuts[iv], right = cuts[iv + 1];
+      if (!(right > left)) continue;
+      const double f0 = hermite_z(z0, v0[2], z1, v1[2], dt, left) - zs;
+      const double f1 = hermite_z(z0, v0[2], z1, v1[2], dt, right) - zs;
+      if (!(f0 != 0.0) || !(sgn(f0) != sgn(f1))) continue;
+      double lo = left, hi = right, flo = f0;
+      for (int it = 0; it < 60; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        const double fm = hermite_z(z0, v0[2], z1, v1[2], dt, mid) - zs;
+        if (sgn(fm) == sgn(flo)) {
+          lo = mid;
+          flo = fm;
+        } else {
+          hi = mid;
+        }
+      }
+      const double ss = 0.5 * (lo + hi);
+      double h00, h10, h01, h11;
+      hbasis(ss, h00, h10, h01, h11);
+      const double s2 = ss * ss;
+      const double d00 = 6 * s2 - 6 * ss, d10 = 3 * s2 - 4 * ss + 1, d01 = -6 * s2 + 6 * ss, d11 = 3 * s2 - 2 * ss;
+      const unsigned long long k = atomicAdd(count, 1ULL);
+      if ((int64_t)k < cap) {
+        spt_event* e = events + k;
+        e->group = group;
+        e->particle = particle;
+        e->screen = si;
+        e->interval = iv;
+        e->batch = batch;
+        e->t = t0 + ss * dt;
+        for (int c = 0; c < 3; ++c) {
+          e->pos[c] = h00 * p0[c] + (h10 * dt) * v0[c] + h01 * p1[c] + (h11 * dt) * v1[c];
+          e->vel[c] = (d00 / dt) * p0[c] + d10 * v0[c] + (d01 / dt) * p1[c] + d11 * v1[c];
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ trace kernel
+struct TraceParams {
+  spt_geometry G;
+  BemView bem;
+  ScreenParams SP;
+  int n_groups;
+  int64_t n_total;
+  const int64_t* offsets;
+  const int32_t* order;
+  double* pos;
+  double* vel;
+  const double* t_birth;
+  const double* charge;
+  const double* mass;
+  double* t_death;
+  double* kbuf;
+  double q_over_m, a2, t_max, rtol, atol, safety, fmin_, fmax_, init_frac;
+  int64_t max_steps;
+  int64_t* attempts;
+  int64_t* accepted;
+  int64_t* active_stages;
+  double* margin;
+  int32_t* status;
+  spt_event* events;
+  int64_t ev_cap;
+  unsigned long long* ev_count;
+  int* queue;
+};
+
+template <int T, int NMAX>
+struct TraceShared {
+  double sx[NMAX], sy[NMAX], sz[NMAX], sq[NMAX];
+  double fx[NMAX], fy[NMAX], fz[NMAX];
+  double cx[NMAX], cy[NMAX], cz[NMAX];
+  double ox[NMAX], oy[NMAX], oz[NMAX];
+  int sact[NMAX];
+  int cidx[NMAX];
+  int need[NMAX];
+  double red[T];
+  int slot;
+  int m;
+  BemShared<T, NMAX> bem;
+};
+
+// External field of every active particle of the group at the stage
+// positions in LDS (sx, sy, sz) into (fx, fy, fz).
+template <int T, int NMAX>
+__device__ void stage_field(const TraceParams& P, int n, TraceShared<T, NMAX>& sh) {
+  const int tid = threadIdx.x;
+  if (P.G.kind != SPT_GEOM_BEM_POST) {
+    if (tid < n && sh.sact[tid])
+      geometry_field_point(P.G, sh.sx[tid], sh.sy[tid], sh.sz[tid], 0, 0, 0, sh.fx[tid], sh.fy[tid], sh.fz[tid]);
+    return;
+  }
+  if (tid < n) sh.need[tid] = sh.sact[tid] && bem_needed(P.G, sh.sx[tid], sh.sy[tid], sh.sz[tid]);
+  __syncthreads();
+  if (tid == 0) {
+    int m = 0;
+    for (int i = 0; i < n; ++i)
+      if (sh.need[i]) sh.cidx[m++] = i;
+    sh.m = m;
+  }
+  __syncthreads();
+  const int m = sh.m;
+  if (tid < m) {
+    const int i = sh.cidx[tid];
+    sh.cx[tid] = sh.sx[i];
+    sh.cy[tid] = sh.sy[i];
+    sh.cz[tid] = sh.sz[i];
+  }
+  __syncthreads();
+  bem_pert_block<T, NMAX>(P.bem, m, sh.cx, sh.cy, sh.cz, sh.ox, sh.oy, sh.oz, sh.bem);
+  if (tid < m) {
+    const int i = sh.cidx[tid];
+    geometry_field_point(P.G, sh.sx[i], sh.sy[i], sh.sz[i], sh.ox[tid], sh.oy[tid], sh.oz[tid], sh.fx[i], sh.fy[i],
+                         sh.fz[i]);
+  }
+  if (tid < n && sh.sact[tid] && !sh.need[tid])
+    geometry_field_point(P.G, sh.sx[tid], sh.sy[tid], sh.sz[tid], 0, 0, 0, sh.fx[tid], sh.fy[tid], sh.fz[tid]);
+  __syncthreads();
+}
+
+// Per-particle scratch slots (coalesced across a group's threads):
+// 0..41  k of stage st, component c (0-2 position, 3-5 velocity): st*6+c
+// 42..47 candidate end-of-step position and velocity
+#define KB(sl) P.kbuf[(size_t)(sl) * (size_t)P.n_total + (size_t)gi]
+constexpr int kSlots = 48;
+
+// Integrate one group from its earliest birth to t_max (reference
+// batched_rk45.py:149-323 for a single group; _attempt_step 326-401).
+// Particle state lives in global memory (caller's pos/vel plus the stage
+// scratch) so that only a handful of scalars are live across the field
+// gather.
+template <int T, int NMAX>
+__device__ void run_group(const TraceParams& P, int g, TraceShared<T, NMAX>& sh) {
+  const int tid = threadIdx.x;
+  const int64_t off = P.offsets[g];
+  const int n = (int)(P.offsets[g + 1] - off);
+  const bool mine = tid < n;
+  const int64_t gi = off + tid;
+  double tb = INFINITY, td = INFINITY;
+  if (mine) {
+    tb = P.t_birth[gi];
+    sh.sq[tid] = P.charge[gi];
+  }
+  double t = block_min<T>(mine ? tb : INFINITY, sh.red);
+  double h = (P.t_max - t) * P.init_frac;
+  bool running = true;
+  int64_t attempts = 0, nacc = 0, act_st = 0;
+  double margin = INFINITY;
+  int status = SPT_STATUS_DONE;
+  const double tol = 1e-9 * fmax(P.t_max, 1e-30);
+  const double ncomp = 6.0 * n;
+  while (running) {
+    if (attempts >= P.max_steps) {
+      status = SPT_STATUS_STEPCAP;
+      break;
+    }
+    // next birth checkpoint: smallest t_birth > t (reference 228-233)
+    const double n_reached = block_sum<T>((mine && tb <= t) ? 1.0 : 0.0, sh.red);
+    const double nxt_cp = block_min<T>((mine && tb > t) ? tb : INFINITY, sh.red);
+    const bool exhausted = n_reached >= (double)n;
+    const double next = exhausted ? P.t_max : nxt_cp;
+    const double target = fmin(next, P.t_max);
+    const double hr = fmin(h, target - t);
+    const bool active = mine && (t >= tb) && (t < td);
+    const double n_act = block_sum<T>(active ? 1.0 : 0.0, sh.red);
+#pragma unroll 1
+    for (int st = 0; st < 7; ++st) {
+      if (mine) {
+        double ps[3], vs[3];
+        if (st < 6) {
+          for (int c = 0; c < 3; ++c) {
+            ps[c] = P.pos[3 * gi + c];
+            vs[c] = P.vel[3 * gi + c];
+          }
+          for (int j = 0; j < st; ++j) {
+            const double ha = hr * c_A[st][j];
+            for (int c = 0; c < 3; ++c) {
+              ps[c] = ps[c] + ha * KB(j * 6 + c);
+              vs[c] = vs[c] + ha * KB(j * 6 + 3 + c);
+            }
+          }
+        } else {
+          for (int c = 0; c < 3; ++c) {
+            ps[c] = KB(42 + c);
+            vs[c] = KB(45 + c);
+          }
+        }
+        sh.sx[tid] = ps[0];
+        sh.sy[tid] = ps[1];
+        sh.sz[tid] = ps[2];
+        sh.sact[tid] = active ? 1 : 0;
+        for (int c = 0; c < 3; ++c) KB(st * 6 + c) = active ? vs[c] : 0.0;
+      }
+      __syncthreads();
+      stage_field<T, NMAX>(P, n, sh);
+      if (mine) {
+        double Fc[3], Fi[3];
+        pair_forces_i(P.G, tid, n, sh.sx, sh.sy, sh.sz, sh.sq, sh.sact, P.a2, Fc, Fi);
+        const double mass = P.mass[gi];
+        const double F[3] = {sh.fx[tid], sh.fy[tid], sh.fz[tid]};
+        for (int c = 0; c < 3; ++c) {
+          const double ac = n == 1 ? 0.0 : Fc[c] / mass;
+          const double a = P.q_over_m * F[c] + ac + Fi[c] / mass;
+          KB(st * 6 + 3 + c) = active ? a : 0.0;
+        }
+        if (st == 5) {
+          for (int c = 0; c < 3; ++c) {
+            double sp = c_B[0] * KB(c), sv = c_B[0] * KB(3 + c);
+            for (int k = 1; k < 6; ++k) {
+              sp = sp + c_B[k] * KB(k * 6 + c);
+              sv = sv + c_B[k] * KB(k * 6 + 3 + c);
+            }
+            KB(42 + c) = P.pos[3 * gi + c] + hr * sp;
+            KB(45 + c) = P.vel[3 * gi + c] + hr * sv;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    double lp = 0.0, lv = 0.0;
+    if (mine) {
+      for (int c = 0; c < 3; ++c) {
+        double ep = c_E[0] * KB(c), ev = c_E[0] * KB(3 + c);
+        for (int k = 1; k < 7; ++k) {
+          ep = ep + c_E[k] * KB(k * 6 + c);
+          ev = ev + c_E[k] * KB(k * 6 + 3 + c);
+        }
+        ep = hr * ep;
+        ev = hr * ev;
+        const double sp = P.atol + P.rtol * fmax(fabs(P.pos[3 * gi + c]), fabs(KB(42 + c)));
+        const double sv = kLight * P.atol + P.rtol * fmax(fabs(P.vel[3 * gi + c]), fabs(KB(45 + c)));
+        const double a = ep / sp, b = ev / sv;
+        lp += a * a;
+        lv += b * b;
+      }
+    }
+    const double sq = block_sum<T>(lp, sh.red) + block_sum<T>(lv, sh.red);
+    double err = sqrt(sq / ncomp);
+    if (err == 0.0) err = 2.2250738585072014e-308;
+    const bool acc = err <= 1.0;
+    double fac = P.safety * pow(err, -1.0 / 5);
+    fac = fmin(fmax(fac, P.fmin_), P.fmax_);
+    const int64_t batch = attempts;
+    attempts += 1;
+    margin = fmin(margin, fabs(err - 1.0));
+    act_st += 7 * (int64_t)n_act;
+    h = hr * fac;
+    if (acc) {
+      nacc += 1;
+      const double t_old = t;
+      t = t + hr;
+      if (!exhausted && t >= next - tol) h = (P.t_max - t) * P.init_frac;
+      if (mine) {
+        double x0[3], v0[3], x1[3], v1[3];
+        for (int c = 0; c < 3; ++c) {
+          x0[c] = P.pos[3 * gi + c];
+          v0[c] = P.vel[3 * gi + c];
+          x1[c] = KB(42 + c);
+          v1[c] = KB(45 + c);
+          P.pos[3 * gi + c] = x1[c];
+          P.vel[3 * gi + c] = v1[c];
+        }
+        if (active && kill_point(P.G, x1[0], x1[1], x1[2])) td = t;
+        screens_particle(P.SP, P.events, P.ev_cap, P.ev_count, g, tid, batch, t_old, t, x0, v0, x1, v1);
+      }
+      running = t < P.t_max * (1 - 1e-14);
+    }
+  }
+  if (mine) P.t_death[gi] = td;
+  if (tid == 0) {
+    P.attempts[g] = attempts;
+    P.accepted[g] = nacc;
+    P.active_stages[g] = act_st;
+    P.margin[g] = margin;
+    P.status[g] = status;
+  }
+  __syncthreads();
+}
+
+template <int T, int NMAX>
+__global__ __launch_bounds__(T) void trace_kernel(TraceParams P) {
+  __shared__ TraceShared<T, NMAX> sh;
+  while (true) {
+    if (threadIdx.x == 0) sh.slot = atomicAdd(P.queue, 1);
+    __syncthreads();
+    const int q = sh.slot;
+    __syncthreads();
+    if (q >= P.n_groups) break;
+    const int g = P.order ? P.order[q] : q;
+    run_group<T, NMAX>(P, g, sh);
+  }
+}
+
+__global__ void reset_int_kernel(int* p, int v) { *p = v; }
+__global__ void reset_u64_kernel(unsigned long long* p) { *p = 0ULL; }
+
+// ----------------------------------------------- standalone field kernels
+template <int T, int NMAX>
+struct FieldShared {
+  double px[NMAX], py[NMAX], pz[NMAX];
+  double ox[NMAX], oy[NMAX], oz[NMAX];
+  int cidx[NMAX];
+  int need[NMAX];
+  int m;
+  BemShared<T, NMAX> bem;
+};
+
+// evaluate_axisymmetric_field over PB points per block.
+template <int T, int NMAX, int PB>
+__global__ __launch_bounds__(T) void bem_pert_kernel(BemView B, const double* pos, int64_t n, double* E) {
+  __shared__ FieldShared<T, NMAX> sh;
+  const int64_t base = (int64_t)blockIdx.x * PB;
+  const int m = (int)min((int64_t)PB, n - base);
+  const int tid = threadIdx.x;
+  if (tid < m) {
+    sh.px[tid] = pos[3 * (base + tid)];
+    sh.py[tid] = pos[3 * (base + tid) + 1];
+    sh.pz[tid] = pos[3 * (base + tid) + 2];
+  }
+  __syncthreads();
+  bem_pert_block<T, NMAX>(B, m, sh.px, sh.py, sh.pz, sh.ox, sh.oy, sh.oz, sh.bem);
+  if (tid < m) {
+    E[3 * (base + tid)] = sh.ox[tid];
+    E[3 * (base + tid) + 1] = sh.oy[tid];
+    E[3 * (base + tid) + 2] = sh.oz[tid];
+  }
+}
+
+// Geometry.field over PB points per block (BEM part compacted to the
+// points that need it).
+template <int T, int NMAX, int PB>
+__global__ __launch_bounds__(T) void geometry_field_kernel(spt_geometry G, BemView B, const double* pos, int64_t n,
+                                                           double* E) {
+  __shared__ FieldShared<T, NMAX> sh;
+  __shared__ double sx[PB], sy[PB], sz[PB];
+  const int64_t base = (int64_t)blockIdx.x * PB;
+  const int m_all = (int)min((int64_t)PB, n - base);
+  const int tid = threadIdx.x;
+  if (tid < m_all) {
+    sx[tid] = pos[3 * (base + tid)];
+    sy[tid] = pos[3 * (base + tid) + 1];
+    sz[tid] = pos[3 * (base + tid) + 2];
+    sh.need[tid] = G.kind == SPT_GEOM_BEM_POST && bem_needed(G, sx[tid], sy[tid], sz[tid]);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int m = 0;
+    for (int i = 0; i < m_all; ++i)
+      if (sh.need[i]) sh.cidx[m++] = i;
+    sh.m = m;
+  }
+  __syncthreads();
+  const int m = sh.m;
+  if (tid < m) {
+    const int i = sh.cidx[tid];
+    sh.px[tid] = sx[i];
+    sh.py[tid] = sy[i];
+    sh.pz[tid] = sz[i];
+  }
+  __syncthreads();
+  bem_pert_block<T, NMAX>(B, m, sh.px, sh.py, sh.pz, sh.ox, sh.oy, sh.oz, sh.bem);
+  if (tid < m) {
+    const int i = sh.cidx[tid];
+    double ex, ey, ez;
+    geometry_field_point(G, sx[i], sy[i], sz[i], sh.ox[tid], sh.oy[tid], sh.oz[tid], ex, ey, ez);
+    E[3 * (base + i)] = ex;
+    E[3 * (base + i) + 1] = ey;
+    E[3 * (base + i) + 2] = ez;
+  }
+  if (tid < m_all && !sh.need[tid]) {
+    double ex, ey, ez;
+    geometry_field_point(G, sx[tid], sy[tid], sz[tid], 0, 0, 0, ex, ey, ez);
+    E[3 * (base + tid)] = ex;
+    E[3 * (base + tid) + 1] = ey;
+    E[3 * (base + tid) + 2] = ez;
+  }
+}
+
+// In-group pair forces: one block per group, threads over its particles.
+template <int NMAX>
+__global__ __launch_bounds__(NMAX) void pair_forces_kernel(spt_geometry G, const double* pos, const double* charge,
+                                                           const uint8_t* active, const int64_t* offsets, double a2,
+                                                           double* Fc_out, double* Fi_out) {
+  __shared__ double sx[NMAX], sy[NMAX], sz[NMAX], sq[NMAX];
+  __shared__ int act[NMAX];
+  const int g = blockIdx.x;
+  const int64_t off = offsets[g];
+  const int n = (int)(offsets[g + 1] - off);
+  const int i = threadIdx.x;
+  if (i < n) {
+    sx[i] = pos[3 * (off + i)];
+    sy[i] = pos[3 * (off + i) + 1];
+    sz[i] = pos[3 * (off + i) + 2];
+    sq[i] = charge[off + i];
+    act[i] = active[off + i] ? 1 : 0;
+  }
+  __syncthreads();
+  if (i < n) {
+    double Fc[3], Fi[3];
+    pair_forces_i(G, i, n, sx, sy, sz, sq, act, a2, Fc, Fi);
+    for (int c = 0; c < 3; ++c) {
+      if (Fc_out) Fc_out[3 * (off + i) + c] = Fc[c];
+      if (Fi_out) Fi_out[3 * (off + i) + c] = Fi[c];
+    }
+  }
+}
+
+__global__ void kill_kernel(spt_geometry G, const double* pos, int64_t n, uint8_t* out) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  out[k] = kill_point(G, pos[3 * k], pos[3 * k + 1], pos[3 * k + 2]) ? 1 : 0;
+}
+
+// ------------------------------------------------------- BEM assembly
+// Entry (i, j) of the hat-function collocation operator, split as
+// lo[i, j] (hat of node j over segment j) and hi[i, j] (hat of node j+1
+// over segment j): adaptive bisection while length > refine_ratio *
+// distance(node i, midpoint), depth <= max_depth, Gauss-4 leaves
+// (reference bem/axisymmetric.py:275-302). Depth-first with an explicit
+// stack; leaf order matches the reference's left-then-right recursion.
+__global__ void assemble_kernel(const double* prof, int n, double refine_ratio, int max_depth, double* lo_out,
+                                double* hi_out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = (int64_t)n * (n - 1);
+  if (k >= total) return;
+  const int i = (int)(k / (n - 1)), j = (int)(k - (int64_t)i * (n - 1));
+  const double fr = prof[2 * i], fz = prof[2 * i + 1];
+  double st[48][6];
+  int sd[48];
+  int top = 0;
+  st[0][0] = prof[2 * j];
+  st[0][1] = prof[2 * j + 1];
+  st[0][2] = prof[2 * j + 2];
+  st[0][3] = prof[2 * j + 3];
+  st[0][4] = 0.0;
+  st[0][5] = 1.0;
+  sd[0] = 0;
+  top = 1;
+  double lo = 0.0, hi = 0.0;
+  while (top > 0) {
+    --top;
+    const double a0r = st[top][0], a0z = st[top][1], a1r = st[top][2], a1z = st[top][3];
+    const double u0 = st[top][4], u1 = st[top][5];
+    const int depth = sd[top];
+    const double mr = 0.5 * (a0r + a1r), mz = 0.5 * (a0z + a1z);
+    const double dr = a1r - a0r, dz = a1z - a0z;
+    const double length = sqrt(dr * dr + dz * dz);
+    const double er = fr - mr, ez = fz - mz;
+    const double dist = sqrt(er * er + ez * ez);
+    if (depth < max_depth && length > refine_ratio * dist && top + 2 <= 48) {
+      const double um = 0.5 * (u0 + u1);
+      // push right then left so the left half is integrated first
+      st[top][0] = mr; st[top][1] = mz; st[top][2] = a1r; st[top][3] = a1z; st[top][4] = um; st[top][5] = u1;
+      sd[top] = depth + 1;
+      ++top;
+      st[top][0] = a0r; st[top][1] = a0z; st[top][2] = mr; st[top][3] = mz; st[top][4] = u0; st[top][5] = um;
+      sd[top] = depth + 1;
+      ++top;
+      continue;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double t = 0.5 * (c_gx4[q] + 1.0);
+      const double rs = a0r + t * (a1r - a0r);
+      const double zs = a0z + t * (a1z - a0z);
+      const double hs = u0 + t * (u1 - u0);
+      const double ls = (1.0 - u0) + t * ((1.0 - u1) - (1.0 - u0));
+      const double gq = 2.0 * kPi * rs * ring_potential(fr, fz - zs, rs) * (length * 0.5 * c_gw4[q]);
+      lo += ls * gq;
+      hi += hs * gq;
+    }
+  }
+  lo_out[k] = lo;
+  hi_out[k] = hi;
+}
+
+// A[i, j] = hi[i, j-1] + lo[i, j], column-major for LAPACK-style solvers,
+// and the Dirichlet right-hand side E_gun * z_i.
+__global__ void assemble_combine_kernel(const double* prof, int n, const double* lo, const double* hi, double E_gun,
+                                        double* A_cm, double* b) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (int64_t)n * n) return;
+  const int j = (int)(k / n), i = (int)(k - (int64_t)j * n);
+  double a = 0.0;
+  if (j > 0) a = a + hi[(int64_t)i * (n - 1) + (j - 1)];
+  if (j < n - 1) a = a + lo[(int64_t)i * (n - 1) + j];
+  A_cm[k] = a;
+  if (j == 0 && b) b[i] = E_gun * prof[2 * i + 1];
+}
+
+// ================================================================ host side
+struct spt_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int n_cu = 0;
+  std::string err;
+  // BEM tables
+  int bem_nodes = 0, bem_sub = 0;
+  double* d_prof = nullptr;
+  double* d_sigma = nullptr;
+  double* d_segs = nullptr;
+  double4* d_nodes = nullptr;
+  // scratch
+  double* d_kbuf = nullptr;
+  size_t kbuf_elems = 0;
+  int* d_queue = nullptr;
+  rocblas_handle blas = nullptr;
+};
+
+static int fail(spt_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail(ctx, SPT_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+static BemView bem_view(const spt_ctx* c) {
+  BemView B;
+  B.n_seg = c->bem_nodes - 1;
+  B.nq = c->bem_sub * 4;
+  B.nodes = c->d_nodes;
+  B.segs = c->d_segs;
+  B.sigma = c->d_sigma;
+  return B;
+}
+
+static void free_bem(spt_ctx* c) {
+  if (c->d_prof) (void)hipFree(c->d_prof);
+  if (c->d_sigma) (void)hipFree(c->d_sigma);
+  if (c->d_segs) (void)hipFree(c->d_segs);
+  if (c->d_nodes) (void)hipFree(c->d_nodes);
+  c->d_prof = c->d_sigma = c->d_segs = nullptr;
+  c->d_nodes = nullptr;
+  c->bem_nodes = 0;
+}
+
+extern "C" {
+
+int spt_create(int device, spt_ctx** out) {
+  if (!out) return SPT_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return SPT_EINVAL;
+  spt_ctx* c = new spt_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete c;
+    return SPT_EHIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    delete c;
+    return SPT_EHIP;
+  }
+  c->n_cu = prop.multiProcessorCount;
+  if (hipMalloc(&c->d_queue, 64) != hipSuccess) {
+    delete c;
+    return SPT_ENOMEM;
+  }
+  *out = c;
+  return SPT_OK;
+}
+
+void spt_destroy(spt_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  free_bem(c);
+  if (c->d_kbuf) (void)hipFree(c->d_kbuf);
+  if (c->d_queue) (void)hipFree(c->d_queue);
+  if (c->blas) rocblas_destroy_handle(c->blas);
+  delete c;
+}
+
+const char* spt_last_error(const spt_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int spt_set_stream(spt_ctx* c, void* s) {
+  if (!c) return SPT_EINVAL;
+  c->stream = (hipStream_t)s;
+  if (c->blas) rocblas_set_stream(c->blas, c->stream);
+  return SPT_OK;
+}
+
+int spt_sync(spt_ctx* c) {
+  if (!c) return SPT_EINVAL;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SPT_OK;
+}
+
+int spt_device_cus(spt_ctx* c, int* n) {
+  if (!c || !n) return SPT_EINVAL;
+  *n = c->n_cu;
+  return SPT_OK;
+}
+
+int spt_bem_upload(spt_ctx* c, const double* prof, int32_t n_nodes, const double* sigma, int32_t n_sub) {
+  if (!c) return SPT_EINVAL;
+  if (!prof || !sigma || n_nodes < 2 || n_sub < 1 || n_sub > 64)
+    return fail(c, SPT_EINVAL, "spt_bem_upload: need n_nodes >= 2 and 1 <= n_subdiv <= 64");
+  HIPCHK(c, hipSetDevice(c->device));
+  free_bem(c);
+  const int n_seg = n_nodes - 1;
+  const size_t nq = (size_t)n_seg * n_sub * 4;
+  HIPCHK(c, hipMalloc(&c->d_prof, sizeof(double) * 2 * n_nodes));
+  HIPCHK(c, hipMalloc(&c->d_sigma, sizeof(double) * n_nodes));
+  HIPCHK(c, hipMalloc(&c->d_segs, sizeof(double) * SEG_STRIDE * n_seg));
+  HIPCHK(c, hipMalloc(&c->d_nodes, sizeof(double4) * nq));
+  HIPCHK(c, hipMemcpyAsync(c->d_prof, prof, sizeof(double) * 2 * n_nodes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_sigma, sigma, sizeof(double) * n_nodes, hipMemcpyHostToDevice, c->stream));
+  seg_table_kernel<<<(n_seg + 255) / 256, 256, 0, c->stream>>>(c->d_prof, n_seg, c->d_segs);
+  node_table_kernel<<<(unsigned)((nq + 255) / 256), 256, 0, c->stream>>>(c->d_prof, c->d_sigma, n_seg, n_sub,
+                                                                          c->d_nodes);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->bem_nodes = n_nodes;
+  c->bem_sub = n_sub;
+  return SPT_OK;
+}
+
+static int assemble_device(spt_ctx* c, const double* prof, int n, double rr, int md, double E_gun, double** A_cm,
+                           double** b) {
+  if (n < 2 || n > 20000) return fail(c, SPT_EINVAL, "assemble: need 2 <= n_nodes <= 20000");
+  if (md < 0 || md > 40) return fail(c, SPT_EINVAL, "assemble: need 0 <= max_depth <= 40");
+  double *d_prof = nullptr, *d_lo = nullptr, *d_hi = nullptr;
+  const int64_t tot = (int64_t)n * (n - 1);
+  HIPCHK(c, hipMalloc(&d_prof, sizeof(double) * 2 * n));
+  HIPCHK(c, hipMalloc(&d_lo, sizeof(double) * tot));
+  HIPCHK(c, hipMalloc(&d_hi, sizeof(double) * tot));
+  HIPCHK(c, hipMalloc(A_cm, sizeof(double) * (size_t)n * n));
+  HIPCHK(c, hipMalloc(b, sizeof(double) * n));
+  HIPCHK(c, hipMemcpyAsync(d_prof, prof, sizeof(double) * 2 * n, hipMemcpyHostToDevice, c->stream));
+  assemble_kernel<<<(unsigned)((tot + 127) / 128), 128, 0, c->stream>>>(d_prof, n, rr, md, d_lo, d_hi);
+  assemble_combine_kernel<<<(unsigned)(((int64_t)n * n + 255) / 256), 256, 0, c->stream>>>(d_prof, n, d_lo, d_hi,
+                                                                                           E_gun, *A_cm, *b);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  (void)hipFree(d_prof);
+  (void)hipFree(d_lo);
+  (void)hipFree(d_hi);
+  return SPT_OK;
+}
+
+int spt_bem_assemble(spt_ctx* c, const double* prof, int32_t n, double rr, int32_t md, double* A_out) {
+  if (!c || !prof || !A_out) return SPT_EINVAL;
+  HIPCHK(c, hipSetDevice(c->device));
+  double *A = nullptr, *b = nullptr;
+  int rc = assemble_device(c, prof, n, rr, md, 0.0, &A, &b);
+  if (rc) return rc;
+  // column-major on device -> row-major host
+  double* tmp = new double[(size_t)n * n];
+  hipError_t e = hipMemcpy(tmp, A, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) A_out[(size_t)i * n + j] = tmp[(size_t)j * n + i];
+  delete[] tmp;
+  (void)hipFree(A);
+  (void)hipFree(b);
+  if (e != hipSuccess) return fail(c, SPT_EHIP, hipGetErrorString(e));
+  return SPT_OK;
+}
+
+int spt_bem_solve(spt_ctx* c, const double* prof, int32_t n, double E_gun, double rr, int32_t md, double* sigma_out) {
+  if (!c || !prof || !sigma_out) return SPT_EINVAL;
+  HIPCHK(c, hipSetDevice(c->device));
+  double *A = nullptr, *b = nullptr;
+  int rc = assemble_device(c, prof, n, rr, md, E_gun, &A, &b);
+  if (rc) return rc;
+  if (!c->blas) {
+    if (rocblas_create_handle(&c->blas) != rocblas_status_success) return fail(c, SPT_EHIP, "rocblas_create_handle");
+    rocblas_set_stream(c->blas, c->stream);
+  }
+  rocblas_int *ipiv = nullptr, *info = nullptr;
+  HIPCHK(c, hipMalloc(&ipiv, sizeof(rocblas_int) * n));
+  HIPCHK(c, hipMalloc(&info, sizeof(rocblas_int)));
+  rocblas_status s1 = rocsolver_dgetrf(c->blas, n, n, A, n, ipiv, info);
+  rocblas_status s2 = rocsolver_dgetrs(c->blas, rocblas_operation_none, n, 1, A, n, ipiv, b, n);
+  rocblas_int h_info = 0;
+  HIPCHK(c, hipMemcpyAsync(&h_info, info, sizeof(rocblas_int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(sigma_out, b, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  (void)hipFree(ipiv);
+  (void)hipFree(info);
+  (void)hipFree(A);
+  (void)hipFree(b);
+  if (s1 != rocblas_status_success || s2 != rocblas_status_success) return fail(c, SPT_EHIP, "rocsolver LU failed");
+  if (h_info != 0) return fail(c, SPT_EINVAL, "collocation matrix is singular");
+  return SPT_OK;
+}
+
+#define FIELD_T 256
+#define FIELD_PB 16
+
+int spt_bem_pert_field(spt_ctx* c, const double* pos, int64_t n, double* E) {
+  if (!c) return SPT_EINVAL;
+  if (!c->bem_nodes) return fail(c, SPT_ENOBEM, "no BEM table uploaded");
+  if (n < 0 || (n > 0 && (!pos || !E))) return fail(c, SPT_EINVAL, "spt_bem_pert_field: bad arguments");
+  if (n == 0) return SPT_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  const unsigned nb = (unsigned)((n + FIELD_PB - 1) / FIELD_PB);
+  bem_pert_kernel<FIELD_T, FIELD_PB, FIELD_PB><<<nb, FIELD_T, 0, c->stream>>>(bem_view(c), pos, n, E);
+  HIPCHK(c, hipGetLastError());
+  return SPT_OK;
+}
+
+static int check_geom(spt_ctx* c, const spt_geometry* g) {
+  if (!g) return fail(c, SPT_EINVAL, "null geometry");
+  if (g->kind < SPT_GEOM_FLAT || g->kind > SPT_GEOM_BEM_POST) return fail(c, SPT_EINVAL, "unknown geometry kind");
+  if (g->image < SPT_IMAGE_NONE || g->image > SPT_IMAGE_TIP3) return fail(c, SPT_EINVAL, "unknown image model");
+  if (g->kind == SPT_GEOM_BEM_POST) {
+    if (!c->bem_nodes) return fail(c, SPT_ENOBEM, "BEM geometry without an uploaded table");
+    if (!(g->z_handoff > 0.0)) return fail(c, SPT_EINVAL, "z_handoff must be positive");
+  }
+  if (g->kind != SPT_GEOM_FLAT && !(g->R > 0.0)) return fail(c, SPT_EINVAL, "R must be positive");
+  return SPT_OK;
+}
+
+int spt_geometry_field(spt_ctx* c, const spt_geometry* g, const double* pos, int64_t n, double* E) {
+  if (!c) return SPT_EINVAL;
+  int rc = check_geom(c, g);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && (!pos || !E))) return fail(c, SPT_EINVAL, "spt_geometry_field: bad arguments");
+  if (n == 0) return SPT_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  const unsigned nb = (unsigned)((n + FIELD_PB - 1) / FIELD_PB);
+  geometry_field_kernel<FIELD_T, FIELD_PB, FIELD_PB><<<nb, FIELD_T, 0, c->stream>>>(*g, bem_view(c), pos, n, E);
+  HIPCHK(c, hipGetLastError());
+  return SPT_OK;
+}
+
+int spt_pair_forces(spt_ctx* c, const spt_geometry* g, const double* pos, const double* charge, const uint8_t* active,
+                    const int64_t* offsets, int32_t n_groups, double a, double* Fc, double* Fi) {
+  if (!c) return SPT_EINVAL;
+  int rc = check_geom(c, g);
+  if (rc) return rc;
+  if (n_groups < 0 || (n_groups > 0 && (!pos || !charge || !active || !offsets)))
+    return fail(c, SPT_EINVAL, "spt_pair_forces: bad arguments");
+  if (n_groups == 0) return SPT_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  // group sizes are checked against the block size on the host side of
+  // the caller's offsets: read them back (small) to validate.
+  int64_t* h = new int64_t[n_groups + 1];
+  hipError_t e = hipMemcpy(h, offsets, sizeof(int64_t) * (n_groups + 1), hipMemcpyDeviceToHost);
+  int64_t mx = 0;
+  bool bad = false;
+  for (int i = 0; e == hipSuccess && i < n_groups; ++i) {
+    const int64_t s = h[i + 1] - h[i];
+    if (s < 0) bad = true;
+    mx = s > mx ? s : mx;
+  }
+  delete[] h;
+  if (e != hipSuccess) return fail(c, SPT_EHIP, hipGetErrorString(e));
+  if (bad || mx > 1024) return fail(c, SPT_EINVAL, "group sizes must be in [0, 1024]");
+  pair_forces_kernel<1024><<<n_groups, 1024, 0, c->stream>>>(*g, pos, charge, active, offsets, a * a, Fc, Fi);
+  HIPCHK(c, hipGetLastError());
+  return SPT_OK;
+}
+
+int spt_kill_mask(spt_ctx* c, const spt_geometry* g, const double* pos, int64_t n, uint8_t* out) {
+  if (!c) return SPT_EINVAL;
+  int rc = check_geom(c, g);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && (!pos || !out))) return fail(c, SPT_EINVAL, "spt_kill_mask: bad arguments");
+  if (n == 0) return SPT_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  kill_kernel<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(*g, pos, n, out);
+  HIPCHK(c, hipGetLastError());
+  return SPT_OK;
+}
+
+}  // extern "C"
+
+template <int T, int NMAX>
+static int launch_trace(spt_ctx* c, const TraceParams& P, int n_groups) {
+  int per_cu = 0;
+  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_kernel<T, NMAX>, T, 0));
+  if (per_cu < 1) per_cu = 1;
+  int64_t grid = (int64_t)per_cu * c->n_cu;
+  if (grid > n_groups) grid = n_groups;
+  trace_kernel<T, NMAX><<<(unsigned)grid, T, 0, c->stream>>>(P);
+  HIPCHK(c, hipGetLastError());
+  return SPT_OK;
+}
+
+extern "C" {
+
+int spt_trace(spt_ctx* c, const spt_trace_args* a) {
+  if (!c || !a) return SPT_EINVAL;
+  int rc = check_geom(c, &a->geom);
+  if (rc) return rc;
+  if (a->n_groups < 0 || a->n_particles < 0) return fail(c, SPT_EINVAL, "negative sizes");
+  if (a->n_screens < 0 || a->n_screens > SPT_MAX_SCREENS) return fail(c, SPT_EINVAL, "too many screens");
+  if (a->n_groups == 0) return SPT_OK;
+  if (!a->group_offsets || !a->pos || !a->vel || !a->t_birth || !a->charge || !a->mass || !a->t_death ||
+      !a->attempts || !a->accepted || !a->active_stages || !a->margin || !a->status || !a->event_count ||
+      (a->event_capacity > 0 && !a->events))
+    return fail(c, SPT_EINVAL, "spt_trace: null pointer");
+  if (!(a->rtol >= 0.0) || !(a->atol >= 0.0) || !(a->max_steps > 0)) return fail(c, SPT_EINVAL, "bad tolerances");
+  HIPCHK(c, hipSetDevice(c->device));
+  // validate the CSR offsets and find the largest group
+  int64_t* h = new int64_t[a->n_groups + 1];
+  hipError_t e = hipMemcpy(h, a->group_offsets, sizeof(int64_t) * (a->n_groups + 1), hipMemcpyDeviceToHost);
+  int64_t mx = 0;
+  bool bad = e != hipSuccess || h[0] != 0 || h[a->n_groups] != a->n_particles;
+  for (int i = 0; !bad && i < a->n_groups; ++i) {
+    const int64_t s = h[i + 1] - h[i];
+    if (s < 1) bad = true;
+    mx = s > mx ? s : mx;
+  }
+  delete[] h;
+  if (e != hipSuccess) return fail(c, SPT_EHIP, hipGetErrorString(e));
+  if (bad) return fail(c, SPT_EINVAL, "group_offsets must start at 0, end at n_particles, with sizes >= 1");
+  int T = a->block_threads ? a->block_threads : 256;
+  if (T != 64 && T != 128 && T != 256 && T != 512 && T != 1024)
+    return fail(c, SPT_EINVAL, "block_threads must be 64, 128, 256, 512 or 1024");
+  const int nmax_of_T = T <= 256 ? T : (T == 512 ? 256 : 64);
+  if (mx > nmax_of_T) {
+    if (a->block_threads) return fail(c, SPT_EINVAL, "largest group exceeds what block_threads supports");
+    T = mx <= 64 ? 64 : (mx <= 128 ? 128 : 256);
+    if (mx > 256) return fail(c, SPT_EINVAL, "groups larger than 256 particles are not supported");
+  }
+  const size_t need = (size_t)kSlots * (size_t)a->n_particles;
+  if (need > c->kbuf_elems) {
+    if (c->d_kbuf) (void)hipFree(c->d_kbuf);
+    c->d_kbuf = nullptr;
+    c->kbuf_elems = 0;
+    HIPCHK(c, hipMalloc(&c->d_kbuf, sizeof(double) * need));
+    c->kbuf_elems = need;
+  }
+  TraceParams P;
+  memset(&P, 0, sizeof(P));
+  P.G = a->geom;
+  P.bem = bem_view(c);
+  P.SP.n = a->n_screens;
+  for (int i = 0; i < a->n_screens; ++i) P.SP.z[i] = a->screens[i];
+  P.n_groups = a->n_groups;
+  P.n_total = a->n_particles;
+  P.offsets = a->group_offsets;
+  P.order = a->group_order;
+  P.pos = a->pos;
+  P.vel = a->vel;
+  P.t_birth = a->t_birth;
+  P.charge = a->charge;
+  P.mass = a->mass;
+  P.t_death = a->t_death;
+  P.kbuf = c->d_kbuf;
+  P.q_over_m = a->q_over_m;
+  P.a2 = a->plummer_radius * a->plummer_radius;
+  P.t_max = a->t_max;
+  P.rtol = a->rtol;
+  P.atol = a->atol;
+  P.safety = a->safety;
+  P.fmin_ = a->min_factor;
+  P.fmax_ = a->max_factor;
+  P.init_frac = a->initial_step_fraction;
+  P.max_steps = a->max_steps;
+  P.attempts = a->attempts;
+  P.accepted = a->accepted;
+  P.active_stages = a->active_stages;
+  P.margin = a->margin;
+  P.status = a->status;
+  P.events = a->events;
+  P.ev_cap = a->event_capacity;
+  P.ev_count = (unsigned long long*)a->event_count;
+  P.queue = c->d_queue;
+  reset_int_kernel<<<1, 1, 0, c->stream>>>(c->d_queue, 0);
+  reset_u64_kernel<<<1, 1, 0, c->stream>>>(P.ev_count);
+  switch (T) {
+    case 64: return launch_trace<64, 64>(c, P, a->n_groups);
+    case 128: return launch_trace<128, 128>(c, P, a->n_groups);
+    case 256: return launch_trace<256, 256>(c, P, a->n_groups);
+    case 512: return launch_trace<512, 256>(c, P, a->n_groups);
+    default: return launch_trace<1024, 64>(c, P, a->n_groups);
+  }
+}
+
+}  // extern "C"

@@ -18,29 +18,48 @@ constexpr double kInv4Pi2 = 1.0 / (4.0 * kPi * kPi);
 constexpr double kInv2Pi2 = 1.0 / (2.0 * kPi * kPi);
 constexpr double kInvPi = 1.0 / kPi;
 
+// Coefficient tables in constant memory: uniform across a wavefront, so
+// they sit in scalar registers. Only K's pair is needed on device (E comes
+// from K and dK/dp below), which keeps the table within the scalar file.
 __constant__ double c_PK[11] = SPT_ELLIP_PK;
 __constant__ double c_QK[11] = SPT_ELLIP_QK;
-__constant__ double c_PE[11] = SPT_ELLIP_PE;
-__constant__ double c_QE[11] = SPT_ELLIP_QE;
 
-__device__ __forceinline__ double horner11(const double* c, double x) {
-  double s = c[10];
+// Horner evaluation of a degree-10 polynomial and its derivative.
+__device__ __forceinline__ void horner11_d(const double* c, double x, double& s, double& d) {
+  s = c[10];
+  d = 0.0;
 #pragma unroll
-  for (int i = 9; i >= 0; --i) s = fma(s, x, c[i]);
-  return s;
+  for (int i = 9; i >= 0; --i) {
+    d = fma(d, x, s);
+    s = fma(s, x, c[i]);
+  }
 }
 
-// K(1-p), E(1-p) for p in [0, 1]: degree-10 P/Q pairs in the logarithmic
-// form of DLMF 19.12.1-2 (coefficients fitted by gen_ellip_coeffs.py).
+// K(1-p), E(1-p) for p in [0, 1].
+//   K(1-p) = PK(p) - ln(p) QK(p)  (logarithmic form of DLMF 19.12.1,
+//   coefficients fitted by gen_ellip_coeffs.py, ~1.2 ulp);
+//   E from Legendre's derivative relation dK/dm = (E - (1-m)K)/(2m(1-m))
+//   (DLMF 19.4.1) with m = 1 - p:  E = p K - 2 (1-p) p dK/dp,
+//   p dK/dp = p PK'(p) - QK(p) - p ln(p) QK'(p)  (~1.6 ulp).
 __device__ __forceinline__ void ellip_ke_c(double p, double& K, double& E) {
   const double lp = log(p);
-  K = fma(-lp, horner11(c_QK, p), horner11(c_PK, p));
-  E = fma(-lp * p, horner11(c_QE, p), horner11(c_PE, p));
+  double P, dP, Q, dQ;
+  horner11_d(c_PK, p, P, dP);
+  horner11_d(c_QK, p, Q, dQ);
+  K = fma(-lp, Q, P);
+  const double pkp = fma(p, dP, -Q) - (p * lp) * dQ;
+  E = fma(p, K, -2.0 * (1.0 - p) * pkp);
 }
 
 __device__ __forceinline__ double ellip_k_c(double p) {
   const double lp = log(p);
-  return fma(-lp, horner11(c_QK, p), horner11(c_PK, p));
+  double s = c_PK[10], q = c_QK[10];
+#pragma unroll
+  for (int i = 9; i >= 0; --i) {
+    s = fma(s, p, c_PK[i]);
+    q = fma(q, p, c_QK[i]);
+  }
+  return fma(-lp, q, s);
 }
 
 // 1/x to ~1 ulp: hardware reciprocal estimate plus two Newton steps.

@@ -1,0 +1,182 @@
+/* spt.h -- C ABI of libspt_hip.so, the MI355X (gfx950) implementation of
+ * the specific_particle_tracer tracing path.
+ *
+ * The reference (AdamCBartnik/BEM, package specific_particle_tracer) is pure
+ * Python with no FFI; its plug-in surfaces are the backend name
+ * (backend.py:18-41), the Geometry ABC (geometry.py:37-94) and the
+ * integrator callback protocol (batched_rk45.py:55-148). Each entry point
+ * below names the reference function it replaces. A Python package binds
+ * them with ctypes (bem_amd/_lib.py).
+ *
+ * Conventions
+ *  - Pointers documented "device" are caller-owned, contiguous device
+ *    buffers (e.g. torch.Tensor.data_ptr()); "host" pointers are ordinary
+ *    host memory. The library owns only its context, its scratch and the
+ *    BEM tables uploaded into it, and frees none of the caller's memory.
+ *  - Positions are (n, 3) row-major float64 in metres, SI units throughout.
+ *  - Every call returns SPT_OK (0) or a negative code; spt_last_error()
+ *    gives the message. No C++ exception crosses the boundary.
+ *  - One context per device. A context is not re-entrant: one host thread
+ *    uses it at a time. Work is enqueued on the context's stream;
+ *    spt_sync() waits for it.
+ */
+#ifndef SPT_H
+#define SPT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPT_OK 0
+#define SPT_EINVAL (-1)
+#define SPT_ENOMEM (-2)
+#define SPT_EHIP (-3)
+#define SPT_ENOBEM (-4)
+#define SPT_ESTEPCAP (-5)
+
+#define SPT_MAX_SCREENS 16
+
+/* Geometry kinds (reference geometry.py:97-256, bem/geometry.py:58-507). */
+#define SPT_GEOM_FLAT 0     /* FlatCathode: GunField, plane image            */
+#define SPT_GEOM_TIP 1      /* HemisphericalTip: closed-form sphere field    */
+#define SPT_GEOM_BEM_POST 2 /* HemisphericalTipBEMGeometry (H = 0) and the   */
+                            /* hemisphere-capped post (H > 0): uniform field */
+                            /* + uploaded BEM perturbation, smoothstep blend */
+                            /* to the flat cathode between z_handoff/2 and   */
+                            /* z_handoff                                     */
+
+/* Image-charge models (reference forces.py:60-191). */
+#define SPT_IMAGE_NONE 0
+#define SPT_IMAGE_PLANE 1  /* image at z' = -2 z0 - z, charge -q            */
+#define SPT_IMAGE_TIP3 2   /* sphere R - z0 + plane z0: three images        */
+
+typedef struct spt_ctx spt_ctx;
+
+typedef struct spt_geometry {
+  int32_t kind;        /* SPT_GEOM_*                                         */
+  int32_t image;       /* SPT_IMAGE_*                                        */
+  double E_gun;        /* asymptotic Ez [V/m]                                */
+  double R;            /* tip / post radius [m]                              */
+  double H;            /* post height [m] (cap centre at z = H); 0 = bare    */
+                       /* hemisphere                                         */
+  double z0;           /* image-plane recession [m]                          */
+  double z_handoff;    /* BEM-to-flat blend height [m] (SPT_GEOM_BEM_POST)   */
+  int32_t has_kill_z;  /* kill at z <= kill_z_below                          */
+  int32_t has_z_max;   /* kill at z >= z_max                                 */
+  double kill_z_below;
+  double z_max;
+} spt_geometry;
+
+/* One screen crossing (reference screens.py:18-33,96-115). `batch` is the
+ * group's attempt index of the accepted step that produced it, which is
+ * the reference's loop-iteration index; sorting by (batch, interval, group,
+ * particle) within a screen gives the reference's recording order. */
+typedef struct spt_event {
+  int32_t group;
+  int32_t particle; /* index within the group */
+  int32_t screen;
+  int32_t interval; /* 0..2: monotone piece of the cubic Hermite */
+  int64_t batch;
+  double t;
+  double pos[3];
+  double vel[3];
+} spt_event;
+
+/* Group status words written by spt_trace. */
+#define SPT_STATUS_DONE 0
+#define SPT_STATUS_STEPCAP 1
+
+typedef struct spt_trace_args {
+  spt_geometry geom;
+  int32_t n_groups;
+  int32_t block_threads;        /* 0 = library default                      */
+  int64_t n_particles;
+  const int64_t* group_offsets; /* device, n_groups + 1 (CSR)               */
+  const int32_t* group_order;   /* device, n_groups: queue order, or NULL   */
+  double* pos;                  /* device (n, 3), in/out                    */
+  double* vel;                  /* device (n, 3), in/out                    */
+  const double* t_birth;        /* device (n)                               */
+  const double* charge;         /* device (n) signed charge [C]             */
+  const double* mass;           /* device (n) particle mass [kg]            */
+  double* t_death;              /* device (n), out (inf = alive)            */
+  double q_over_m;              /* species charge/mass for the field term   */
+  double plummer_radius;
+  double t_max;
+  double rtol, atol;
+  double safety, min_factor, max_factor, initial_step_fraction;
+  int64_t max_steps;
+  int32_t n_screens;
+  int32_t pad0;
+  double screens[SPT_MAX_SCREENS];
+  /* outputs, device */
+  int64_t* attempts;            /* per group                                */
+  int64_t* accepted;            /* per group                                */
+  int64_t* active_stages;       /* per group: 7 x active particles summed   */
+  double* margin;               /* per group: min |error_norm - 1|          */
+  int32_t* status;              /* per group: SPT_STATUS_*                  */
+  spt_event* events;            /* capacity entries                         */
+  int64_t event_capacity;
+  uint64_t* event_count;        /* total found (may exceed capacity)        */
+} spt_trace_args;
+
+/* Context. */
+int spt_create(int device, spt_ctx** out);
+void spt_destroy(spt_ctx* ctx);
+const char* spt_last_error(const spt_ctx* ctx);
+int spt_set_stream(spt_ctx* ctx, void* hip_stream);
+int spt_sync(spt_ctx* ctx);
+int spt_device_cus(spt_ctx* ctx, int* n_cu);
+
+/* BEM tables. Replaces AxisymmetricBEMSolution.quadrature/_prepare_field_data
+ * (reference bem/axisymmetric.py:305-338,464-498): builds the fixed
+ * Gauss-4 node table (rho, z, area*sigma) and per-segment geometry on the
+ * device. profile_rz and sigma are host arrays of n_nodes entries. */
+int spt_bem_upload(spt_ctx* ctx, const double* profile_rz, int32_t n_nodes, const double* sigma,
+                   int32_t n_subdiv);
+
+/* Collocation matrix of the P1 hat basis (reference
+ * bem/axisymmetric.py:275-302,430-441), row-major into host A_out
+ * (n_nodes x n_nodes). */
+int spt_bem_assemble(spt_ctx* ctx, const double* profile_rz, int32_t n_nodes, double refine_ratio,
+                     int32_t max_depth, double* A_out);
+
+/* AxisymmetricBEMSolution.solve (reference bem/axisymmetric.py:411-443) for
+ * Dirichlet data E_gun * z: device assembly plus an FP64 LU solve; sigma_out
+ * is host (n_nodes). */
+int spt_bem_solve(spt_ctx* ctx, const double* profile_rz, int32_t n_nodes, double E_gun, double refine_ratio,
+                  int32_t max_depth, double* sigma_out);
+
+/* evaluate_axisymmetric_field (reference bem/axisymmetric.py:501-605) on the
+ * uploaded table: perturbation field at n device points into device E. */
+int spt_bem_pert_field(spt_ctx* ctx, const double* pos, int64_t n, double* E);
+
+/* Geometry.field (reference geometry.py:144-145,227-228;
+ * bem/geometry.py:319-324): full external field [V/m] at device points. */
+int spt_geometry_field(spt_ctx* ctx, const spt_geometry* geom, const double* pos, int64_t n, double* E);
+
+/* Pair forces within CSR groups (reference forces.py:32-191; Geometry.image_force):
+ * coulomb_out gets the Plummer-softened in-group Coulomb force, image_out the
+ * image-charge force of geom->image. Either output may be NULL. active is
+ * one byte per particle. All device. */
+int spt_pair_forces(spt_ctx* ctx, const spt_geometry* geom, const double* pos, const double* charge,
+                    const uint8_t* active, const int64_t* group_offsets, int32_t n_groups,
+                    double plummer_radius, double* coulomb_out, double* image_out);
+
+/* Geometry.kill_mask (+ z_max, reference geometry.py:304-322): one byte per
+ * device point. */
+int spt_kill_mask(spt_ctx* ctx, const spt_geometry* geom, const double* pos, int64_t n, uint8_t* out);
+
+/* batched_rk45.integrate + make_accel_fn + find_crossings_grouped
+ * (reference batched_rk45.py:55-401, geometry.py:265-301, screens.py:43-168,
+ * tracker.py:340-372) as one persistent kernel: every group advances with
+ * its own t and h, accept/reject on device, screen events appended to
+ * args->events. Asynchronous; call spt_sync before reading outputs. */
+int spt_trace(spt_ctx* ctx, const spt_trace_args* args);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPT_H */

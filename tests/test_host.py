@@ -1,0 +1,161 @@
+"""Host-side checks that need no GPU: the C ABI library and its exports,
+the device K/E coefficients, emission ordering and samplers, profiles,
+tracker set-up."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+from scipy import special
+
+from conftest import ROOT
+from bem_amd import emission
+from bem_amd.bem.mesh import n_for_length, post_profile, sphere_profile
+
+HEADER = os.path.join(ROOT, "include", "spt.h")
+LIB = os.path.join(ROOT, "bem_amd", "libspt_hip.so")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(spt_\w+)\s*\(", src, re.M)))
+
+
+def test_header_and_binding_agree():
+    from bem_amd import _lib
+
+    assert declared_functions() == sorted(_lib.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(LIB):
+        pytest.skip("libspt_hip.so not built")
+    lib = ctypes.CDLL(LIB)
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_struct_layouts():
+    from bem_amd import _lib
+
+    assert ctypes.sizeof(_lib.Geometry_t) == 72
+    assert ctypes.sizeof(_lib.Event_t) == 80
+    assert _lib.TraceArgs_t.n_groups.offset == 72
+    assert _lib.TraceArgs_t.screens.offset % 8 == 0
+
+
+def _coeffs(name):
+    src = open(os.path.join(ROOT, "bem_amd", "csrc", "ellip_coeffs.h")).read()
+    return [float(x) for x in re.search(r"SPT_ELLIP_%s \{ (.*) \}" % name, src).group(1).split(",")]
+
+
+def _device_ke(p):
+    """The device formula (spt_device.h ellip_ke_c) restated in NumPy."""
+    PK, QK = _coeffs("PK"), _coeffs("QK")
+
+    def hd(c, x):
+        s = np.full_like(x, c[10])
+        d = np.zeros_like(x)
+        for a in reversed(c[:10]):
+            d = d * x + s
+            s = s * x + a
+        return s, d
+
+    lp = np.log(p)
+    P, dP = hd(PK, p)
+    Q, dQ = hd(QK, p)
+    K = P - lp * Q
+    pkp = p * dP - Q - (p * lp) * dQ
+    return K, p * K - 2.0 * (1.0 - p) * pkp
+
+
+def test_device_ellip_coefficients_against_scipy_and_identities():
+    rng = np.random.default_rng(0)
+    p = np.concatenate([10 ** rng.uniform(-300, 0, 4000), rng.random(4000), [1.0]])
+    K, E = _device_ke(p)
+    np.testing.assert_allclose(K, special.ellipkm1(p), rtol=4e-16 * 4)
+    np.testing.assert_allclose(E, special.ellipe(1 - p), rtol=4e-16 * 4)
+    # K(0) = E(0) = pi/2; E(1) = 1 (DLMF 19.6.1)
+    K1, E1 = _device_ke(np.array([1.0]))
+    assert abs(K1[0] - np.pi / 2) < 4e-16 and abs(E1[0] - np.pi / 2) < 4e-16
+    # Legendre relation at m = 1/2 (DLMF 19.7.1): 2 E K - K^2 = pi/2
+    Kh, Eh = _device_ke(np.array([0.5]))
+    assert abs(2 * Eh[0] * Kh[0] - Kh[0] ** 2 - np.pi / 2) < 2e-15
+
+
+def test_group_indices_sort_and_errors():
+    ids = np.array([5, 3, 9, 1])
+    np.testing.assert_array_equal(emission.group_indices(ids, 2), [[3, 1], [0, 2]])
+    with pytest.raises(ValueError):
+        emission.group_indices(ids, 3)
+    with pytest.raises(ValueError):
+        emission.group_indices(np.array([1, 1, 2, 3]), 2)
+    order, off = emission.group_offsets(ids, [1, 3])
+    np.testing.assert_array_equal(order, [3, 1, 0, 2])
+    np.testing.assert_array_equal(off, [0, 1, 4])
+    with pytest.raises(ValueError):
+        emission.group_offsets(ids, [2, 3])
+
+
+def test_samplers_are_seeded():
+    a = emission.flat_gaussian_source(16, seed=0)
+    b = emission.flat_gaussian_source(16, seed=0)
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k])
+    c = emission.cap_source(32, 1e-6, 10e-6, seed=0)
+    r = np.sqrt(c["x"] ** 2 + c["y"] ** 2 + (c["z"] - 10e-6) ** 2)
+    np.testing.assert_allclose(r, 1e-6, rtol=1e-12)
+    th = np.degrees(np.arccos((c["z"] - 10e-6) / 1e-6))
+    assert th.max() <= 60.0
+    # momentum points outward (cos-weighted about the normal)
+    n = np.stack([c["x"], c["y"], c["z"] - 10e-6], -1)
+    p = np.stack([c["px"], c["py"], c["pz"]], -1)
+    assert np.all(np.sum(n * p, -1) >= 0)
+
+
+def test_profiles():
+    s = sphere_profile(1.0, n_for_length(np.pi, np.pi / 39))
+    assert s.shape == (40, 2)
+    p = post_profile(1e-6, 10e-6, 400)
+    assert p.shape == (401, 2)
+    assert p[0, 0] == 0.0 and p[-1, 0] == 0.0
+    L = np.linalg.norm(np.diff(p, axis=0), axis=1)
+    assert L.max() / L.min() < 1.02
+    np.testing.assert_allclose(p[:, 1], -p[::-1, 1], atol=1e-20)   # mirror symmetric
+    np.testing.assert_allclose(p[:, 0], p[::-1, 0], atol=1e-20)
+
+
+def test_tracker_setup_matches_oracle():
+    from bem_amd import FlatCathode, SpecificParticleTracer
+    from bem_amd.particles import ParticleGroup
+    from oracle import tracker as otr
+
+    src = emission.flat_gaussian_source(12, seed=1)
+    src["id"] = np.arange(12)[::-1] + 100
+    pg = ParticleGroup(dict(src, weight=np.full(12, 1.602176634e-19), status=np.ones(12), species="electron"))
+    tr = SpecificParticleTracer(pg, 4, FlatCathode(-1e8), [1e-6])
+    st = otr.setup(src["x"], src["y"], src["z"], src["px"], src["py"], src["pz"], src["t"], src["id"], 4)
+    np.testing.assert_array_equal(tr.pos0.reshape(3, 4, 3), st["pos"])
+    np.testing.assert_array_equal(tr.vel0.reshape(3, 4, 3), st["vel"])
+    np.testing.assert_array_equal(tr.ids.reshape(3, 4), st["ids"])
+    assert tr.t_max == otr.default_t_max(st, -1e8, [1e-6])
+    with pytest.raises(ValueError):
+        SpecificParticleTracer(pg, 4, FlatCathode(-1e8), [1e-6], n_workers=2)
+    with pytest.raises(ValueError):
+        SpecificParticleTracer(pg, 5, FlatCathode(-1e8), [1e-6])
+    with pytest.raises(ValueError):
+        SpecificParticleTracer(pg, 4, FlatCathode(-1e8), [1e-6], backend="cpu")
+    with pytest.warns(UserWarning):
+        SpecificParticleTracer(pg, 4, FlatCathode(-1e8), [2e-6], z_max=1e-6)
+
+
+def test_weight_override_warning():
+    from bem_amd import FlatCathode, SpecificParticleTracer
+    from bem_amd.particles import ParticleGroup
+
+    src = emission.flat_gaussian_source(4, seed=1)
+    pg = ParticleGroup(dict(src, weight=np.full(4, 2e-19), status=np.ones(4), species="electron"))
+    with pytest.warns(UserWarning):
+        tr = SpecificParticleTracer(pg, 2, FlatCathode(-1e8), [1e-6])
+    np.testing.assert_array_equal(tr.weight, 1.602176634e-19)
