@@ -53,7 +53,10 @@ __constant__ double c_A[6][5] = {
 __constant__ double c_B[6] = {35.0 / 384, 0.0, 500.0 / 1113, 125.0 / 192, -2187.0 / 6784, 11.0 / 84};
 __constant__ double c_E[7] = {71.0 / 57600, 0.0, -71.0 / 16695, 71.0 / 1920, -17253.0 / 339200, 22.0 / 525, -1.0 / 40};
 
-constexpr double kCoulomb = 8987551785.9720688;  // 1/(4 pi eps0), eps0 = 8.8541878188e-12
+// 1/(4 pi eps0) with CODATA 2022 eps0, folded in the same operation order
+// as the reference (constants.py:14): 8987551786.170797.
+constexpr double kVacuumPermittivity = 8.8541878188e-12;
+constexpr double kCoulomb = 1.0 / (4.0 * kPi * kVacuumPermittivity);
 constexpr double kLight = 299792458.0;
 
 // ---------------------------------------------------------------- tables
