@@ -22,6 +22,7 @@ SPT_MAX_SCREENS = 16
 GEOM_FLAT, GEOM_TIP, GEOM_BEM_POST = 0, 1, 2
 IMAGE_NONE, IMAGE_PLANE, IMAGE_TIP3 = 0, 1, 2
 STATUS_DONE, STATUS_STEPCAP = 0, 1
+MODE_PERSISTENT, MODE_LOCKSTEP = 0, 1
 
 EXPORTS = (
     "spt_create", "spt_destroy", "spt_last_error", "spt_set_stream", "spt_sync", "spt_device_cus",
@@ -68,7 +69,7 @@ class TraceArgs_t(ctypes.Structure):
         ("safety", ctypes.c_double), ("min_factor", ctypes.c_double), ("max_factor", ctypes.c_double),
         ("initial_step_fraction", ctypes.c_double),
         ("max_steps", ctypes.c_int64),
-        ("n_screens", ctypes.c_int32), ("pad0", ctypes.c_int32),
+        ("n_screens", ctypes.c_int32), ("mode", ctypes.c_int32),
         ("screens", ctypes.c_double * SPT_MAX_SCREENS),
         ("attempts", _p), ("accepted", _p), ("active_stages", _p), ("margin", _p), ("status", _p),
         ("events", _p), ("event_capacity", ctypes.c_int64), ("event_count", _p),

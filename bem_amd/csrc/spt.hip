@@ -749,8 +749,11 @@ __global__ __launch_bounds__(T) void trace_kernel(TraceParams P) {
   }
 }
 
+
+
 __global__ void reset_int_kernel(int* p, int v) { *p = v; }
 __global__ void reset_u64_kernel(unsigned long long* p) { *p = 0ULL; }
+__global__ void reset_u32_kernel(unsigned int* p) { *p = 0u; }
 
 // ----------------------------------------------- standalone field kernels
 template <int T, int NMAX>
@@ -833,6 +836,299 @@ __global__ __launch_bounds__(T) void geometry_field_kernel(spt_geometry G, BemVi
     E[3 * (base + tid) + 2] = ez;
   }
 }
+
+// ============================================================ lockstep mode
+// The reference's schedule (batched_rk45.py:219-319): every running group
+// makes one attempt per iteration. Per attempt: 7 x (field gather over all
+// particles, then per-group accel), then one per-group end-of-attempt
+// kernel that also prepares the next attempt. Group scalars and particle
+// stage data live in global scratch between kernels.
+//
+// Extra per-particle slots: 48..50 stage position, 51..53 field there.
+constexpr int kSlotsLockstep = 54;
+constexpr int LS_RUNNING = 1, LS_EXHAUSTED = 2;
+
+struct LockParams {
+  TraceParams T;
+  const int32_t* pgroup;   // particle -> group
+  uint8_t* active;         // per particle, frozen for the current attempt
+  double* g_t;
+  double* g_h;
+  double* g_hr;
+  double* g_next;
+  double* g_nact;
+  int32_t* g_flags;
+  unsigned int* done;      // finished groups
+};
+
+__global__ void ls_pgroup_kernel(const int64_t* offsets, int n_groups, int32_t* pgroup) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_groups) return;
+  for (int64_t i = offsets[g]; i < offsets[g + 1]; ++i) pgroup[i] = g;
+}
+
+#define LKB(sl) L.T.kbuf[(size_t)(sl) * (size_t)L.T.n_total + (size_t)gi]
+
+// Start an attempt for group g at (t, h): birth checkpoint, clipped step,
+// frozen active set, stage-0 state (reference batched_rk45.py:223-249,348).
+template <int TG>
+__device__ void ls_begin(const LockParams& L, int g, int n, double t, double h, int64_t attempts, double* red) {
+  const int tid = threadIdx.x;
+  const bool mine = tid < n;
+  const int64_t gi = L.T.offsets[g] + tid;
+  if (attempts >= L.T.max_steps) {
+    if (tid == 0) {
+      L.T.status[g] = SPT_STATUS_STEPCAP;
+      L.g_flags[g] = 0;
+      atomicAdd(L.done, 1u);
+    }
+    return;
+  }
+  const double tb = mine ? L.T.t_birth[gi] : INFINITY;
+  const double n_reached = block_sum<TG>((mine && tb <= t) ? 1.0 : 0.0, red);
+  const double nxt_cp = block_min<TG>((mine && tb > t) ? tb : INFINITY, red);
+  const bool exhausted = n_reached >= (double)n;
+  const double next = exhausted ? L.T.t_max : nxt_cp;
+  const double target = fmin(next, L.T.t_max);
+  const double hr = fmin(h, target - t);
+  const bool active = mine && (t >= tb) && (t < L.T.t_death[gi]);
+  const double n_act = block_sum<TG>(active ? 1.0 : 0.0, red);
+  if (mine) {
+    L.active[gi] = active ? 1 : 0;
+    for (int c = 0; c < 3; ++c) {
+      const double ps = L.T.pos[3 * gi + c], vs = L.T.vel[3 * gi + c];
+      LKB(c) = active ? vs : 0.0;
+      LKB(48 + c) = ps;
+    }
+  }
+  if (tid == 0) {
+    L.g_hr[g] = hr;
+    L.g_next[g] = next;
+    L.g_nact[g] = n_act;
+    L.g_flags[g] = LS_RUNNING | (exhausted ? LS_EXHAUSTED : 0);
+  }
+}
+
+template <int TG>
+__global__ __launch_bounds__(TG) void ls_init_kernel(LockParams L) {
+  __shared__ double red[TG];
+  const int g = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int64_t off = L.T.offsets[g];
+  const int n = (int)(L.T.offsets[g + 1] - off);
+  const bool mine = tid < n;
+  const int64_t gi = off + tid;
+  if (mine) L.T.t_death[gi] = INFINITY;
+  const double t = block_min<TG>(mine ? L.T.t_birth[gi] : INFINITY, red);
+  const double h = (L.T.t_max - t) * L.T.init_frac;
+  if (tid == 0) {
+    L.g_t[g] = t;
+    L.g_h[g] = h;
+    L.T.attempts[g] = 0;
+    L.T.accepted[g] = 0;
+    L.T.active_stages[g] = 0;
+    L.T.margin[g] = INFINITY;
+    L.T.status[g] = SPT_STATUS_DONE;
+  }
+  __syncthreads();
+  ls_begin<TG>(L, g, n, t, h, 0, red);
+}
+
+// External field at the stage positions of every active particle of every
+// running group; PB particles per block, T/PB threads per particle.
+template <int T, int NMAX>
+__global__ __launch_bounds__(T) void ls_field_kernel(LockParams L, int PB) {
+  __shared__ FieldShared<T, NMAX> sh;
+  __shared__ double sx[NMAX], sy[NMAX], sz[NMAX];
+  __shared__ int act[NMAX];
+  const int64_t base = (int64_t)blockIdx.x * PB;
+  const int m_all = (int)min((int64_t)PB, L.T.n_total - base);
+  const int tid = threadIdx.x;
+  const spt_geometry& G = L.T.G;
+  if (tid < m_all) {
+    const int64_t gi = base + tid;
+    const int g = L.pgroup[gi];
+    act[tid] = (L.g_flags[g] & LS_RUNNING) && L.active[gi];
+    sx[tid] = LKB(48);
+    sy[tid] = LKB(49);
+    sz[tid] = LKB(50);
+    sh.need[tid] = act[tid] && G.kind == SPT_GEOM_BEM_POST && bem_needed(G, sx[tid], sy[tid], sz[tid]);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int m = 0;
+    for (int i = 0; i < m_all; ++i)
+      if (sh.need[i]) sh.cidx[m++] = i;
+    sh.m = m;
+  }
+  __syncthreads();
+  const int m = sh.m;
+  if (tid < m) {
+    const int i = sh.cidx[tid];
+    sh.px[tid] = sx[i];
+    sh.py[tid] = sy[i];
+    sh.pz[tid] = sz[i];
+  }
+  __syncthreads();
+  bem_pert_block<T, NMAX>(L.T.bem, m, sh.px, sh.py, sh.pz, sh.ox, sh.oy, sh.oz, sh.bem);
+  if (tid < m) {
+    const int i = sh.cidx[tid];
+    const int64_t gi = base + i;
+    double ex, ey, ez;
+    geometry_field_point(G, sx[i], sy[i], sz[i], sh.ox[tid], sh.oy[tid], sh.oz[tid], ex, ey, ez);
+    LKB(51) = ex;
+    LKB(52) = ey;
+    LKB(53) = ez;
+  }
+  if (tid < m_all && act[tid] && !sh.need[tid]) {
+    const int64_t gi = base + tid;
+    double ex, ey, ez;
+    geometry_field_point(G, sx[tid], sy[tid], sz[tid], 0, 0, 0, ex, ey, ez);
+    LKB(51) = ex;
+    LKB(52) = ey;
+    LKB(53) = ez;
+  }
+}
+
+// Acceleration of stage st for one group (reference geometry.py:289-299,
+// batched_rk45.py:362-363), then the next stage's state.
+template <int TG>
+__global__ __launch_bounds__(TG) void ls_accel_kernel(LockParams L, int st) {
+  __shared__ double sx[TG], sy[TG], sz[TG], sq[TG];
+  __shared__ int act[TG];
+  const int g = blockIdx.x;
+  if (!(L.g_flags[g] & LS_RUNNING)) return;
+  const int tid = threadIdx.x;
+  const int64_t off = L.T.offsets[g];
+  const int n = (int)(L.T.offsets[g + 1] - off);
+  const bool mine = tid < n;
+  const int64_t gi = off + tid;
+  if (mine) {
+    sx[tid] = LKB(48);
+    sy[tid] = LKB(49);
+    sz[tid] = LKB(50);
+    sq[tid] = L.T.charge[gi];
+    act[tid] = L.active[gi];
+  }
+  __syncthreads();
+  if (!mine) return;
+  const bool active = act[tid];
+  const double hr = L.g_hr[g];
+  double Fc[3], Fi[3];
+  pair_forces_i(L.T.G, tid, n, sx, sy, sz, sq, act, L.T.a2, Fc, Fi);
+  const double mass = L.T.mass[gi];
+  for (int c = 0; c < 3; ++c) {
+    const double ac = n == 1 ? 0.0 : Fc[c] / mass;
+    const double a = L.T.q_over_m * LKB(51 + c) + ac + Fi[c] / mass;
+    LKB(st * 6 + 3 + c) = active ? a : 0.0;
+  }
+  if (st < 5) {
+    const int ns = st + 1;
+    for (int c = 0; c < 3; ++c) {
+      double ps = L.T.pos[3 * gi + c], vs = L.T.vel[3 * gi + c];
+      for (int j = 0; j < ns; ++j) {
+        const double ha = hr * c_A[ns][j];
+        ps = ps + ha * LKB(j * 6 + c);
+        vs = vs + ha * LKB(j * 6 + 3 + c);
+      }
+      LKB(ns * 6 + c) = active ? vs : 0.0;
+      LKB(48 + c) = ps;
+    }
+  } else if (st == 5) {
+    for (int c = 0; c < 3; ++c) {
+      double sp = c_B[0] * LKB(c), sv = c_B[0] * LKB(3 + c);
+      for (int k = 1; k < 6; ++k) {
+        sp = sp + c_B[k] * LKB(k * 6 + c);
+        sv = sv + c_B[k] * LKB(k * 6 + 3 + c);
+      }
+      const double pn = L.T.pos[3 * gi + c] + hr * sp;
+      const double vn = L.T.vel[3 * gi + c] + hr * sv;
+      LKB(42 + c) = pn;
+      LKB(45 + c) = vn;
+      LKB(36 + c) = active ? vn : 0.0;
+      LKB(48 + c) = pn;
+    }
+  }
+}
+
+// Error norm, accept/reject, commit, birth reset, kill, screens (reference
+// batched_rk45.py:250-317, 372-399), then the group's next attempt.
+template <int TG>
+__global__ __launch_bounds__(TG) void ls_end_kernel(LockParams L) {
+  __shared__ double red[TG];
+  const int g = blockIdx.x;
+  const int flags = L.g_flags[g];
+  if (!(flags & LS_RUNNING)) return;
+  const int tid = threadIdx.x;
+  const int64_t off = L.T.offsets[g];
+  const int n = (int)(L.T.offsets[g + 1] - off);
+  const bool mine = tid < n;
+  const int64_t gi = off + tid;
+  const double hr = L.g_hr[g];
+  double lp = 0.0, lv = 0.0;
+  if (mine) {
+    for (int c = 0; c < 3; ++c) {
+      double ep = c_E[0] * LKB(c), ev = c_E[0] * LKB(3 + c);
+      for (int k = 1; k < 7; ++k) {
+        ep = ep + c_E[k] * LKB(k * 6 + c);
+        ev = ev + c_E[k] * LKB(k * 6 + 3 + c);
+      }
+      ep = hr * ep;
+      ev = hr * ev;
+      const double sp = L.T.atol + L.T.rtol * fmax(fabs(L.T.pos[3 * gi + c]), fabs(LKB(42 + c)));
+      const double sv = kLight * L.T.atol + L.T.rtol * fmax(fabs(L.T.vel[3 * gi + c]), fabs(LKB(45 + c)));
+      const double a = ep / sp, b = ev / sv;
+      lp += a * a;
+      lv += b * b;
+    }
+  }
+  const double sq = block_sum<TG>(lp, red) + block_sum<TG>(lv, red);
+  double err = sqrt(sq / (6.0 * n));
+  if (err == 0.0) err = 2.2250738585072014e-308;
+  const bool acc = err <= 1.0;
+  double fac = L.T.safety * pow(err, -1.0 / 5);
+  fac = fmin(fmax(fac, L.T.fmin_), L.T.fmax_);
+  const int64_t batch = L.T.attempts[g];
+  double t = L.g_t[g];
+  double h = hr * fac;
+  bool running = true;
+  if (acc) {
+    const double t_old = t;
+    t = t + hr;
+    const double tol = 1e-9 * fmax(L.T.t_max, 1e-30);
+    if (!(flags & LS_EXHAUSTED) && t >= L.g_next[g] - tol) h = (L.T.t_max - t) * L.T.init_frac;
+    if (mine) {
+      double x0[3], v0[3], x1[3], v1[3];
+      for (int c = 0; c < 3; ++c) {
+        x0[c] = L.T.pos[3 * gi + c];
+        v0[c] = L.T.vel[3 * gi + c];
+        x1[c] = LKB(42 + c);
+        v1[c] = LKB(45 + c);
+        L.T.pos[3 * gi + c] = x1[c];
+        L.T.vel[3 * gi + c] = v1[c];
+      }
+      if (L.active[gi] && kill_point(L.T.G, x1[0], x1[1], x1[2])) L.T.t_death[gi] = t;
+      screens_particle(L.T.SP, L.T.events, L.T.ev_cap, L.T.ev_count, g, tid, batch, t_old, t, x0, v0, x1, v1);
+    }
+    running = t < L.T.t_max * (1 - 1e-14);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    L.T.attempts[g] = batch + 1;
+    if (acc) L.T.accepted[g] += 1;
+    L.T.margin[g] = fmin(L.T.margin[g], fabs(err - 1.0));
+    L.T.active_stages[g] += 7 * (int64_t)L.g_nact[g];
+    L.g_t[g] = t;
+    L.g_h[g] = h;
+    if (!running) {
+      L.g_flags[g] = 0;
+      atomicAdd(L.done, 1u);
+    }
+  }
+  if (!running) return;
+  ls_begin<TG>(L, g, n, t, h, batch + 1, red);
+}
+#undef LKB
 
 // In-group pair forces: one block per group, threads over its particles.
 template <int NMAX>
@@ -963,6 +1259,17 @@ struct spt_ctx {
   size_t kbuf_elems = 0;
   int* d_queue = nullptr;
   rocblas_handle blas = nullptr;
+  // lockstep scratch
+  hipStream_t own = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  int32_t* d_pgroup = nullptr;
+  uint8_t* d_active = nullptr;
+  size_t part_cap = 0;
+  double* d_gdbl = nullptr;   // 5 x group doubles
+  int32_t* d_gflags = nullptr;
+  size_t group_cap = 0;
+  unsigned int* d_done = nullptr;
+  unsigned int* h_done = nullptr;
 };
 
 static int fail(spt_ctx* c, int code, const std::string& msg) {
@@ -1030,6 +1337,15 @@ void spt_destroy(spt_ctx* c) {
   free_bem(c);
   if (c->d_kbuf) (void)hipFree(c->d_kbuf);
   if (c->d_queue) (void)hipFree(c->d_queue);
+  if (c->d_pgroup) (void)hipFree(c->d_pgroup);
+  if (c->d_active) (void)hipFree(c->d_active);
+  if (c->d_gdbl) (void)hipFree(c->d_gdbl);
+  if (c->d_gflags) (void)hipFree(c->d_gflags);
+  if (c->d_done) (void)hipFree(c->d_done);
+  if (c->h_done) (void)hipHostFree(c->h_done);
+  if (c->ev_in) (void)hipEventDestroy(c->ev_in);
+  if (c->ev_out) (void)hipEventDestroy(c->ev_out);
+  if (c->own) (void)hipStreamDestroy(c->own);
   if (c->blas) rocblas_destroy_handle(c->blas);
   delete c;
 }
@@ -1244,6 +1560,83 @@ static int launch_trace(spt_ctx* c, const TraceParams& P, int n_groups) {
   return SPT_OK;
 }
 
+
+constexpr int kLockstepUnroll = 4;  // attempts per captured graph replay
+
+template <int TG>
+static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, int PB) {
+  hipStream_t s = c->own;
+  const unsigned nb_field = (unsigned)((n + PB - 1) / PB);
+  ls_pgroup_kernel<<<(n_groups + 255) / 256, 256, 0, s>>>(L.T.offsets, n_groups, (int32_t*)L.pgroup);
+  reset_u32_kernel<<<1, 1, 0, s>>>(L.done);
+  reset_u64_kernel<<<1, 1, 0, s>>>(L.T.ev_count);
+  ls_init_kernel<TG><<<n_groups, TG, 0, s>>>(L);
+  HIPCHK(c, hipGetLastError());
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  for (int u = 0; u < kLockstepUnroll; ++u) {
+    for (int st = 0; st < 7; ++st) {
+      ls_field_kernel<256, 64><<<nb_field, 256, 0, s>>>(L, PB);
+      ls_accel_kernel<TG><<<n_groups, TG, 0, s>>>(L, st);
+    }
+    ls_end_kernel<TG><<<n_groups, TG, 0, s>>>(L);
+  }
+  HIPCHK(c, hipStreamEndCapture(s, &graph));
+  HIPCHK(c, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  int64_t iters = 0;
+  int rc = SPT_OK;
+  while (true) {
+    hipError_t e = hipGraphLaunch(exec, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->h_done, L.done, sizeof(unsigned int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      rc = fail(c, SPT_EHIP, std::string("lockstep replay: ") + hipGetErrorString(e));
+      break;
+    }
+    iters += kLockstepUnroll;
+    if (*c->h_done >= (unsigned)n_groups) break;
+    if (iters > L.T.max_steps + 2 * kLockstepUnroll) {
+      rc = fail(c, SPT_EHIP, "lockstep schedule did not terminate");
+      break;
+    }
+  }
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  return rc;
+}
+
+static int ensure_lockstep_scratch(spt_ctx* c, int64_t n, int n_groups) {
+  if (!c->own) {
+    HIPCHK(c, hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming));
+    HIPCHK(c, hipMalloc(&c->d_done, 64));
+    HIPCHK(c, hipHostMalloc(&c->h_done, 64, hipHostMallocDefault));
+  }
+  if ((size_t)n > c->part_cap) {
+    if (c->d_pgroup) (void)hipFree(c->d_pgroup);
+    if (c->d_active) (void)hipFree(c->d_active);
+    c->d_pgroup = nullptr;
+    c->d_active = nullptr;
+    c->part_cap = 0;
+    HIPCHK(c, hipMalloc(&c->d_pgroup, sizeof(int32_t) * n));
+    HIPCHK(c, hipMalloc(&c->d_active, n));
+    c->part_cap = n;
+  }
+  if ((size_t)n_groups > c->group_cap) {
+    if (c->d_gdbl) (void)hipFree(c->d_gdbl);
+    if (c->d_gflags) (void)hipFree(c->d_gflags);
+    c->d_gdbl = nullptr;
+    c->d_gflags = nullptr;
+    c->group_cap = 0;
+    HIPCHK(c, hipMalloc(&c->d_gdbl, sizeof(double) * 5 * n_groups));
+    HIPCHK(c, hipMalloc(&c->d_gflags, sizeof(int32_t) * n_groups));
+    c->group_cap = n_groups;
+  }
+  return SPT_OK;
+}
+
 extern "C" {
 
 int spt_trace(spt_ctx* c, const spt_trace_args* a) {
@@ -1281,7 +1674,8 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
     T = mx <= 64 ? 64 : (mx <= 128 ? 128 : 256);
     if (mx > 256) return fail(c, SPT_EINVAL, "groups larger than 256 particles are not supported");
   }
-  const size_t need = (size_t)kSlots * (size_t)a->n_particles;
+  if (a->mode != SPT_MODE_PERSISTENT && a->mode != SPT_MODE_LOCKSTEP) return fail(c, SPT_EINVAL, "unknown mode");
+  const size_t need = (size_t)(a->mode == SPT_MODE_LOCKSTEP ? kSlotsLockstep : kSlots) * (size_t)a->n_particles;
   if (need > c->kbuf_elems) {
     if (c->d_kbuf) (void)hipFree(c->d_kbuf);
     c->d_kbuf = nullptr;
@@ -1325,6 +1719,39 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
   P.ev_cap = a->event_capacity;
   P.ev_count = (unsigned long long*)a->event_count;
   P.queue = c->d_queue;
+  if (a->mode == SPT_MODE_LOCKSTEP) {
+    int rc2 = ensure_lockstep_scratch(c, a->n_particles, a->n_groups);
+    if (rc2) return rc2;
+    LockParams L;
+    L.T = P;
+    L.pgroup = c->d_pgroup;
+    L.active = c->d_active;
+    L.g_t = c->d_gdbl;
+    L.g_h = c->d_gdbl + a->n_groups;
+    L.g_hr = c->d_gdbl + 2 * (size_t)a->n_groups;
+    L.g_next = c->d_gdbl + 3 * (size_t)a->n_groups;
+    L.g_nact = c->d_gdbl + 4 * (size_t)a->n_groups;
+    L.g_flags = c->d_gflags;
+    L.done = c->d_done;
+    int PB = 64;
+    if (a->geom.kind == SPT_GEOM_BEM_POST) {
+      const int64_t want = a->n_particles / (12 * (int64_t)c->n_cu);
+      PB = 1;
+      while (PB * 2 <= want && PB < 64) PB *= 2;
+    }
+    HIPCHK(c, hipEventRecord(c->ev_in, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->own, c->ev_in, 0));
+    int rc3;
+    if (mx <= 64)
+      rc3 = run_lockstep<64>(c, L, a->n_groups, a->n_particles, PB);
+    else if (mx <= 128)
+      rc3 = run_lockstep<128>(c, L, a->n_groups, a->n_particles, PB);
+    else
+      rc3 = run_lockstep<256>(c, L, a->n_groups, a->n_particles, PB);
+    HIPCHK(c, hipEventRecord(c->ev_out, c->own));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_out, 0));
+    return rc3;
+  }
   reset_int_kernel<<<1, 1, 0, c->stream>>>(c->d_queue, 0);
   reset_u64_kernel<<<1, 1, 0, c->stream>>>(P.ev_count);
   switch (T) {

@@ -9,7 +9,11 @@ state and reads back final state, counters and screen events.
 Additions: ``group_sizes`` (uneven groups, a CSR layout instead of
 ``n_emit``-sized chunks), ``block_threads`` (workgroup size), and
 ``last_run`` (per-group attempt/accept counters, particle-stage counts,
-device time).
+device time), and ``mode``: "persistent" (one workgroup integrates whole
+groups from a device queue), "lockstep" (every running group makes one
+attempt per iteration, per-stage kernels replayed from a HIP graph; the
+field gather spreads each particle over many threads) or "auto" (lockstep
+when there are fewer groups than twice the compute units).
 """
 import warnings
 
@@ -34,7 +38,8 @@ class SpecificParticleTracer:
 
     def __init__(self, initial_particles, n_emit, geometry, screens, t_out=None,
                  plummer_radius=DEFAULT_PLUMMER_RADIUS, backend="mi355x", n_workers=1, rtol=None, atol=None,
-                 t_max=None, z_max=None, group_sizes=None, block_threads=0, device=0, max_steps=10_000_000):
+                 t_max=None, z_max=None, group_sizes=None, block_threads=0, device=0, max_steps=10_000_000,
+                 mode="auto"):
         if n_workers != 1:
             raise ValueError(f"n_workers is only supported for backend='cpu' (got backend={backend!r}, "
                              f"n_workers={n_workers!r})")
@@ -63,6 +68,9 @@ class SpecificParticleTracer:
         self.rtol = rtol if rtol is not None else DEFAULT_RTOL
         self.atol = atol if atol is not None else DEFAULT_ATOL
         self.block_threads = int(block_threads)
+        if mode not in ("auto", "persistent", "lockstep"):
+            raise ValueError(f"mode must be 'auto', 'persistent' or 'lockstep', got {mode!r}")
+        self.mode = mode
         self.max_steps = int(max_steps)
         self._setup_state(group_sizes)
         self.t_max = t_max if t_max is not None else self._default_t_max()
@@ -126,6 +134,17 @@ class SpecificParticleTracer:
         sizes = np.diff(self.offsets)
         return np.argsort(-sizes, kind="stable").astype(np.int32)
 
+    def _mode_code(self, ctx):
+        if self.mode == "persistent":
+            return _lib.MODE_PERSISTENT
+        if self.mode == "lockstep":
+            return _lib.MODE_LOCKSTEP
+        import ctypes
+
+        n_cu = ctypes.c_int(0)
+        ctx.check(ctx.lib.spt_device_cus(ctx.h, ctypes.byref(n_cu)), "spt_device_cus")
+        return _lib.MODE_LOCKSTEP if self.n_groups < 2 * n_cu.value else _lib.MODE_PERSISTENT
+
     def trace(self, event_capacity=None):
         """Run spt_trace; returns the raw device results as NumPy arrays."""
         import torch
@@ -172,6 +191,7 @@ class SpecificParticleTracer:
         a.safety, a.min_factor, a.max_factor, a.initial_step_fraction = 0.9, 0.2, 10.0, 1e-6
         a.max_steps = self.max_steps
         a.n_screens = ns
+        a.mode = self._mode_code(ctx)
         for i, z in enumerate(self.screen_positions):
             a.screens[i] = float(z)
         a.attempts, a.accepted, a.active_stages = _lib.ptr(attempts), _lib.ptr(accepted), _lib.ptr(act)

@@ -84,6 +84,17 @@ typedef struct spt_event {
   double vel[3];
 } spt_event;
 
+/* Execution modes of spt_trace.
+ *  PERSISTENT: one persistent kernel; each workgroup integrates whole
+ *    groups taken from a device queue (largest first). Best with many
+ *    groups: no lockstep, no per-stage launches.
+ *  LOCKSTEP: the reference's own schedule (every running group makes one
+ *    attempt per iteration), as per-stage kernels replayed from a captured
+ *    HIP graph; the field gather spreads each particle over many threads,
+ *    so a few groups still fill the GPU. spt_trace blocks until done. */
+#define SPT_MODE_PERSISTENT 0
+#define SPT_MODE_LOCKSTEP 1
+
 /* Group status words written by spt_trace. */
 #define SPT_STATUS_DONE 0
 #define SPT_STATUS_STEPCAP 1
@@ -108,7 +119,7 @@ typedef struct spt_trace_args {
   double safety, min_factor, max_factor, initial_step_fraction;
   int64_t max_steps;
   int32_t n_screens;
-  int32_t pad0;
+  int32_t mode;                 /* SPT_MODE_*                               */
   double screens[SPT_MAX_SCREENS];
   /* outputs, device */
   int64_t* attempts;            /* per group                                */

@@ -133,7 +133,7 @@ def test_bem_assembly_and_solve(torch_cuda):
     np.testing.assert_allclose(s, d["sigma"], rtol=1e-10, atol=1e-11 * np.abs(d["sigma"]).max())
 
 
-def _tracker_for(name, block_threads=0):
+def _tracker_for(name, block_threads=0, mode="auto"):
     from bem_amd import FlatCathode, SpecificParticleTracer
     from bem_amd.bem.geometry import HemisphericalTipBEMGeometry, PostTipBEMGeometry
     from bem_amd.particles import ParticleGroup
@@ -153,14 +153,16 @@ def _tracker_for(name, block_threads=0):
         p = golden("bem_post80.npz")
         geo = PostTipBEMGeometry(-1e8, float(p["R"]), float(p["H"]), n_segments=80, sigma=p["sigma"])
     tr = SpecificParticleTracer(pg, int(d["n_emit"]), geo, list(d["screens"]), rtol=float(d["rtol"]),
-                                atol=float(d["atol"]), t_max=float(d["t_max"]), block_threads=block_threads)
+                                atol=float(d["atol"]), t_max=float(d["t_max"]), block_threads=block_threads,
+                                mode=mode)
     return d, tr
 
 
+@pytest.mark.parametrize("mode", ["persistent", "lockstep"])
 @pytest.mark.parametrize("name", ["c1small", "tipbem", "post80", "c1"])
-def test_trace_matches_reference_runs(torch_cuda, name):
-    _say("trace", name)
-    d, tr = _tracker_for(name)
+def test_trace_matches_reference_runs(torch_cuda, name, mode):
+    _say("trace", name, mode)
+    d, tr = _tracker_for(name, mode=mode)
     screens, _ = tr.run()
     out = tr.last_run
     G = tr.n_groups
@@ -184,7 +186,7 @@ def test_trace_block_sizes_agree(torch_cuda):
     _say("trace block sizes")
     outs = []
     for T in (64, 256, 1024):
-        d, tr = _tracker_for("post80", block_threads=T)
+        d, tr = _tracker_for("post80", block_threads=T, mode="persistent")
         tr.run()
         outs.append(tr.last_run)
     for o in outs[1:]:
@@ -192,7 +194,8 @@ def test_trace_block_sizes_agree(torch_cuda):
         np.testing.assert_allclose(o["pos"], outs[0]["pos"], rtol=1e-11, atol=1e-11 * np.abs(outs[0]["pos"]).max())
 
 
-def test_uneven_groups_match_per_group_oracle(torch_cuda):
+@pytest.mark.parametrize("mode", ["persistent", "lockstep"])
+def test_uneven_groups_match_per_group_oracle(torch_cuda, mode):
     """CSR groups of sizes 1..7: each group equals an oracle run of that
     group alone (groups never interact; RMS normalised by 6 * size)."""
     from bem_amd import FlatCathode, SpecificParticleTracer
@@ -207,7 +210,7 @@ def test_uneven_groups_match_per_group_oracle(torch_cuda):
     src["t"] = np.linspace(0, 4e-14, n)
     pg = ParticleGroup(dict(src, weight=np.full(n, 1.602176634e-19), status=np.ones(n), species="electron"))
     tr = SpecificParticleTracer(pg, None, FlatCathode(-1e8, z0=3e-9), [0.3e-6, 1e-6], rtol=1e-8, t_max=1.2e-12,
-                                group_sizes=sizes)
+                                group_sizes=sizes, mode=mode)
     screens, _ = tr.run()
     out = tr.last_run
     start = 0
