@@ -115,6 +115,11 @@ __global__ void node_table_kernel(const double* prof, const double* sigma, int n
 }
 
 // ------------------------------------------------------- BEM field gather
+// Minimum waves per SIMD requested for the field-gather kernels (caps their
+// VGPR budget at 512 / this).
+#ifndef SPT_FIELD_MIN_WAVES
+#define SPT_FIELD_MIN_WAVES 1
+#endif
 // Near segment (closer than its own length): s = s0 + d sinh(v), Gauss-8 in
 // intervals of width <= 0.5 over [asinh(-s0/d), asinh((L-s0)/d)], density
 // linear between the segment's nodal values (reference
@@ -224,8 +229,16 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
   RingAcc acc;
   acc.zero();
   if (on) {
+    // Work unit = one sub-panel (4 Gauss nodes) of one segment, so slices
+    // stay balanced however n_seg compares with S. The near test is per
+    // segment; the unit with sub == 0 of a near segment integrates that
+    // whole segment with the sinh rule, the segment's other units skip.
     const double re = sh.re[p], ze = sh.ze[p], re2 = re * re;
-    for (int j = s; j < B.n_seg; j += S) {
+    const int n_sub = B.nq >> 2;
+    const int n_units = B.n_seg * n_sub;
+    for (int u = s; u < n_units; u += S) {
+      const int j = u / n_sub;
+      const int sub = u - j * n_sub;
       const double* g = B.segs + (size_t)j * SEG_STRIDE;
       const double L = g[S_L], ur = g[S_UR], uz = g[S_UZ];
       const double dr = re - g[S_P0R], dz = ze - g[S_P0Z];
@@ -234,11 +247,11 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
       const double qr = dr - proj * ur, qz = dz - proj * uz;
       const double dist = sqrt(qr * qr + qz * qz);
       if (dist < L) {
-        near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, acc);
+        if (sub == 0) near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, acc);
       } else {
-        const double4* nd = B.nodes + (size_t)j * B.nq;
-#pragma unroll 1
-        for (int q = 0; q < B.nq; ++q) {
+        const double4* nd = B.nodes + (size_t)u * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
           const double4 v = nd[q];
           ring_accum(re, re2, ze - v.y, v.x, v.z, acc);
         }
@@ -767,8 +780,9 @@ struct FieldShared {
 };
 
 // evaluate_axisymmetric_field over PB points per block.
-template <int T, int NMAX, int PB>
-__global__ __launch_bounds__(T) void bem_pert_kernel(BemView B, const double* pos, int64_t n, double* E) {
+template <int T, int NMAX>
+__global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void bem_pert_kernel(BemView B, const double* pos, int64_t n, double* E,
+                                                                          int PB) {
   __shared__ FieldShared<T, NMAX> sh;
   const int64_t base = (int64_t)blockIdx.x * PB;
   const int m = (int)min((int64_t)PB, n - base);
@@ -789,11 +803,12 @@ __global__ __launch_bounds__(T) void bem_pert_kernel(BemView B, const double* po
 
 // Geometry.field over PB points per block (BEM part compacted to the
 // points that need it).
-template <int T, int NMAX, int PB>
-__global__ __launch_bounds__(T) void geometry_field_kernel(spt_geometry G, BemView B, const double* pos, int64_t n,
-                                                           double* E) {
+template <int T, int NMAX>
+__global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void geometry_field_kernel(spt_geometry G, BemView B,
+                                                                                const double* pos, int64_t n,
+                                                                                double* E, int PB) {
   __shared__ FieldShared<T, NMAX> sh;
-  __shared__ double sx[PB], sy[PB], sz[PB];
+  __shared__ double sx[NMAX], sy[NMAX], sz[NMAX];
   const int64_t base = (int64_t)blockIdx.x * PB;
   const int m_all = (int)min((int64_t)PB, n - base);
   const int tid = threadIdx.x;
@@ -937,7 +952,7 @@ __global__ __launch_bounds__(TG) void ls_init_kernel(LockParams L) {
 // External field at the stage positions of every active particle of every
 // running group; PB particles per block, T/PB threads per particle.
 template <int T, int NMAX>
-__global__ __launch_bounds__(T) void ls_field_kernel(LockParams L, int PB) {
+__global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_field_kernel(LockParams L, int PB) {
   __shared__ FieldShared<T, NMAX> sh;
   __shared__ double sx[NMAX], sy[NMAX], sz[NMAX];
   __shared__ int act[NMAX];
@@ -1467,7 +1482,16 @@ int spt_bem_solve(spt_ctx* c, const double* prof, int32_t n, double E_gun, doubl
 }
 
 #define FIELD_T 256
-#define FIELD_PB 16
+#define FIELD_NMAX 64
+
+// Points per block for a field gather over n points: enough blocks for
+// about three resident workgroups per compute unit, at most FIELD_NMAX.
+static int field_points_per_block(const spt_ctx* c, int64_t n) {
+  const int64_t want = n / (12 * (int64_t)c->n_cu);
+  int PB = 1;
+  while (PB * 2 <= want && PB < FIELD_NMAX) PB *= 2;
+  return PB;
+}
 
 int spt_bem_pert_field(spt_ctx* c, const double* pos, int64_t n, double* E) {
   if (!c) return SPT_EINVAL;
@@ -1475,8 +1499,9 @@ int spt_bem_pert_field(spt_ctx* c, const double* pos, int64_t n, double* E) {
   if (n < 0 || (n > 0 && (!pos || !E))) return fail(c, SPT_EINVAL, "spt_bem_pert_field: bad arguments");
   if (n == 0) return SPT_OK;
   HIPCHK(c, hipSetDevice(c->device));
-  const unsigned nb = (unsigned)((n + FIELD_PB - 1) / FIELD_PB);
-  bem_pert_kernel<FIELD_T, FIELD_PB, FIELD_PB><<<nb, FIELD_T, 0, c->stream>>>(bem_view(c), pos, n, E);
+  const int PB = field_points_per_block(c, n);
+  const unsigned nb = (unsigned)((n + PB - 1) / PB);
+  bem_pert_kernel<FIELD_T, FIELD_NMAX><<<nb, FIELD_T, 0, c->stream>>>(bem_view(c), pos, n, E, PB);
   HIPCHK(c, hipGetLastError());
   return SPT_OK;
 }
@@ -1500,8 +1525,9 @@ int spt_geometry_field(spt_ctx* c, const spt_geometry* g, const double* pos, int
   if (n < 0 || (n > 0 && (!pos || !E))) return fail(c, SPT_EINVAL, "spt_geometry_field: bad arguments");
   if (n == 0) return SPT_OK;
   HIPCHK(c, hipSetDevice(c->device));
-  const unsigned nb = (unsigned)((n + FIELD_PB - 1) / FIELD_PB);
-  geometry_field_kernel<FIELD_T, FIELD_PB, FIELD_PB><<<nb, FIELD_T, 0, c->stream>>>(*g, bem_view(c), pos, n, E);
+  const int PB = g->kind == SPT_GEOM_BEM_POST ? field_points_per_block(c, n) : FIELD_NMAX;
+  const unsigned nb = (unsigned)((n + PB - 1) / PB);
+  geometry_field_kernel<FIELD_T, FIELD_NMAX><<<nb, FIELD_T, 0, c->stream>>>(*g, bem_view(c), pos, n, E, PB);
   HIPCHK(c, hipGetLastError());
   return SPT_OK;
 }
@@ -1733,12 +1759,7 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
     L.g_nact = c->d_gdbl + 4 * (size_t)a->n_groups;
     L.g_flags = c->d_gflags;
     L.done = c->d_done;
-    int PB = 64;
-    if (a->geom.kind == SPT_GEOM_BEM_POST) {
-      const int64_t want = a->n_particles / (12 * (int64_t)c->n_cu);
-      PB = 1;
-      while (PB * 2 <= want && PB < 64) PB *= 2;
-    }
+    const int PB = a->geom.kind == SPT_GEOM_BEM_POST ? field_points_per_block(c, a->n_particles) : FIELD_NMAX;
     HIPCHK(c, hipEventRecord(c->ev_in, c->stream));
     HIPCHK(c, hipStreamWaitEvent(c->own, c->ev_in, 0));
     int rc3;
