@@ -24,6 +24,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <cmath>
 #include <string>
 
 #include "../../include/spt.h"
@@ -229,31 +230,54 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
   RingAcc acc;
   acc.zero();
   if (on) {
-    // Work unit = one sub-panel (4 Gauss nodes) of one segment, so slices
-    // stay balanced however n_seg compares with S. The near test is per
-    // segment; the unit with sub == 0 of a near segment integrates that
-    // whole segment with the sinh rule, the segment's other units skip.
     const double re = sh.re[p], ze = sh.ze[p], re2 = re * re;
-    const int n_sub = B.nq >> 2;
-    const int n_units = B.n_seg * n_sub;
-    for (int u = s; u < n_units; u += S) {
-      const int j = u / n_sub;
-      const int sub = u - j * n_sub;
-      const double* g = B.segs + (size_t)j * SEG_STRIDE;
-      const double L = g[S_L], ur = g[S_UR], uz = g[S_UZ];
-      const double dr = re - g[S_P0R], dz = ze - g[S_P0Z];
-      const double along = dr * ur + dz * uz;
-      const double proj = fmin(fmax(along, 0.0), L);
-      const double qr = dr - proj * ur, qz = dz - proj * uz;
-      const double dist = sqrt(qr * qr + qz * qz);
-      if (dist < L) {
-        if (sub == 0) near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, acc);
-      } else {
-        const double4* nd = B.nodes + (size_t)u * 4;
+    if (B.n_seg >= 4 * S) {
+      // Many segments per slice: stride over whole segments.
+      for (int j = s; j < B.n_seg; j += S) {
+        const double* g = B.segs + (size_t)j * SEG_STRIDE;
+        const double L = g[S_L], ur = g[S_UR], uz = g[S_UZ];
+        const double dr = re - g[S_P0R], dz = ze - g[S_P0Z];
+        const double along = dr * ur + dz * uz;
+        const double proj = fmin(fmax(along, 0.0), L);
+        const double qr = dr - proj * ur, qz = dz - proj * uz;
+        const double dist = sqrt(qr * qr + qz * qz);
+        if (dist < L) {
+          near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, acc);
+        } else {
+          const double4* nd = B.nodes + (size_t)j * B.nq;
+#pragma unroll 1
+          for (int q = 0; q < B.nq; ++q) {
+            const double4 v = nd[q];
+            ring_accum(re, re2, ze - v.y, v.x, v.z, acc);
+          }
+        }
+      }
+    } else {
+      // Few segments per slice: stride over sub-panels (4 Gauss nodes) so
+      // slices stay balanced. The near test is per segment; the unit with
+      // sub == 0 of a near segment integrates that whole segment with the
+      // sinh rule, the segment's other units skip.
+      const int n_sub = B.nq >> 2;
+      const int n_units = B.n_seg * n_sub;
+      for (int u = s; u < n_units; u += S) {
+        const int j = u / n_sub;
+        const int sub = u - j * n_sub;
+        const double* g = B.segs + (size_t)j * SEG_STRIDE;
+        const double L = g[S_L], ur = g[S_UR], uz = g[S_UZ];
+        const double dr = re - g[S_P0R], dz = ze - g[S_P0Z];
+        const double along = dr * ur + dz * uz;
+        const double proj = fmin(fmax(along, 0.0), L);
+        const double qr = dr - proj * ur, qz = dz - proj * uz;
+        const double dist = sqrt(qr * qr + qz * qz);
+        if (dist < L) {
+          if (sub == 0) near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, acc);
+        } else {
+          const double4* nd = B.nodes + (size_t)u * 4;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const double4 v = nd[q];
-          ring_accum(re, re2, ze - v.y, v.x, v.z, acc);
+          for (int q = 0; q < 4; ++q) {
+            const double4 v = nd[q];
+            ring_accum(re, re2, ze - v.y, v.x, v.z, acc);
+          }
         }
       }
     }
@@ -1589,9 +1613,36 @@ static int launch_trace(spt_ctx* c, const TraceParams& P, int n_groups) {
 
 constexpr int kLockstepUnroll = 4;  // attempts per captured graph replay
 
+// Field-gather launch shape for n points: points per block and block size
+// (192 or 256 threads). At three resident waves per SIMD a compute unit
+// holds four 3-wave or three 4-wave blocks; pick the shape whose block
+// count fills whole rounds best (e.g. 1024 points on 256 CUs: 1024
+// one-point blocks of 192 threads are exactly one round).
+struct FieldShape {
+  int T, PB;
+};
+static FieldShape field_shape(const spt_ctx* c, int64_t n, bool bem) {
+  if (!bem) return {256, FIELD_NMAX};
+  const int PB = field_points_per_block(c, n);
+  const int64_t blocks = (n + PB - 1) / PB;
+  double best = -1.0;
+  FieldShape out{256, PB};
+  for (int T : {192, 256}) {
+    const int64_t cap = (int64_t)c->n_cu * (12 / (T / 64));
+    const double rounds = (double)blocks / (double)cap;
+    const double eff = rounds / std::ceil(rounds);
+    if (eff > best + 1e-9) {
+      best = eff;
+      out = {T, PB};
+    }
+  }
+  return out;
+}
+
 template <int TG>
-static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, int PB) {
+static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, FieldShape fs) {
   hipStream_t s = c->own;
+  const int PB = fs.PB;
   const unsigned nb_field = (unsigned)((n + PB - 1) / PB);
   ls_pgroup_kernel<<<(n_groups + 255) / 256, 256, 0, s>>>(L.T.offsets, n_groups, (int32_t*)L.pgroup);
   reset_u32_kernel<<<1, 1, 0, s>>>(L.done);
@@ -1603,7 +1654,10 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, int P
   HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   for (int u = 0; u < kLockstepUnroll; ++u) {
     for (int st = 0; st < 7; ++st) {
-      ls_field_kernel<256, 64><<<nb_field, 256, 0, s>>>(L, PB);
+      if (fs.T == 192)
+        ls_field_kernel<192, 64><<<nb_field, 192, 0, s>>>(L, PB);
+      else
+        ls_field_kernel<256, 64><<<nb_field, 256, 0, s>>>(L, PB);
       ls_accel_kernel<TG><<<n_groups, TG, 0, s>>>(L, st);
     }
     ls_end_kernel<TG><<<n_groups, TG, 0, s>>>(L);
@@ -1759,16 +1813,16 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
     L.g_nact = c->d_gdbl + 4 * (size_t)a->n_groups;
     L.g_flags = c->d_gflags;
     L.done = c->d_done;
-    const int PB = a->geom.kind == SPT_GEOM_BEM_POST ? field_points_per_block(c, a->n_particles) : FIELD_NMAX;
+    const FieldShape fs = field_shape(c, a->n_particles, a->geom.kind == SPT_GEOM_BEM_POST);
     HIPCHK(c, hipEventRecord(c->ev_in, c->stream));
     HIPCHK(c, hipStreamWaitEvent(c->own, c->ev_in, 0));
     int rc3;
     if (mx <= 64)
-      rc3 = run_lockstep<64>(c, L, a->n_groups, a->n_particles, PB);
+      rc3 = run_lockstep<64>(c, L, a->n_groups, a->n_particles, fs);
     else if (mx <= 128)
-      rc3 = run_lockstep<128>(c, L, a->n_groups, a->n_particles, PB);
+      rc3 = run_lockstep<128>(c, L, a->n_groups, a->n_particles, fs);
     else
-      rc3 = run_lockstep<256>(c, L, a->n_groups, a->n_particles, PB);
+      rc3 = run_lockstep<256>(c, L, a->n_groups, a->n_particles, fs);
     HIPCHK(c, hipEventRecord(c->ev_out, c->own));
     HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_out, 0));
     return rc3;
