@@ -23,6 +23,7 @@ GEOM_FLAT, GEOM_TIP, GEOM_BEM_POST = 0, 1, 2
 IMAGE_NONE, IMAGE_PLANE, IMAGE_TIP3 = 0, 1, 2
 STATUS_DONE, STATUS_STEPCAP = 0, 1
 MODE_PERSISTENT, MODE_LOCKSTEP = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
 
 EXPORTS = (
     "spt_create", "spt_destroy", "spt_last_error", "spt_set_stream", "spt_sync", "spt_device_cus",
@@ -70,6 +71,7 @@ class TraceArgs_t(ctypes.Structure):
         ("initial_step_fraction", ctypes.c_double),
         ("max_steps", ctypes.c_int64),
         ("n_screens", ctypes.c_int32), ("mode", ctypes.c_int32),
+        ("precision", ctypes.c_int32), ("pad0", ctypes.c_int32),
         ("screens", ctypes.c_double * SPT_MAX_SCREENS),
         ("attempts", _p), ("accepted", _p), ("active_stages", _p), ("margin", _p), ("status", _p),
         ("events", _p), ("event_capacity", ctypes.c_int64), ("event_count", _p),
@@ -101,10 +103,10 @@ def load():
             "spt_bem_assemble": (ctypes.c_int, [_p, _p, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _p]),
             "spt_bem_solve": (ctypes.c_int, [_p, _p, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_int32, _p]),
-            "spt_bem_pert_field": (ctypes.c_int, [_p, _p, ctypes.c_int64, _p]),
-            "spt_geometry_field": (ctypes.c_int, [_p, P(Geometry_t), _p, ctypes.c_int64, _p]),
+            "spt_bem_pert_field": (ctypes.c_int, [_p, _p, ctypes.c_int64, _p, ctypes.c_int32]),
+            "spt_geometry_field": (ctypes.c_int, [_p, P(Geometry_t), _p, ctypes.c_int64, _p, ctypes.c_int32]),
             "spt_pair_forces": (ctypes.c_int, [_p, P(Geometry_t), _p, _p, _p, _p, ctypes.c_int32,
-                                               ctypes.c_double, _p, _p]),
+                                               ctypes.c_double, _p, _p, ctypes.c_int32]),
             "spt_kill_mask": (ctypes.c_int, [_p, P(Geometry_t), _p, ctypes.c_int64, _p]),
             "spt_trace": (ctypes.c_int, [_p, P(TraceArgs_t)]),
         }

@@ -44,7 +44,7 @@ class AxisymmetricBEMSolution:
     def solve(cls, profile, E_gun, refine_ratio=1.0, max_depth=20, device=0):
         return cls(profile, solve_sigma(profile, E_gun, refine_ratio, max_depth, device), device)
 
-    def field(self, points, n_subdiv=8):
+    def field(self, points, n_subdiv=8, precision="fp64"):
         import torch
         from ..fields import _to_device, _from_device
 
@@ -58,7 +58,9 @@ class AxisymmetricBEMSolution:
         pos, meta = _to_device(points, ctx)
         out = torch.empty_like(pos)
         ctx.bind_stream()
-        ctx.check(ctx.lib.spt_bem_pert_field(ctx.h, _lib.ptr(pos), pos.shape[0], _lib.ptr(out)),
-                  "spt_bem_pert_field")
+        from ..geometry import _precision_code
+
+        ctx.check(ctx.lib.spt_bem_pert_field(ctx.h, _lib.ptr(pos), pos.shape[0], _lib.ptr(out),
+                                             _precision_code(precision)), "spt_bem_pert_field")
         ctx.sync()
         return _from_device(out, meta)

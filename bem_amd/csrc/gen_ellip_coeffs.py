@@ -7,7 +7,8 @@ k'^2 = p):
     K(1-p) = PK(p) - ln(p) * QK(p)
     E(1-p) = PE(p) - ln(p) * p * QE(p)
 
-PK, QK, PE, QE are degree-10 polynomials. They are fitted jointly by
+PK, QK, PE, QE are degree-10 polynomials for FP64. A second, degree-6 pair
+PK32/QK32 serves the FP32 (mixed-precision) field path. They are fitted jointly by
 weighted (relative-error) linear least squares at 50 significant digits,
 with two Lawson reweighting passes toward the minimax solution. The
 target values come from the DLMF 19.12 series themselves for p < 1/4
@@ -60,7 +61,8 @@ def fit_points():
     return xs
 
 
-def joint_fit(fun, q_shift, xs, weights):
+def joint_fit(fun, q_shift, xs, weights, deg=None):
+    DEG = deg if deg is not None else globals()["DEG"]
     m, n = len(xs), 2 * (DEG + 1)
     A = mp.matrix(m, n)
     b = mp.matrix(m, 1)
@@ -76,12 +78,12 @@ def joint_fit(fun, q_shift, xs, weights):
     return [sol[j] for j in range(DEG + 1)], [sol[DEG + 1 + j] for j in range(DEG + 1)]
 
 
-def fit(fun, q_shift):
+def fit(fun, q_shift, deg=None):
     xs = fit_points()
     fx = [fun(x) for x in xs]
     weights = [mp.mpf(1)] * len(xs)
     for _ in range(3):
-        P, Q = joint_fit(fun, q_shift, xs, weights)
+        P, Q = joint_fit(fun, q_shift, xs, weights, deg)
         errs = []
         for x, f in zip(xs, fx):
             v = mp.polyval(P[::-1], x) - mp.log(x) * x**q_shift * mp.polyval(Q[::-1], x)
@@ -109,7 +111,36 @@ def check(P, Q, q_shift, fun):
     return worst / 2.0**-52
 
 
+def check32(P, Q):
+    """Worst K and E (Legendre-derivative form) error of the FP32 pair,
+    evaluated in float32, in units of 2^-23 relative."""
+    P32, Q32 = np.float32(P), np.float32(Q)
+
+    def hd(c, x):
+        s, d = np.float32(c[-1]), np.float32(0)
+        for a in reversed(c[:-1]):
+            d = np.float32(d * x + s)
+            s = np.float32(s * x + a)
+        return s, d
+
+    rng = np.random.default_rng(0)
+    ps = np.concatenate([10 ** rng.uniform(-30, 0, 800), rng.uniform(0, 1, 800), [1.0]]).astype(np.float32)
+    wk = we = 0.0
+    for p in ps:
+        Pv, dP = hd(P32, p)
+        Qv, dQ = hd(Q32, p)
+        lp = np.float32(np.log(p))
+        K = np.float32(Pv - lp * Qv)
+        pkp = np.float32(p * dP - Qv - (p * lp) * dQ)
+        E = np.float32(p * K - np.float32(2) * (np.float32(1) - p) * pkp)
+        wk = max(wk, float(abs(K / k_of_complement(mp.mpf(float(p))) - 1)))
+        we = max(we, float(abs(E / e_of_complement(mp.mpf(float(p))) - 1)))
+    return wk / 2.0**-23, we / 2.0**-23
+
+
 def main():
+    PK32, QK32 = fit(k_of_complement, 0, deg=6)
+    ulp32 = check32(PK32, QK32)
     PK, QK = fit(k_of_complement, 0)
     PE, QE = fit(e_of_complement, 1)
     ulp_k = check(PK, QK, 0, k_of_complement)
@@ -122,6 +153,11 @@ def main():
         f.write("#pragma once\n")
         for name, c in (("PK", PK), ("QK", QK), ("PE", PE), ("QE", QE)):
             body = ", ".join(repr(x) for x in c)
+            f.write(f"#define SPT_ELLIP_{name} {{ {body} }}\n")
+        f.write(f"// FP32 pair (degree 6), float32 evaluation: K {ulp32[0]:.2f}, E via dK/dp {ulp32[1]:.2f} "
+                "(units of 2^-23 rel).\n")
+        for name, c in (("PK32", PK32), ("QK32", QK32)):
+            body = ", ".join(repr(float(np.float32(x))) + "f" for x in c)
             f.write(f"#define SPT_ELLIP_{name} {{ {body} }}\n")
     print(f"wrote {out}: K {ulp_k:.2f} ulp, E {ulp_e:.2f} ulp", file=sys.stderr)
 

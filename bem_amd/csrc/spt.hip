@@ -170,7 +170,7 @@ struct BemShared {
 // points 1e-4 L off the surface (554-557), then per segment either its far
 // Gauss nodes or its near sinh rule. Partial sums are combined in slice
 // order, so results are deterministic. m must be block-uniform.
-template <int T, int NMAX>
+template <int T, int NMAX, int PR>
 __device__ void bem_pert_block(const BemView& B, int m, const double* px, const double* py, const double* pz,
                                double* ox, double* oy, double* oz, BemShared<T, NMAX>& sh) {
   if (m <= 0) return;
@@ -229,6 +229,8 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
   __syncthreads();
   RingAcc acc;
   acc.zero();
+  RingAcc32 acc32;
+  acc32.zero();
   if (on) {
     const double re = sh.re[p], ze = sh.ze[p], re2 = re * re;
     if (B.n_seg >= 4 * S) {
@@ -243,6 +245,13 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
         const double dist = sqrt(qr * qr + qz * qz);
         if (dist < L) {
           near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, acc);
+        } else if (PR == SPT_PREC_MIXED && dist >= 2.0 * L) {
+          const double4* nd = B.nodes + (size_t)j * B.nq;
+#pragma unroll 1
+          for (int q = 0; q < B.nq; ++q) {
+            const double4 v = nd[q];
+            ring_accum32(re, re2, ze - v.y, v.x, (float)v.z, acc32, acc);
+          }
         } else {
           const double4* nd = B.nodes + (size_t)j * B.nq;
 #pragma unroll 1
@@ -271,6 +280,13 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
         const double dist = sqrt(qr * qr + qz * qz);
         if (dist < L) {
           if (sub == 0) near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, acc);
+        } else if (PR == SPT_PREC_MIXED && dist >= 2.0 * L) {
+          const double4* nd = B.nodes + (size_t)u * 4;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const double4 v = nd[q];
+            ring_accum32(re, re2, ze - v.y, v.x, (float)v.z, acc32, acc);
+          }
         } else {
           const double4* nd = B.nodes + (size_t)u * 4;
 #pragma unroll
@@ -280,6 +296,10 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
           }
         }
       }
+    }
+    if (PR == SPT_PREC_MIXED) {
+      acc.er_main += acc32.er.value();
+      acc.ez_main += acc32.ez.value();
     }
   }
   __syncthreads();  // bd2/bseg reads done before acc (aliased) is written
@@ -400,6 +420,75 @@ __device__ __forceinline__ void pair_term(double xi, double yi, double zi, doubl
   fz += c * dz;
 }
 
+// FP32 pair term with Kahan accumulation: separations in FP64, then
+// rounded; k q_i q_j rounded once.
+__device__ __forceinline__ void pair_term32(double xi, double yi, double zi, double qi, double xj, double yj,
+                                            double zj, double qj, double a2, Kahan32& fx, Kahan32& fy,
+                                            Kahan32& fz) {
+  const float dx = (float)(xi - xj), dy = (float)(yi - yj), dz = (float)(zi - zj);
+  const float r2 = dx * dx + dy * dy + dz * dz + (float)a2;
+  const float r3 = r2 * sqrtf(r2);
+  const float c = (float)(kCoulomb * qi * qj) / r3;
+  fx.add(c * dx);
+  fy.add(c * dy);
+  fz.add(c * dz);
+}
+
+// In-group Coulomb and image forces on particle i from LDS arrays, FP32
+// with compensated sums (mixed precision).
+__device__ void pair_forces_i32(const spt_geometry& G, int i, int n, const double* sx, const double* sy,
+                                const double* sz, const double* sq, const int* act, double a2, double* Fc,
+                                double* Fi) {
+  Fc[0] = Fc[1] = Fc[2] = 0.0;
+  Fi[0] = Fi[1] = Fi[2] = 0.0;
+  if (!act[i]) return;
+  const double xi = sx[i], yi = sy[i], zi = sz[i], qi = sq[i];
+  Kahan32 cx, cy, cz;
+  cx.zero();
+  cy.zero();
+  cz.zero();
+  for (int j = 0; j < n; ++j) {
+    if (!act[j]) continue;
+    pair_term32(xi, yi, zi, qi, sx[j], sy[j], sz[j], sq[j], a2, cx, cy, cz);
+  }
+  Fc[0] = cx.value();
+  Fc[1] = cy.value();
+  Fc[2] = cz.value();
+  if (G.image == SPT_IMAGE_NONE) return;
+  Kahan32 ix, iy, iz;
+  ix.zero();
+  iy.zero();
+  iz.zero();
+  if (G.image == SPT_IMAGE_PLANE) {
+    for (int j = 0; j < n; ++j) {
+      if (!act[j]) continue;
+      pair_term32(xi, yi, zi, qi, sx[j], sy[j], -2.0 * G.z0 - sz[j], -sq[j], a2, ix, iy, iz);
+    }
+  } else {
+    const double a = G.R - G.z0;
+    for (int j = 0; j < n; ++j) {
+      if (!act[j]) continue;
+      const double xj = sx[j], yj = sy[j], zj = sz[j], qj = sq[j];
+      double d = sqrt(xj * xj + yj * yj + zj * zj);
+      d = d > 0.0 ? d : a;
+      const double sc = (a * a) / (d * d);
+      const double six = xj * sc, siy = yj * sc, siz = zj * sc;
+      const double siq = -qj * (a / d);
+      pair_term32(xi, yi, zi, qi, six, siy, siz, siq, a2, ix, iy, iz);
+      pair_term32(xi, yi, zi, qi, xj, yj, -2.0 * G.z0 - zj, -qj, a2, ix, iy, iz);
+      pair_term32(xi, yi, zi, qi, six, siy, -2.0 * G.z0 - siz, -siq, a2, ix, iy, iz);
+    }
+  }
+  Fi[0] = ix.value();
+  Fi[1] = iy.value();
+  Fi[2] = iz.value();
+}
+
+template <int PR>
+__device__ __forceinline__ void pair_forces_p(const spt_geometry& G, int i, int n, const double* sx,
+                                              const double* sy, const double* sz, const double* sq, const int* act,
+                                              double a2, double* Fc, double* Fi);
+
 // In-group Coulomb and image forces on particle i from LDS arrays
 // (reference forces.py:32-191). Inactive partners contribute nothing.
 __device__ void pair_forces_i(const spt_geometry& G, int i, int n, const double* sx, const double* sy,
@@ -434,6 +523,16 @@ __device__ void pair_forces_i(const spt_geometry& G, int i, int n, const double*
     }
     for (int c = 0; c < 3; ++c) Fi[c] = f1[c] + f2[c] + f3[c];
   }
+}
+
+template <int PR>
+__device__ __forceinline__ void pair_forces_p(const spt_geometry& G, int i, int n, const double* sx,
+                                              const double* sy, const double* sz, const double* sq, const int* act,
+                                              double a2, double* Fc, double* Fi) {
+  if (PR == SPT_PREC_MIXED)
+    pair_forces_i32(G, i, n, sx, sy, sz, sq, act, a2, Fc, Fi);
+  else
+    pair_forces_i(G, i, n, sx, sy, sz, sq, act, a2, Fc, Fi);
 }
 
 // ----------------------------------------------------------- screens
@@ -577,7 +676,7 @@ struct TraceShared {
 
 // External field of every active particle of the group at the stage
 // positions in LDS (sx, sy, sz) into (fx, fy, fz).
-template <int T, int NMAX>
+template <int T, int NMAX, int PR>
 __device__ void stage_field(const TraceParams& P, int n, TraceShared<T, NMAX>& sh) {
   const int tid = threadIdx.x;
   if (P.G.kind != SPT_GEOM_BEM_POST) {
@@ -602,7 +701,7 @@ __device__ void stage_field(const TraceParams& P, int n, TraceShared<T, NMAX>& s
     sh.cz[tid] = sh.sz[i];
   }
   __syncthreads();
-  bem_pert_block<T, NMAX>(P.bem, m, sh.cx, sh.cy, sh.cz, sh.ox, sh.oy, sh.oz, sh.bem);
+  bem_pert_block<T, NMAX, PR>(P.bem, m, sh.cx, sh.cy, sh.cz, sh.ox, sh.oy, sh.oz, sh.bem);
   if (tid < m) {
     const int i = sh.cidx[tid];
     geometry_field_point(P.G, sh.sx[i], sh.sy[i], sh.sz[i], sh.ox[tid], sh.oy[tid], sh.oz[tid], sh.fx[i], sh.fy[i],
@@ -624,7 +723,7 @@ constexpr int kSlots = 48;
 // Particle state lives in global memory (caller's pos/vel plus the stage
 // scratch) so that only a handful of scalars are live across the field
 // gather.
-template <int T, int NMAX>
+template <int T, int NMAX, int PR>
 __device__ void run_group(const TraceParams& P, int g, TraceShared<T, NMAX>& sh) {
   const int tid = threadIdx.x;
   const int64_t off = P.offsets[g];
@@ -687,10 +786,10 @@ __device__ void run_group(const TraceParams& P, int g, TraceShared<T, NMAX>& sh)
         for (int c = 0; c < 3; ++c) KB(st * 6 + c) = active ? vs[c] : 0.0;
       }
       __syncthreads();
-      stage_field<T, NMAX>(P, n, sh);
+      stage_field<T, NMAX, PR>(P, n, sh);
       if (mine) {
         double Fc[3], Fi[3];
-        pair_forces_i(P.G, tid, n, sh.sx, sh.sy, sh.sz, sh.sq, sh.sact, P.a2, Fc, Fi);
+        pair_forces_p<PR>(P.G, tid, n, sh.sx, sh.sy, sh.sz, sh.sq, sh.sact, P.a2, Fc, Fi);
         const double mass = P.mass[gi];
         const double F[3] = {sh.fx[tid], sh.fy[tid], sh.fz[tid]};
         for (int c = 0; c < 3; ++c) {
@@ -772,7 +871,7 @@ __device__ void run_group(const TraceParams& P, int g, TraceShared<T, NMAX>& sh)
   __syncthreads();
 }
 
-template <int T, int NMAX>
+template <int T, int NMAX, int PR>
 __global__ __launch_bounds__(T) void trace_kernel(TraceParams P) {
   __shared__ TraceShared<T, NMAX> sh;
   while (true) {
@@ -782,7 +881,7 @@ __global__ __launch_bounds__(T) void trace_kernel(TraceParams P) {
     __syncthreads();
     if (q >= P.n_groups) break;
     const int g = P.order ? P.order[q] : q;
-    run_group<T, NMAX>(P, g, sh);
+    run_group<T, NMAX, PR>(P, g, sh);
   }
 }
 
@@ -804,7 +903,7 @@ struct FieldShared {
 };
 
 // evaluate_axisymmetric_field over PB points per block.
-template <int T, int NMAX>
+template <int T, int NMAX, int PR>
 __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void bem_pert_kernel(BemView B, const double* pos, int64_t n, double* E,
                                                                           int PB) {
   __shared__ FieldShared<T, NMAX> sh;
@@ -817,7 +916,7 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void bem_pert_kernel(BemVie
     sh.pz[tid] = pos[3 * (base + tid) + 2];
   }
   __syncthreads();
-  bem_pert_block<T, NMAX>(B, m, sh.px, sh.py, sh.pz, sh.ox, sh.oy, sh.oz, sh.bem);
+  bem_pert_block<T, NMAX, PR>(B, m, sh.px, sh.py, sh.pz, sh.ox, sh.oy, sh.oz, sh.bem);
   if (tid < m) {
     E[3 * (base + tid)] = sh.ox[tid];
     E[3 * (base + tid) + 1] = sh.oy[tid];
@@ -827,7 +926,7 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void bem_pert_kernel(BemVie
 
 // Geometry.field over PB points per block (BEM part compacted to the
 // points that need it).
-template <int T, int NMAX>
+template <int T, int NMAX, int PR>
 __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void geometry_field_kernel(spt_geometry G, BemView B,
                                                                                 const double* pos, int64_t n,
                                                                                 double* E, int PB) {
@@ -858,7 +957,7 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void geometry_field_kernel(
     sh.pz[tid] = sz[i];
   }
   __syncthreads();
-  bem_pert_block<T, NMAX>(B, m, sh.px, sh.py, sh.pz, sh.ox, sh.oy, sh.oz, sh.bem);
+  bem_pert_block<T, NMAX, PR>(B, m, sh.px, sh.py, sh.pz, sh.ox, sh.oy, sh.oz, sh.bem);
   if (tid < m) {
     const int i = sh.cidx[tid];
     double ex, ey, ez;
@@ -975,7 +1074,7 @@ __global__ __launch_bounds__(TG) void ls_init_kernel(LockParams L) {
 
 // External field at the stage positions of every active particle of every
 // running group; PB particles per block, T/PB threads per particle.
-template <int T, int NMAX>
+template <int T, int NMAX, int PR>
 __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_field_kernel(LockParams L, int PB) {
   __shared__ FieldShared<T, NMAX> sh;
   __shared__ double sx[NMAX], sy[NMAX], sz[NMAX];
@@ -1009,7 +1108,7 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_field_kernel(LockPa
     sh.pz[tid] = sz[i];
   }
   __syncthreads();
-  bem_pert_block<T, NMAX>(L.T.bem, m, sh.px, sh.py, sh.pz, sh.ox, sh.oy, sh.oz, sh.bem);
+  bem_pert_block<T, NMAX, PR>(L.T.bem, m, sh.px, sh.py, sh.pz, sh.ox, sh.oy, sh.oz, sh.bem);
   if (tid < m) {
     const int i = sh.cidx[tid];
     const int64_t gi = base + i;
@@ -1031,7 +1130,7 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_field_kernel(LockPa
 
 // Acceleration of stage st for one group (reference geometry.py:289-299,
 // batched_rk45.py:362-363), then the next stage's state.
-template <int TG>
+template <int TG, int PR>
 __global__ __launch_bounds__(TG) void ls_accel_kernel(LockParams L, int st) {
   __shared__ double sx[TG], sy[TG], sz[TG], sq[TG];
   __shared__ int act[TG];
@@ -1054,7 +1153,7 @@ __global__ __launch_bounds__(TG) void ls_accel_kernel(LockParams L, int st) {
   const bool active = act[tid];
   const double hr = L.g_hr[g];
   double Fc[3], Fi[3];
-  pair_forces_i(L.T.G, tid, n, sx, sy, sz, sq, act, L.T.a2, Fc, Fi);
+  pair_forces_p<PR>(L.T.G, tid, n, sx, sy, sz, sq, act, L.T.a2, Fc, Fi);
   const double mass = L.T.mass[gi];
   for (int c = 0; c < 3; ++c) {
     const double ac = n == 1 ? 0.0 : Fc[c] / mass;
@@ -1170,7 +1269,7 @@ __global__ __launch_bounds__(TG) void ls_end_kernel(LockParams L) {
 #undef LKB
 
 // In-group pair forces: one block per group, threads over its particles.
-template <int NMAX>
+template <int NMAX, int PR>
 __global__ __launch_bounds__(NMAX) void pair_forces_kernel(spt_geometry G, const double* pos, const double* charge,
                                                            const uint8_t* active, const int64_t* offsets, double a2,
                                                            double* Fc_out, double* Fi_out) {
@@ -1190,7 +1289,7 @@ __global__ __launch_bounds__(NMAX) void pair_forces_kernel(spt_geometry G, const
   __syncthreads();
   if (i < n) {
     double Fc[3], Fi[3];
-    pair_forces_i(G, i, n, sx, sy, sz, sq, act, a2, Fc, Fi);
+    pair_forces_p<PR>(G, i, n, sx, sy, sz, sq, act, a2, Fc, Fi);
     for (int c = 0; c < 3; ++c) {
       if (Fc_out) Fc_out[3 * (off + i) + c] = Fc[c];
       if (Fi_out) Fi_out[3 * (off + i) + c] = Fi[c];
@@ -1517,7 +1616,7 @@ static int field_points_per_block(const spt_ctx* c, int64_t n) {
   return PB;
 }
 
-int spt_bem_pert_field(spt_ctx* c, const double* pos, int64_t n, double* E) {
+int spt_bem_pert_field(spt_ctx* c, const double* pos, int64_t n, double* E, int32_t precision) {
   if (!c) return SPT_EINVAL;
   if (!c->bem_nodes) return fail(c, SPT_ENOBEM, "no BEM table uploaded");
   if (n < 0 || (n > 0 && (!pos || !E))) return fail(c, SPT_EINVAL, "spt_bem_pert_field: bad arguments");
@@ -1525,7 +1624,10 @@ int spt_bem_pert_field(spt_ctx* c, const double* pos, int64_t n, double* E) {
   HIPCHK(c, hipSetDevice(c->device));
   const int PB = field_points_per_block(c, n);
   const unsigned nb = (unsigned)((n + PB - 1) / PB);
-  bem_pert_kernel<FIELD_T, FIELD_NMAX><<<nb, FIELD_T, 0, c->stream>>>(bem_view(c), pos, n, E, PB);
+  if (precision == SPT_PREC_MIXED)
+    bem_pert_kernel<FIELD_T, FIELD_NMAX, SPT_PREC_MIXED><<<nb, FIELD_T, 0, c->stream>>>(bem_view(c), pos, n, E, PB);
+  else
+    bem_pert_kernel<FIELD_T, FIELD_NMAX, SPT_PREC_FP64><<<nb, FIELD_T, 0, c->stream>>>(bem_view(c), pos, n, E, PB);
   HIPCHK(c, hipGetLastError());
   return SPT_OK;
 }
@@ -1542,7 +1644,8 @@ static int check_geom(spt_ctx* c, const spt_geometry* g) {
   return SPT_OK;
 }
 
-int spt_geometry_field(spt_ctx* c, const spt_geometry* g, const double* pos, int64_t n, double* E) {
+int spt_geometry_field(spt_ctx* c, const spt_geometry* g, const double* pos, int64_t n, double* E,
+                       int32_t precision) {
   if (!c) return SPT_EINVAL;
   int rc = check_geom(c, g);
   if (rc) return rc;
@@ -1551,13 +1654,18 @@ int spt_geometry_field(spt_ctx* c, const spt_geometry* g, const double* pos, int
   HIPCHK(c, hipSetDevice(c->device));
   const int PB = g->kind == SPT_GEOM_BEM_POST ? field_points_per_block(c, n) : FIELD_NMAX;
   const unsigned nb = (unsigned)((n + PB - 1) / PB);
-  geometry_field_kernel<FIELD_T, FIELD_NMAX><<<nb, FIELD_T, 0, c->stream>>>(*g, bem_view(c), pos, n, E, PB);
+  if (precision == SPT_PREC_MIXED)
+    geometry_field_kernel<FIELD_T, FIELD_NMAX, SPT_PREC_MIXED><<<nb, FIELD_T, 0, c->stream>>>(*g, bem_view(c), pos, n,
+                                                                                              E, PB);
+  else
+    geometry_field_kernel<FIELD_T, FIELD_NMAX, SPT_PREC_FP64><<<nb, FIELD_T, 0, c->stream>>>(*g, bem_view(c), pos, n,
+                                                                                             E, PB);
   HIPCHK(c, hipGetLastError());
   return SPT_OK;
 }
 
 int spt_pair_forces(spt_ctx* c, const spt_geometry* g, const double* pos, const double* charge, const uint8_t* active,
-                    const int64_t* offsets, int32_t n_groups, double a, double* Fc, double* Fi) {
+                    const int64_t* offsets, int32_t n_groups, double a, double* Fc, double* Fi, int32_t precision) {
   if (!c) return SPT_EINVAL;
   int rc = check_geom(c, g);
   if (rc) return rc;
@@ -1579,7 +1687,12 @@ int spt_pair_forces(spt_ctx* c, const spt_geometry* g, const double* pos, const 
   delete[] h;
   if (e != hipSuccess) return fail(c, SPT_EHIP, hipGetErrorString(e));
   if (bad || mx > 1024) return fail(c, SPT_EINVAL, "group sizes must be in [0, 1024]");
-  pair_forces_kernel<1024><<<n_groups, 1024, 0, c->stream>>>(*g, pos, charge, active, offsets, a * a, Fc, Fi);
+  if (precision == SPT_PREC_MIXED)
+    pair_forces_kernel<1024, SPT_PREC_MIXED><<<n_groups, 1024, 0, c->stream>>>(*g, pos, charge, active, offsets, a * a,
+                                                                              Fc, Fi);
+  else
+    pair_forces_kernel<1024, SPT_PREC_FP64><<<n_groups, 1024, 0, c->stream>>>(*g, pos, charge, active, offsets, a * a,
+                                                                             Fc, Fi);
   HIPCHK(c, hipGetLastError());
   return SPT_OK;
 }
@@ -1598,14 +1711,14 @@ int spt_kill_mask(spt_ctx* c, const spt_geometry* g, const double* pos, int64_t 
 
 }  // extern "C"
 
-template <int T, int NMAX>
+template <int T, int NMAX, int PR>
 static int launch_trace(spt_ctx* c, const TraceParams& P, int n_groups) {
   int per_cu = 0;
-  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_kernel<T, NMAX>, T, 0));
+  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_kernel<T, NMAX, PR>, T, 0));
   if (per_cu < 1) per_cu = 1;
   int64_t grid = (int64_t)per_cu * c->n_cu;
   if (grid > n_groups) grid = n_groups;
-  trace_kernel<T, NMAX><<<(unsigned)grid, T, 0, c->stream>>>(P);
+  trace_kernel<T, NMAX, PR><<<(unsigned)grid, T, 0, c->stream>>>(P);
   HIPCHK(c, hipGetLastError());
   return SPT_OK;
 }
@@ -1639,7 +1752,7 @@ static FieldShape field_shape(const spt_ctx* c, int64_t n, bool bem) {
   return out;
 }
 
-template <int TG>
+template <int TG, int PR>
 static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, FieldShape fs) {
   hipStream_t s = c->own;
   const int PB = fs.PB;
@@ -1655,10 +1768,10 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
   for (int u = 0; u < kLockstepUnroll; ++u) {
     for (int st = 0; st < 7; ++st) {
       if (fs.T == 192)
-        ls_field_kernel<192, 64><<<nb_field, 192, 0, s>>>(L, PB);
+        ls_field_kernel<192, 64, PR><<<nb_field, 192, 0, s>>>(L, PB);
       else
-        ls_field_kernel<256, 64><<<nb_field, 256, 0, s>>>(L, PB);
-      ls_accel_kernel<TG><<<n_groups, TG, 0, s>>>(L, st);
+        ls_field_kernel<256, 64, PR><<<nb_field, 256, 0, s>>>(L, PB);
+      ls_accel_kernel<TG, PR><<<n_groups, TG, 0, s>>>(L, st);
     }
     ls_end_kernel<TG><<<n_groups, TG, 0, s>>>(L);
   }
@@ -1755,6 +1868,9 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
     if (mx > 256) return fail(c, SPT_EINVAL, "groups larger than 256 particles are not supported");
   }
   if (a->mode != SPT_MODE_PERSISTENT && a->mode != SPT_MODE_LOCKSTEP) return fail(c, SPT_EINVAL, "unknown mode");
+  if (a->precision != SPT_PREC_FP64 && a->precision != SPT_PREC_MIXED) return fail(c, SPT_EINVAL, "unknown precision");
+  if (a->precision == SPT_PREC_MIXED && a->mode == SPT_MODE_PERSISTENT && T != 256)
+    return fail(c, SPT_EINVAL, "mixed precision in persistent mode is built for 256-thread workgroups only");
   const size_t need = (size_t)(a->mode == SPT_MODE_LOCKSTEP ? kSlotsLockstep : kSlots) * (size_t)a->n_particles;
   if (need > c->kbuf_elems) {
     if (c->d_kbuf) (void)hipFree(c->d_kbuf);
@@ -1817,12 +1933,16 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
     HIPCHK(c, hipEventRecord(c->ev_in, c->stream));
     HIPCHK(c, hipStreamWaitEvent(c->own, c->ev_in, 0));
     int rc3;
+    const bool mixed = a->precision == SPT_PREC_MIXED;
     if (mx <= 64)
-      rc3 = run_lockstep<64>(c, L, a->n_groups, a->n_particles, fs);
+      rc3 = mixed ? run_lockstep<64, SPT_PREC_MIXED>(c, L, a->n_groups, a->n_particles, fs)
+                  : run_lockstep<64, SPT_PREC_FP64>(c, L, a->n_groups, a->n_particles, fs);
     else if (mx <= 128)
-      rc3 = run_lockstep<128>(c, L, a->n_groups, a->n_particles, fs);
+      rc3 = mixed ? run_lockstep<128, SPT_PREC_MIXED>(c, L, a->n_groups, a->n_particles, fs)
+                  : run_lockstep<128, SPT_PREC_FP64>(c, L, a->n_groups, a->n_particles, fs);
     else
-      rc3 = run_lockstep<256>(c, L, a->n_groups, a->n_particles, fs);
+      rc3 = mixed ? run_lockstep<256, SPT_PREC_MIXED>(c, L, a->n_groups, a->n_particles, fs)
+                  : run_lockstep<256, SPT_PREC_FP64>(c, L, a->n_groups, a->n_particles, fs);
     HIPCHK(c, hipEventRecord(c->ev_out, c->own));
     HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_out, 0));
     return rc3;
@@ -1830,11 +1950,13 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
   reset_int_kernel<<<1, 1, 0, c->stream>>>(c->d_queue, 0);
   reset_u64_kernel<<<1, 1, 0, c->stream>>>(P.ev_count);
   switch (T) {
-    case 64: return launch_trace<64, 64>(c, P, a->n_groups);
-    case 128: return launch_trace<128, 128>(c, P, a->n_groups);
-    case 256: return launch_trace<256, 256>(c, P, a->n_groups);
-    case 512: return launch_trace<512, 256>(c, P, a->n_groups);
-    default: return launch_trace<1024, 64>(c, P, a->n_groups);
+    case 64: return launch_trace<64, 64, SPT_PREC_FP64>(c, P, a->n_groups);
+    case 128: return launch_trace<128, 128, SPT_PREC_FP64>(c, P, a->n_groups);
+    case 256:
+      return a->precision == SPT_PREC_MIXED ? launch_trace<256, 256, SPT_PREC_MIXED>(c, P, a->n_groups)
+                                            : launch_trace<256, 256, SPT_PREC_FP64>(c, P, a->n_groups);
+    case 512: return launch_trace<512, 256, SPT_PREC_FP64>(c, P, a->n_groups);
+    default: return launch_trace<1024, 64, SPT_PREC_FP64>(c, P, a->n_groups);
   }
 }
 

@@ -139,6 +139,84 @@ __device__ __forceinline__ double ring_potential(double rho, double z, double a)
   return ellip_k_c(fmin(p, 1.0)) / (2.0 * kPi * kPi * sqrt(D));
 }
 
+// ---------------------------------------------------- FP32 (mixed) path
+// Degree-6 FP32 pair for K (same logarithmic form), E from the same
+// Legendre derivative relation; ~1 and ~1.5 float ulp.
+__constant__ float c_PK32[7] = SPT_ELLIP_PK32;
+__constant__ float c_QK32[7] = SPT_ELLIP_QK32;
+
+__device__ __forceinline__ void horner7f_d(const float* c, float x, float& s, float& d) {
+  s = c[6];
+  d = 0.0f;
+#pragma unroll
+  for (int i = 5; i >= 0; --i) {
+    d = fmaf(d, x, s);
+    s = fmaf(s, x, c[i]);
+  }
+}
+
+__device__ __forceinline__ void ellip_ke_c32(float p, float& K, float& E) {
+  const float lp = __logf(p);
+  float P, dP, Q, dQ;
+  horner7f_d(c_PK32, p, P, dP);
+  horner7f_d(c_QK32, p, Q, dQ);
+  K = fmaf(-lp, Q, P);
+  const float pkp = fmaf(p, dP, -Q) - (p * lp) * dQ;
+  E = fmaf(p, K, -2.0f * (1.0f - p) * pkp);
+}
+
+// Kahan-compensated FP32 accumulator: the true sum is s - c.
+struct Kahan32 {
+  float s, c;
+  __device__ void zero() { s = c = 0.0f; }
+  __device__ __forceinline__ void add(float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return (double)s - (double)c; }
+};
+
+struct RingAcc32 {
+  Kahan32 er, ez;
+  __device__ void zero() {
+    er.zero();
+    ez.zero();
+  }
+};
+
+// FP32 version of ring_accum for one far node. rho, a and the axial offset
+// arrive in FP64 and their sums/differences are formed in FP64 before
+// rounding, so the 1e-5 m absolute coordinates do not cost relative
+// accuracy in the 1e-7 m separations. Near-axis nodes use the FP64 Taylor
+// branch (into the FP64 accumulator).
+__device__ __forceinline__ void ring_accum32(double rho, double rho2, double zz_d, double a, float w, RingAcc32& acc,
+                                             RingAcc& acc64) {
+  const float zz = (float)zz_d;
+  const float zz2 = zz * zz;
+  const float af = (float)a;
+  if ((float)rho2 < 1e-10f * (af * af + zz2)) {
+    double er, ez;
+    ring_field_axis(rho, zz_d, a, er, ez);
+    acc64.er_ax = fma((double)w, er, acc64.er_ax);
+    acc64.ez_ax = fma((double)w, ez, acc64.ez_ax);
+    return;
+  }
+  const float sp = (float)(rho + a), sm = (float)(rho - a);
+  const float D = fmaf(sp, sp, zz2);
+  const float d2 = fmaf(sm, sm, zz2);
+  const float rD = __builtin_amdgcn_rsqf(D);
+  const float p = fminf(d2 * (rD * rD), 1.0f);
+  float K, E;
+  ellip_ke_c32(p, K, E);
+  const float inv_d2 = __builtin_amdgcn_rcpf(d2);
+  const float c = fmaf(-sm, sp, zz2);  // a^2 - rho^2 + zz^2
+  const float wr = w * rD;
+  acc.er.add(wr * fmaf(-c * E, inv_d2, K));
+  acc.ez.add((wr * zz) * (E * inv_d2));
+}
+
 // ---- block reductions (deterministic: fixed tree over thread index) ----
 template <int T>
 __device__ double block_sum(double v, double* red) {

@@ -28,9 +28,10 @@ def _from_device(out, meta, trailing=True):
 
 
 class _DeviceField:
-    def __init__(self, geom_struct, ctx):
+    def __init__(self, geom_struct, ctx, precision=0):
         self.g = geom_struct
         self.ctx = ctx
+        self.precision = precision
 
     def evaluate(self, position):
         import torch
@@ -39,8 +40,8 @@ class _DeviceField:
         pos, meta = _to_device(position, ctx)
         out = torch.empty_like(pos)
         ctx.bind_stream()
-        ctx.check(ctx.lib.spt_geometry_field(ctx.h, self.g, _lib.ptr(pos), pos.shape[0], _lib.ptr(out)),
-                  "spt_geometry_field")
+        ctx.check(ctx.lib.spt_geometry_field(ctx.h, self.g, _lib.ptr(pos), pos.shape[0], _lib.ptr(out),
+                                             self.precision), "spt_geometry_field")
         ctx.sync()
         return _from_device(out, meta)
 

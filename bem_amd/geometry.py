@@ -65,8 +65,8 @@ class Geometry(ABC):
                 ctx.bem_key = key
         return ctx
 
-    def _device_field(self, position):
-        return _DeviceField(self.spt_geometry(), self._ctx()).evaluate(position)
+    def _device_field(self, position, precision="fp64"):
+        return _DeviceField(self.spt_geometry(), self._ctx(), _precision_code(precision)).evaluate(position)
 
     def _device_kill(self, position, z_max=None):
         ctx = self._ctx()
@@ -81,11 +81,18 @@ class Geometry(ABC):
         res = out.bool()
         return _from_device(res, meta, trailing=False)
 
-    def _device_image(self, position, charge, active, plummer_radius):
-        return _pair_forces(self, position, charge, active, plummer_radius, want="image")
+    def _device_image(self, position, charge, active, plummer_radius, precision="fp64"):
+        return _pair_forces(self, position, charge, active, plummer_radius, want="image", precision=precision)
 
 
-def _pair_forces(geom, position, charge, active, plummer_radius, want):
+def _precision_code(precision):
+    codes = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}
+    if precision not in codes:
+        raise ValueError(f"precision must be one of {sorted(codes)}, got {precision!r}")
+    return codes[precision]
+
+
+def _pair_forces(geom, position, charge, active, plummer_radius, want, precision="fp64"):
     """Coulomb or image force over (G, E, 3) arrays via spt_pair_forces."""
     import torch
 
@@ -105,16 +112,17 @@ def _pair_forces(geom, position, charge, active, plummer_radius, want):
     fc = _lib.ptr(out) if want == "coulomb" else None
     fi = _lib.ptr(out) if want == "image" else None
     ctx.check(ctx.lib.spt_pair_forces(ctx.h, g, _lib.ptr(P), _lib.ptr(q), _lib.ptr(a), _lib.ptr(off), G,
-                                      float(plummer_radius), fc, fi), "spt_pair_forces")
+                                      float(plummer_radius), fc, fi, _precision_code(precision)), "spt_pair_forces")
     ctx.sync()
     out = out.reshape(G, E, 3)
     return out.cpu().numpy() if is_np else out
 
 
-def coulomb_force(position, charge, active, plummer_radius, geometry=None):
-    """In-group Plummer-softened Coulomb force (reference forces.py:32-57)."""
+def coulomb_force(position, charge, active, plummer_radius, geometry=None, precision="fp64"):
+    """In-group Plummer-softened Coulomb force (reference forces.py:32-57);
+    precision "mixed" sums in FP32 with Kahan compensation."""
     geom = geometry if geometry is not None else FlatCathode(0.0, z0=None)
-    return _pair_forces(geom, position, charge, active, plummer_radius, want="coulomb")
+    return _pair_forces(geom, position, charge, active, plummer_radius, want="coulomb", precision=precision)
 
 
 class FlatCathode(Geometry):

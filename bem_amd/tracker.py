@@ -13,7 +13,10 @@ device time), and ``mode``: "persistent" (one workgroup integrates whole
 groups from a device queue), "lockstep" (every running group makes one
 attempt per iteration, per-stage kernels replayed from a HIP graph; the
 field gather spreads each particle over many threads) or "auto" (lockstep
-when there are fewer groups than twice the compute units).
+when there are fewer groups than twice the compute units); and
+``precision``: "fp64" (the plain method) or "mixed" (FP32 ring-kernel sums
+over segments at least two panel lengths away and FP32 pair forces, both
+Kahan-compensated; FP64 elsewhere).
 """
 import warnings
 
@@ -39,7 +42,7 @@ class SpecificParticleTracer:
     def __init__(self, initial_particles, n_emit, geometry, screens, t_out=None,
                  plummer_radius=DEFAULT_PLUMMER_RADIUS, backend="mi355x", n_workers=1, rtol=None, atol=None,
                  t_max=None, z_max=None, group_sizes=None, block_threads=0, device=0, max_steps=10_000_000,
-                 mode="auto"):
+                 mode="auto", precision="fp64"):
         if n_workers != 1:
             raise ValueError(f"n_workers is only supported for backend='cpu' (got backend={backend!r}, "
                              f"n_workers={n_workers!r})")
@@ -71,6 +74,10 @@ class SpecificParticleTracer:
         if mode not in ("auto", "persistent", "lockstep"):
             raise ValueError(f"mode must be 'auto', 'persistent' or 'lockstep', got {mode!r}")
         self.mode = mode
+        from .geometry import _precision_code
+
+        self.precision = precision
+        self._precision = _precision_code(precision)
         self.max_steps = int(max_steps)
         self._setup_state(group_sizes)
         self.t_max = t_max if t_max is not None else self._default_t_max()
@@ -192,6 +199,7 @@ class SpecificParticleTracer:
         a.max_steps = self.max_steps
         a.n_screens = ns
         a.mode = self._mode_code(ctx)
+        a.precision = self._precision
         for i, z in enumerate(self.screen_positions):
             a.screens[i] = float(z)
         a.attempts, a.accepted, a.active_stages = _lib.ptr(attempts), _lib.ptr(accepted), _lib.ptr(act)

@@ -41,7 +41,7 @@ def workload(n_groups=64, group=16, n_segments=400, seed=0):
                 rtol=1e-9, atol=1e-12)
 
 
-def gpu_bench(wl, steps, warmup, block_threads, mode="auto"):
+def gpu_bench(wl, steps, warmup, block_threads, mode="auto", precision="fp64"):
     import torch
 
     from bem_amd import SpecificParticleTracer
@@ -53,7 +53,7 @@ def gpu_bench(wl, steps, warmup, block_threads, mode="auto"):
     pg = ParticleGroup(dict(src, weight=np.full(n, 1.602176634e-19), status=np.ones(n), species="electron"))
     geo = PostTipBEMGeometry(wl["E_gun"], wl["R"], wl["H"], n_segments=wl["n_segments"])
     tr = SpecificParticleTracer(pg, wl["group"], geo, wl["screens"], rtol=wl["rtol"], atol=wl["atol"],
-                                z_max=wl["z_max"], block_threads=block_threads, mode=mode)
+                                z_max=wl["z_max"], block_threads=block_threads, mode=mode, precision=precision)
     for _ in range(warmup):
         tr.trace()
     torch.cuda.synchronize()
@@ -115,6 +115,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--block-threads", type=int, default=0)
     ap.add_argument("--mode", default="auto", choices=["auto", "persistent", "lockstep"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
@@ -123,7 +124,7 @@ def main():
               "with no exchange during integration); --gpus above 1 is not supported.", file=sys.stderr)
         sys.exit(2)
     wl = workload()
-    tr, outs, wall = gpu_bench(wl, args.steps, args.warmup, args.block_threads, args.mode)
+    tr, outs, wall = gpu_bench(wl, args.steps, args.warmup, args.block_threads, args.mode, args.precision)
     stages = sum(o["particle_stages"] for o in outs)
     act = sum(int(o["active_stages"].sum()) for o in outs)
     dev_ms = np.array([o["device_ms"] for o in outs])
@@ -141,11 +142,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f64/f32-kahan",
         "data": "synthetic (seeded cap emission, numpy default_rng(0))",
         "config": {"workload": "post-tip 1 um on 10 um post, 400-panel BEM, 1024 e- in 64 groups of 16, fp64 plain, "
                                "rtol 1e-9, screen 90 um, anode kill 100 um",
-                   "form": f"plain FP64 BEM field, {args.mode} schedule",
+                   "form": f"{'plain FP64' if args.precision == 'fp64' else 'mixed FP32-Kahan'} BEM field, "
+                           f"{args.mode} schedule",
                    "n_quad": n_quad, "block_threads": args.block_threads or 256},
         "active_particle_stages_per_s": act / wall,
         "attempts_per_group_mean": float(np.mean(outs[-1]["attempts"])),

@@ -95,6 +95,16 @@ typedef struct spt_event {
 #define SPT_MODE_PERSISTENT 0
 #define SPT_MODE_LOCKSTEP 1
 
+/* Arithmetic of the field gather and pair forces.
+ *  FP64: everything FP64 (the plain method; parity with the reference).
+ *  MIXED: ring-kernel sums over far nodes (segments at least two panel
+ *    lengths away) and pair forces in FP32 with Kahan-compensated
+ *    accumulation; FP64 for coordinate differences, segments within two
+ *    panel lengths (including the log-singular sinh rule), the error norm,
+ *    step control and the collocation solve. */
+#define SPT_PREC_FP64 0
+#define SPT_PREC_MIXED 1
+
 /* Group status words written by spt_trace. */
 #define SPT_STATUS_DONE 0
 #define SPT_STATUS_STEPCAP 1
@@ -120,6 +130,8 @@ typedef struct spt_trace_args {
   int64_t max_steps;
   int32_t n_screens;
   int32_t mode;                 /* SPT_MODE_*                               */
+  int32_t precision;            /* SPT_PREC_*                               */
+  int32_t pad0;
   double screens[SPT_MAX_SCREENS];
   /* outputs, device */
   int64_t* attempts;            /* per group                                */
@@ -161,11 +173,12 @@ int spt_bem_solve(spt_ctx* ctx, const double* profile_rz, int32_t n_nodes, doubl
 
 /* evaluate_axisymmetric_field (reference bem/axisymmetric.py:501-605) on the
  * uploaded table: perturbation field at n device points into device E. */
-int spt_bem_pert_field(spt_ctx* ctx, const double* pos, int64_t n, double* E);
+int spt_bem_pert_field(spt_ctx* ctx, const double* pos, int64_t n, double* E, int32_t precision);
 
 /* Geometry.field (reference geometry.py:144-145,227-228;
  * bem/geometry.py:319-324): full external field [V/m] at device points. */
-int spt_geometry_field(spt_ctx* ctx, const spt_geometry* geom, const double* pos, int64_t n, double* E);
+int spt_geometry_field(spt_ctx* ctx, const spt_geometry* geom, const double* pos, int64_t n, double* E,
+                       int32_t precision);
 
 /* Pair forces within CSR groups (reference forces.py:32-191; Geometry.image_force):
  * coulomb_out gets the Plummer-softened in-group Coulomb force, image_out the
@@ -173,7 +186,7 @@ int spt_geometry_field(spt_ctx* ctx, const spt_geometry* geom, const double* pos
  * one byte per particle. All device. */
 int spt_pair_forces(spt_ctx* ctx, const spt_geometry* geom, const double* pos, const double* charge,
                     const uint8_t* active, const int64_t* group_offsets, int32_t n_groups,
-                    double plummer_radius, double* coulomb_out, double* image_out);
+                    double plummer_radius, double* coulomb_out, double* image_out, int32_t precision);
 
 /* Geometry.kill_mask (+ z_max, reference geometry.py:304-322): one byte per
  * device point. */
