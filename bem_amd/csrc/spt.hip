@@ -1616,18 +1616,55 @@ static int field_points_per_block(const spt_ctx* c, int64_t n) {
   return PB;
 }
 
+}  // extern "C"
+
+// Field-gather launch shape for n points: points per block and block size
+// (192 or 256 threads). At three resident waves per SIMD a compute unit
+// holds four 3-wave or three 4-wave blocks; pick the shape whose block
+// count fills whole rounds best (e.g. 1024 points on 256 CUs: 1024
+// one-point blocks of 192 threads are exactly one round).
+struct FieldShape {
+  int T, PB;
+};
+static FieldShape field_shape(const spt_ctx* c, int64_t n, bool bem) {
+  if (!bem) return {256, FIELD_NMAX};
+  const int PB = field_points_per_block(c, n);
+  const int64_t blocks = (n + PB - 1) / PB;
+  double best = -1.0;
+  FieldShape out{256, PB};
+  for (int T : {192, 256}) {
+    const int64_t cap = (int64_t)c->n_cu * (12 / (T / 64));
+    const double rounds = (double)blocks / (double)cap;
+    const double eff = rounds / std::ceil(rounds);
+    if (eff > best + 1e-9) {
+      best = eff;
+      out = {T, PB};
+    }
+  }
+  return out;
+}
+
+extern "C" {
+
 int spt_bem_pert_field(spt_ctx* c, const double* pos, int64_t n, double* E, int32_t precision) {
   if (!c) return SPT_EINVAL;
   if (!c->bem_nodes) return fail(c, SPT_ENOBEM, "no BEM table uploaded");
   if (n < 0 || (n > 0 && (!pos || !E))) return fail(c, SPT_EINVAL, "spt_bem_pert_field: bad arguments");
   if (n == 0) return SPT_OK;
   HIPCHK(c, hipSetDevice(c->device));
-  const int PB = field_points_per_block(c, n);
-  const unsigned nb = (unsigned)((n + PB - 1) / PB);
-  if (precision == SPT_PREC_MIXED)
-    bem_pert_kernel<FIELD_T, FIELD_NMAX, SPT_PREC_MIXED><<<nb, FIELD_T, 0, c->stream>>>(bem_view(c), pos, n, E, PB);
-  else
-    bem_pert_kernel<FIELD_T, FIELD_NMAX, SPT_PREC_FP64><<<nb, FIELD_T, 0, c->stream>>>(bem_view(c), pos, n, E, PB);
+  const FieldShape fs = field_shape(c, n, true);
+  const unsigned nb = (unsigned)((n + fs.PB - 1) / fs.PB);
+  if (fs.T == 192) {
+    if (precision == SPT_PREC_MIXED)
+      bem_pert_kernel<192, FIELD_NMAX, SPT_PREC_MIXED><<<nb, 192, 0, c->stream>>>(bem_view(c), pos, n, E, fs.PB);
+    else
+      bem_pert_kernel<192, FIELD_NMAX, SPT_PREC_FP64><<<nb, 192, 0, c->stream>>>(bem_view(c), pos, n, E, fs.PB);
+  } else {
+    if (precision == SPT_PREC_MIXED)
+      bem_pert_kernel<256, FIELD_NMAX, SPT_PREC_MIXED><<<nb, 256, 0, c->stream>>>(bem_view(c), pos, n, E, fs.PB);
+    else
+      bem_pert_kernel<256, FIELD_NMAX, SPT_PREC_FP64><<<nb, 256, 0, c->stream>>>(bem_view(c), pos, n, E, fs.PB);
+  }
   HIPCHK(c, hipGetLastError());
   return SPT_OK;
 }
@@ -1725,32 +1762,6 @@ static int launch_trace(spt_ctx* c, const TraceParams& P, int n_groups) {
 
 
 constexpr int kLockstepUnroll = 4;  // attempts per captured graph replay
-
-// Field-gather launch shape for n points: points per block and block size
-// (192 or 256 threads). At three resident waves per SIMD a compute unit
-// holds four 3-wave or three 4-wave blocks; pick the shape whose block
-// count fills whole rounds best (e.g. 1024 points on 256 CUs: 1024
-// one-point blocks of 192 threads are exactly one round).
-struct FieldShape {
-  int T, PB;
-};
-static FieldShape field_shape(const spt_ctx* c, int64_t n, bool bem) {
-  if (!bem) return {256, FIELD_NMAX};
-  const int PB = field_points_per_block(c, n);
-  const int64_t blocks = (n + PB - 1) / PB;
-  double best = -1.0;
-  FieldShape out{256, PB};
-  for (int T : {192, 256}) {
-    const int64_t cap = (int64_t)c->n_cu * (12 / (T / 64));
-    const double rounds = (double)blocks / (double)cap;
-    const double eff = rounds / std::ceil(rounds);
-    if (eff > best + 1e-9) {
-      best = eff;
-      out = {T, PB};
-    }
-  }
-  return out;
-}
 
 template <int TG, int PR>
 static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, FieldShape fs) {
