@@ -23,13 +23,15 @@ GEOM_FLAT, GEOM_TIP, GEOM_BEM_POST = 0, 1, 2
 IMAGE_NONE, IMAGE_PLANE, IMAGE_TIP3 = 0, 1, 2
 STATUS_DONE, STATUS_STEPCAP = 0, 1
 MODE_PERSISTENT, MODE_LOCKSTEP = 0, 1
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_PAIR32 = 0, 1, 2
 
 EXPORTS = (
     "spt_create", "spt_destroy", "spt_last_error", "spt_set_stream", "spt_sync", "spt_device_cus",
     "spt_bem_upload", "spt_bem_assemble", "spt_bem_solve", "spt_bem_pert_field", "spt_geometry_field",
-    "spt_pair_forces", "spt_kill_mask", "spt_trace",
+    "spt_pair_forces", "spt_kill_mask", "spt_trace", "spt_bem_potential", "spt_mlp_nparams", "spt_mlp_grad",
+    "spt_mlp_train", "spt_mlp_infer", "spt_surrogate_set",
 )
+MLP_FOURIER, MLP_WIDTH, MLP_DEPTH = 32, 128, 4
 
 
 class Geometry_t(ctypes.Structure):
@@ -59,6 +61,18 @@ assert EVENT_DTYPE.itemsize == ctypes.sizeof(Event_t) == 80
 _p = ctypes.c_void_p
 
 
+class MlpCfg_t(ctypes.Structure):
+    _fields_ = [
+        ("fourier", ctypes.c_float * (2 * 32)),
+        ("center_z", ctypes.c_double), ("r_unit", ctypes.c_double),
+        ("u_offset", ctypes.c_double * 2), ("u_scale", ctypes.c_double * 2),
+        ("out_scale", ctypes.c_double * 3),
+        ("env_r0", ctypes.c_double), ("env_pow", ctypes.c_double),
+        ("rho_max", ctypes.c_double), ("z_min", ctypes.c_double), ("z_max", ctypes.c_double),
+        ("cos_min", ctypes.c_double), ("band", ctypes.c_double),
+    ]
+
+
 class TraceArgs_t(ctypes.Structure):
     _fields_ = [
         ("geom", Geometry_t),
@@ -71,7 +85,7 @@ class TraceArgs_t(ctypes.Structure):
         ("initial_step_fraction", ctypes.c_double),
         ("max_steps", ctypes.c_int64),
         ("n_screens", ctypes.c_int32), ("mode", ctypes.c_int32),
-        ("precision", ctypes.c_int32), ("pad0", ctypes.c_int32),
+        ("precision", ctypes.c_int32), ("use_surrogate", ctypes.c_int32),
         ("screens", ctypes.c_double * SPT_MAX_SCREENS),
         ("attempts", _p), ("accepted", _p), ("active_stages", _p), ("margin", _p), ("status", _p),
         ("events", _p), ("event_capacity", ctypes.c_int64), ("event_count", _p),
@@ -109,6 +123,15 @@ def load():
                                                ctypes.c_double, _p, _p, ctypes.c_int32]),
             "spt_kill_mask": (ctypes.c_int, [_p, P(Geometry_t), _p, ctypes.c_int64, _p]),
             "spt_trace": (ctypes.c_int, [_p, P(TraceArgs_t)]),
+            "spt_bem_potential": (ctypes.c_int, [_p, _p, ctypes.c_int64, _p]),
+            "spt_mlp_nparams": (ctypes.c_int, [P(ctypes.c_int64)]),
+            "spt_mlp_grad": (ctypes.c_int, [_p, P(MlpCfg_t), _p, _p, _p, ctypes.c_int32, _p, ctypes.c_int32, _p,
+                                            _p]),
+            "spt_mlp_train": (ctypes.c_int, [_p, P(MlpCfg_t), _p, _p, _p, ctypes.c_int64, _p, _p, ctypes.c_int64,
+                                             ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_uint64, _p,
+                                             ctypes.c_int32, _p]),
+            "spt_mlp_infer": (ctypes.c_int, [_p, P(MlpCfg_t), _p, _p, ctypes.c_int64, ctypes.c_int32, _p]),
+            "spt_surrogate_set": (ctypes.c_int, [_p, P(MlpCfg_t), _p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(lib, name)

@@ -44,10 +44,7 @@ class AxisymmetricBEMSolution:
     def solve(cls, profile, E_gun, refine_ratio=1.0, max_depth=20, device=0):
         return cls(profile, solve_sigma(profile, E_gun, refine_ratio, max_depth, device), device)
 
-    def field(self, points, n_subdiv=8, precision="fp64"):
-        import torch
-        from ..fields import _to_device, _from_device
-
+    def _upload(self, n_subdiv):
         ctx = _lib.Context.get(self.device)
         key = ("sol", id(self), self.profile.shape[0], n_subdiv)
         if ctx.bem_key != key:
@@ -55,6 +52,28 @@ class AxisymmetricBEMSolution:
             _, sp = _lib.host_ptr(self.sigma)
             ctx.check(ctx.lib.spt_bem_upload(ctx.h, pp, self.profile.shape[0], sp, int(n_subdiv)), "spt_bem_upload")
             ctx.bem_key = key
+        return ctx
+
+    def potential(self, points):
+        """Perturbation potential at `points` (..., 3), shape points.shape[:-1]
+        (reference bem/axisymmetric.py:445-462: adaptive per-segment
+        quadrature), on the device."""
+        import torch
+        from ..fields import _to_device, _from_device
+
+        ctx = self._upload(8)
+        pos, meta = _to_device(points, ctx)
+        out = torch.empty(pos.shape[0], dtype=torch.float64, device=ctx.torch_device)
+        ctx.bind_stream()
+        ctx.check(ctx.lib.spt_bem_potential(ctx.h, _lib.ptr(pos), pos.shape[0], _lib.ptr(out)), "spt_bem_potential")
+        ctx.sync()
+        return _from_device(out, meta, trailing=False)
+
+    def field(self, points, n_subdiv=8, precision="fp64"):
+        import torch
+        from ..fields import _to_device, _from_device
+
+        ctx = self._upload(n_subdiv)
         pos, meta = _to_device(points, ctx)
         out = torch.empty_like(pos)
         ctx.bind_stream()

@@ -975,6 +975,8 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void geometry_field_kernel(
   }
 }
 
+#include "spt_mlp.inc"
+
 // ============================================================ lockstep mode
 // The reference's schedule (batched_rk45.py:219-319): every running group
 // makes one attempt per iteration. Per attempt: 7 x (field gather over all
@@ -997,6 +999,9 @@ struct LockParams {
   double* g_nact;
   int32_t* g_flags;
   unsigned int* done;      // finished groups
+  uint8_t* sur;            // per particle: surrogate flag of this stage
+  MlpView V;               // the surrogate (when sur != nullptr): split BF16
+  Mlp64View V64;           // and FP64 forms
 };
 
 __global__ void ls_pgroup_kernel(const int64_t* offsets, int n_groups, int32_t* pgroup) {
@@ -1090,6 +1095,7 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_field_kernel(LockPa
     sx[tid] = LKB(48);
     sy[tid] = LKB(49);
     sz[tid] = LKB(50);
+    if (L.sur && L.sur[gi] == 1) act[tid] = 0;  // written by ls_surrogate_kernel
     sh.need[tid] = act[tid] && G.kind == SPT_GEOM_BEM_POST && bem_needed(G, sx[tid], sy[tid], sz[tid]);
   }
   __syncthreads();
@@ -1112,8 +1118,15 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_field_kernel(LockPa
   if (tid < m) {
     const int i = sh.cidx[tid];
     const int64_t gi = base + i;
+    double qx = sh.ox[tid], qy = sh.oy[tid], qz = sh.oz[tid];
+    if (L.sur && L.sur[gi] == 2) {  // blend with the parked surrogate perturbation
+      const double w = surrogate_weight(L.V.cfg, G, sx[i], sy[i], sz[i]);
+      qx = (1.0 - w) * qx + w * LKB(51);
+      qy = (1.0 - w) * qy + w * LKB(52);
+      qz = (1.0 - w) * qz + w * LKB(53);
+    }
     double ex, ey, ez;
-    geometry_field_point(G, sx[i], sy[i], sz[i], sh.ox[tid], sh.oy[tid], sh.oz[tid], ex, ey, ez);
+    geometry_field_point(G, sx[i], sy[i], sz[i], qx, qy, qz, ex, ey, ez);
     LKB(51) = ex;
     LKB(52) = ey;
     LKB(53) = ez;
@@ -1125,6 +1138,78 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_field_kernel(LockPa
     LKB(51) = ex;
     LKB(52) = ey;
     LKB(53) = ez;
+  }
+}
+
+// Surrogate field at the stage positions. Each wavefront takes a run of
+// consecutive particles, compacts those of running groups that are active,
+// need the BEM perturbation and have a surrogate weight w >= 0, and runs
+// the network on them (FP64: 16 per wavefront; three-term BF16 split: 32).
+// Flags for ls_field_kernel: 1 = field written (w = 1); 2 = the network's
+// perturbation is parked in the field slots and the BEM one is to be
+// blended in, (1 - w) BEM + w network; 0 = BEM only.
+template <bool F64>
+__global__ __launch_bounds__(64) void ls_surrogate_kernel(LockParams L) {
+  constexpr int PTS = F64 ? MLP64_PTS : MLP_WAVE_PTS;
+  __shared__ union {
+    Mlp64WaveShared d;
+    MlpWaveShared f;
+  } shu;
+  double* s_rho = F64 ? shu.d.rho : shu.f.rho;
+  double* s_z = F64 ? shu.d.z : shu.f.z;
+  double* s_phi = F64 ? shu.d.phi : shu.f.phi;
+  double* s_er = F64 ? shu.d.er : shu.f.er;
+  double* s_ez = F64 ? shu.d.ez : shu.f.ez;
+  int* s_idx = F64 ? shu.d.idx : shu.f.idx;
+  const int lane = threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * PTS;
+  const int cnt = (int)min((int64_t)PTS, L.T.n_total - base);
+  const spt_geometry& G = L.T.G;
+  bool elig = false;
+  double x = 0.0, y = 0.0, z = 0.0, w = -1.0;
+  if (lane < cnt) {
+    const int64_t gi = base + lane;
+    const int g = L.pgroup[gi];
+    x = LKB(48);
+    y = LKB(49);
+    z = LKB(50);
+    if ((L.g_flags[g] & LS_RUNNING) && L.active[gi] && bem_needed(G, x, y, z))
+      w = surrogate_weight(L.V.cfg, G, x, y, z);
+    elig = w >= 0.0;
+    L.sur[gi] = !elig ? 0 : (w >= 1.0 ? 1 : 2);
+  }
+  const unsigned long long mask = __ballot(elig);
+  const int m = __popcll(mask);
+  if (elig) {
+    const int slot = __popcll(mask & ((1ull << lane) - 1ull));
+    s_rho[slot] = sqrt(x * x + y * y);
+    s_z[slot] = z;
+    s_idx[slot] = lane;
+  }
+  wave_sync();
+  if (m == 0) return;
+  if (F64)
+    mlp_infer64_wave(L.V64, m, s_rho, s_z, shu.d.buf[0], shu.d.buf[1], s_phi, s_er, s_ez);
+  else
+    mlp_infer_wave(L.V, m, s_rho, s_z, shu.f.buf[0], shu.f.buf[1], s_phi, s_er, s_ez);
+  if (lane < m) {
+    const int64_t gi = base + s_idx[lane];
+    const double px = LKB(48), py = LKB(49), pz = LKB(50);
+    const double phi = atan2(py, px);
+    const bool axis = s_rho[lane] == 0.0;
+    const double er = s_er[lane];
+    const double qx = axis ? 0.0 : er * cos(phi), qy = axis ? 0.0 : er * sin(phi), qz = s_ez[lane];
+    if (L.sur[gi] == 1) {
+      double ex, ey, ez;
+      geometry_field_point(G, px, py, pz, qx, qy, qz, ex, ey, ez);
+      LKB(51) = ex;
+      LKB(52) = ey;
+      LKB(53) = ez;
+    } else {
+      LKB(51) = qx;
+      LKB(52) = qy;
+      LKB(53) = qz;
+    }
   }
 }
 
@@ -1380,6 +1465,65 @@ __global__ void assemble_combine_kernel(const double* prof, int n, const double*
   if (j == 0 && b) b[i] = E_gun * prof[2 * i + 1];
 }
 
+// Perturbation potential at points (reference bem/axisymmetric.py:445-462):
+// per point, the segments in order, each by adaptive bisection (split
+// while length > refine_ratio * distance to the midpoint, depth <=
+// max_depth) with Gauss-4 leaves and sigma linear along the segment
+// (reference bem/axisymmetric.py:275-302). Depth-first, left half first.
+__global__ void potential_kernel(const double* prof, const double* sigma, int n, double refine_ratio, int max_depth,
+                                 const double* pos, int64_t np, double* phi) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  const double fr = hypot(pos[3 * k], pos[3 * k + 1]), fz = pos[3 * k + 2];
+  double st[48][6];
+  int sd[48];
+  double total = 0.0;
+  for (int j = 0; j < n - 1; ++j) {
+    st[0][0] = prof[2 * j];
+    st[0][1] = prof[2 * j + 1];
+    st[0][2] = prof[2 * j + 2];
+    st[0][3] = prof[2 * j + 3];
+    st[0][4] = sigma[j];
+    st[0][5] = sigma[j + 1];
+    sd[0] = 0;
+    int top = 1;
+    double seg = 0.0;
+    while (top > 0) {
+      --top;
+      const double a0r = st[top][0], a0z = st[top][1], a1r = st[top][2], a1z = st[top][3];
+      const double s0 = st[top][4], s1 = st[top][5];
+      const int depth = sd[top];
+      const double mr = 0.5 * (a0r + a1r), mz = 0.5 * (a0z + a1z);
+      const double dr = a1r - a0r, dz = a1z - a0z;
+      const double length = sqrt(dr * dr + dz * dz);
+      const double er = fr - mr, ez = fz - mz;
+      const double dist = sqrt(er * er + ez * ez);
+      if (depth < max_depth && length > refine_ratio * dist && top + 2 <= 48) {
+        const double sm = 0.5 * (s0 + s1);
+        st[top][0] = mr; st[top][1] = mz; st[top][2] = a1r; st[top][3] = a1z; st[top][4] = sm; st[top][5] = s1;
+        sd[top] = depth + 1;
+        ++top;
+        st[top][0] = a0r; st[top][1] = a0z; st[top][2] = mr; st[top][3] = mz; st[top][4] = s0; st[top][5] = sm;
+        sd[top] = depth + 1;
+        ++top;
+        continue;
+      }
+      double leaf = 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double t = 0.5 * (c_gx4[q] + 1.0);
+        const double rs = a0r + t * (a1r - a0r);
+        const double zs = a0z + t * (a1z - a0z);
+        const double ss = s0 + t * (s1 - s0);
+        leaf += ss * 2.0 * kPi * rs * ring_potential(fr, fz - zs, rs) * (length * 0.5 * c_gw4[q]);
+      }
+      seg += leaf;
+    }
+    total += seg;
+  }
+  phi[k] = total;
+}
+
 // ================================================================ host side
 struct spt_ctx {
   int device = 0;
@@ -1408,6 +1552,19 @@ struct spt_ctx {
   size_t group_cap = 0;
   unsigned int* d_done = nullptr;
   unsigned int* h_done = nullptr;
+  uint8_t* d_sur = nullptr;
+  // surrogate used by spt_trace
+  bool have_sur = false;
+  spt_mlp_cfg sur_cfg;
+  float* d_sur_p = nullptr;
+  __bf16* d_sur_split = nullptr;
+  double* d_sur_frag = nullptr;
+  // MLP scratch: split weights for spt_mlp_infer, training buffers
+  __bf16* d_mlp_split = nullptr;
+  double* d_mlp_frag = nullptr;
+  float* d_train = nullptr;
+  size_t train_floats = 0;
+  int train_batch = 0;
 };
 
 static int fail(spt_ctx* c, int code, const std::string& msg) {
@@ -1481,6 +1638,13 @@ void spt_destroy(spt_ctx* c) {
   if (c->d_gflags) (void)hipFree(c->d_gflags);
   if (c->d_done) (void)hipFree(c->d_done);
   if (c->h_done) (void)hipHostFree(c->h_done);
+  if (c->d_sur) (void)hipFree(c->d_sur);
+  if (c->d_sur_p) (void)hipFree(c->d_sur_p);
+  if (c->d_sur_split) (void)hipFree(c->d_sur_split);
+  if (c->d_sur_frag) (void)hipFree(c->d_sur_frag);
+  if (c->d_mlp_frag) (void)hipFree(c->d_mlp_frag);
+  if (c->d_mlp_split) (void)hipFree(c->d_mlp_split);
+  if (c->d_train) (void)hipFree(c->d_train);
   if (c->ev_in) (void)hipEventDestroy(c->ev_in);
   if (c->ev_out) (void)hipEventDestroy(c->ev_out);
   if (c->own) (void)hipStreamDestroy(c->own);
@@ -1763,8 +1927,12 @@ static int launch_trace(spt_ctx* c, const TraceParams& P, int n_groups) {
 
 constexpr int kLockstepUnroll = 4;  // attempts per captured graph replay
 
+// PR: SPT_PREC_FP64, SPT_PREC_MIXED (FP32 far-field sums and pairs) or
+// SPT_PREC_PAIR32 (FP64 field, FP32 pairs).
 template <int TG, int PR>
 static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, FieldShape fs) {
+  constexpr int FPR = PR == SPT_PREC_MIXED ? SPT_PREC_MIXED : SPT_PREC_FP64;  // field
+  constexpr int APR = PR == SPT_PREC_FP64 ? SPT_PREC_FP64 : SPT_PREC_MIXED;   // pairs
   hipStream_t s = c->own;
   const int PB = fs.PB;
   const unsigned nb_field = (unsigned)((n + PB - 1) / PB);
@@ -1778,11 +1946,17 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
   HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   for (int u = 0; u < kLockstepUnroll; ++u) {
     for (int st = 0; st < 7; ++st) {
+      if (L.sur) {
+        if (FPR == SPT_PREC_FP64)
+          ls_surrogate_kernel<true><<<(unsigned)((n + MLP64_PTS - 1) / MLP64_PTS), 64, 0, s>>>(L);
+        else
+          ls_surrogate_kernel<false><<<(unsigned)((n + MLP_WAVE_PTS - 1) / MLP_WAVE_PTS), 64, 0, s>>>(L);
+      }
       if (fs.T == 192)
-        ls_field_kernel<192, 64, PR><<<nb_field, 192, 0, s>>>(L, PB);
+        ls_field_kernel<192, 64, FPR><<<nb_field, 192, 0, s>>>(L, PB);
       else
-        ls_field_kernel<256, 64, PR><<<nb_field, 256, 0, s>>>(L, PB);
-      ls_accel_kernel<TG, PR><<<n_groups, TG, 0, s>>>(L, st);
+        ls_field_kernel<256, 64, FPR><<<nb_field, 256, 0, s>>>(L, PB);
+      ls_accel_kernel<TG, APR><<<n_groups, TG, 0, s>>>(L, st);
     }
     ls_end_kernel<TG><<<n_groups, TG, 0, s>>>(L);
   }
@@ -1821,11 +1995,14 @@ static int ensure_lockstep_scratch(spt_ctx* c, int64_t n, int n_groups) {
   if ((size_t)n > c->part_cap) {
     if (c->d_pgroup) (void)hipFree(c->d_pgroup);
     if (c->d_active) (void)hipFree(c->d_active);
+    if (c->d_sur) (void)hipFree(c->d_sur);
     c->d_pgroup = nullptr;
     c->d_active = nullptr;
+    c->d_sur = nullptr;
     c->part_cap = 0;
     HIPCHK(c, hipMalloc(&c->d_pgroup, sizeof(int32_t) * n));
     HIPCHK(c, hipMalloc(&c->d_active, n));
+    HIPCHK(c, hipMalloc(&c->d_sur, n));
     c->part_cap = n;
   }
   if ((size_t)n_groups > c->group_cap) {
@@ -1879,7 +2056,15 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
     if (mx > 256) return fail(c, SPT_EINVAL, "groups larger than 256 particles are not supported");
   }
   if (a->mode != SPT_MODE_PERSISTENT && a->mode != SPT_MODE_LOCKSTEP) return fail(c, SPT_EINVAL, "unknown mode");
-  if (a->precision != SPT_PREC_FP64 && a->precision != SPT_PREC_MIXED) return fail(c, SPT_EINVAL, "unknown precision");
+  if (a->precision != SPT_PREC_FP64 && a->precision != SPT_PREC_MIXED && a->precision != SPT_PREC_PAIR32)
+    return fail(c, SPT_EINVAL, "unknown precision");
+  if (a->precision == SPT_PREC_PAIR32 && a->mode != SPT_MODE_LOCKSTEP)
+    return fail(c, SPT_EINVAL, "precision PAIR32 (FP64 field, FP32 pairs) runs in lockstep mode only");
+  if (a->use_surrogate) {
+    if (a->mode != SPT_MODE_LOCKSTEP) return fail(c, SPT_EINVAL, "the field surrogate runs in lockstep mode only");
+    if (!c->have_sur) return fail(c, SPT_EINVAL, "use_surrogate set but no surrogate (spt_surrogate_set)");
+    if (a->geom.kind != SPT_GEOM_BEM_POST) return fail(c, SPT_EINVAL, "the field surrogate is for BEM geometries");
+  }
   if (a->precision == SPT_PREC_MIXED && a->mode == SPT_MODE_PERSISTENT && T != 256)
     return fail(c, SPT_EINVAL, "mixed precision in persistent mode is built for 256-thread workgroups only");
   const size_t need = (size_t)(a->mode == SPT_MODE_LOCKSTEP ? kSlotsLockstep : kSlots) * (size_t)a->n_particles;
@@ -1940,20 +2125,33 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
     L.g_nact = c->d_gdbl + 4 * (size_t)a->n_groups;
     L.g_flags = c->d_gflags;
     L.done = c->d_done;
+    L.sur = a->use_surrogate ? c->d_sur : nullptr;
+    memset(&L.V, 0, sizeof(L.V));
+    memset(&L.V64, 0, sizeof(L.V64));
+    if (a->use_surrogate) {
+      L.V.cfg = c->sur_cfg;
+      L.V.p = c->d_sur_p;
+      L.V.split = c->d_sur_split;
+      L.V64.cfg = c->sur_cfg;
+      L.V64.p = c->d_sur_p;
+      L.V64.frag = c->d_sur_frag;
+    }
     const FieldShape fs = field_shape(c, a->n_particles, a->geom.kind == SPT_GEOM_BEM_POST);
     HIPCHK(c, hipEventRecord(c->ev_in, c->stream));
     HIPCHK(c, hipStreamWaitEvent(c->own, c->ev_in, 0));
     int rc3;
-    const bool mixed = a->precision == SPT_PREC_MIXED;
+    const int pr = a->precision;
+#define SPT_LS(TG)                                                                                         \
+  (pr == SPT_PREC_MIXED    ? run_lockstep<TG, SPT_PREC_MIXED>(c, L, a->n_groups, a->n_particles, fs)  \
+   : pr == SPT_PREC_PAIR32 ? run_lockstep<TG, SPT_PREC_PAIR32>(c, L, a->n_groups, a->n_particles, fs) \
+                           : run_lockstep<TG, SPT_PREC_FP64>(c, L, a->n_groups, a->n_particles, fs))
     if (mx <= 64)
-      rc3 = mixed ? run_lockstep<64, SPT_PREC_MIXED>(c, L, a->n_groups, a->n_particles, fs)
-                  : run_lockstep<64, SPT_PREC_FP64>(c, L, a->n_groups, a->n_particles, fs);
+      rc3 = SPT_LS(64);
     else if (mx <= 128)
-      rc3 = mixed ? run_lockstep<128, SPT_PREC_MIXED>(c, L, a->n_groups, a->n_particles, fs)
-                  : run_lockstep<128, SPT_PREC_FP64>(c, L, a->n_groups, a->n_particles, fs);
+      rc3 = SPT_LS(128);
     else
-      rc3 = mixed ? run_lockstep<256, SPT_PREC_MIXED>(c, L, a->n_groups, a->n_particles, fs)
-                  : run_lockstep<256, SPT_PREC_FP64>(c, L, a->n_groups, a->n_particles, fs);
+      rc3 = SPT_LS(256);
+#undef SPT_LS
     HIPCHK(c, hipEventRecord(c->ev_out, c->own));
     HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_out, 0));
     return rc3;
@@ -1969,6 +2167,270 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
     case 512: return launch_trace<512, 256, SPT_PREC_FP64>(c, P, a->n_groups);
     default: return launch_trace<1024, 64, SPT_PREC_FP64>(c, P, a->n_groups);
   }
+}
+
+}  // extern "C"
+
+// ============================================================ surrogate host
+template <int TA, int TB, int EPI, bool SPLIT>
+static void mlp_gemm(hipStream_t s, int M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C,
+                     int ldc, const float* bias, const float* Zin, float* Zout, int ldz, float* kpart) {
+  const int tiles = (M / 16) * (N / 16);
+  int ks = 1;
+  if (kpart)
+    while (ks < 64 && tiles * ks < 2048 && (K / 32) % (2 * ks) == 0 && K / (2 * ks) >= 64) ks *= 2;
+  const dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)ks);
+  if (ks == 1) {
+    mlp_gemm_kernel<TA, TB, EPI, SPLIT><<<grid, 256, 0, s>>>(M, N, K, A, lda, B, ldb, C, ldc, bias, Zin, Zout, ldz);
+  } else {  // split-K partials (STORE epilogue, C contiguous with ldc == N), then the ordered sum
+    mlp_gemm_kernel<TA, TB, EPI, SPLIT><<<grid, 256, 0, s>>>(M, N, K / ks, A, lda, B, ldb, kpart, N, bias, Zin,
+                                                             Zout, ldz);
+    mlp_ksum_kernel<<<(unsigned)(((int64_t)M * N + 255) / 256), 256, 0, s>>>(kpart, ks, (int64_t)M * N, C);
+  }
+}
+
+constexpr int kMlpKsMax = 64;
+
+struct MlpTrainBufs {
+  float* X0;
+  float* Tb;
+  float* Z[MLP_DEPTH];
+  float* H[MLP_DEPTH];
+  float* Y;
+  float* dY;
+  float* dZa;
+  float* dZb;
+  float* part;
+  float* grad;
+  float* kpart;
+  float* loss;
+};
+
+static size_t mlp_train_floats(int B) {
+  return (size_t)B * MLP_IN + (size_t)B * 4 + 2 * MLP_DEPTH * (size_t)B * MLP_W + 2 * (size_t)B * MLP_OUTP +
+         2 * (size_t)B * MLP_W + (size_t)(B + 255) / 256 + 64 + MLP_NPARAM + 64 +
+         (size_t)kMlpKsMax * MLP_W * MLP_W + 64;
+}
+
+static MlpTrainBufs mlp_train_bufs(float* base, int B) {
+  MlpTrainBufs T;
+  float* q = base;
+  auto take = [&](size_t n) {
+    float* r = q;
+    q += (n + 63) / 64 * 64;  // 256-byte aligned pieces
+    return r;
+  };
+  T.X0 = take((size_t)B * MLP_IN);
+  T.Tb = take((size_t)B * 3);
+  for (int l = 0; l < MLP_DEPTH; ++l) T.Z[l] = take((size_t)B * MLP_W);
+  for (int l = 0; l < MLP_DEPTH; ++l) T.H[l] = take((size_t)B * MLP_W);
+  T.Y = take((size_t)B * MLP_OUTP);
+  T.dY = take((size_t)B * MLP_OUTP);
+  T.dZa = take((size_t)B * MLP_W);
+  T.dZb = take((size_t)B * MLP_W);
+  T.part = take((size_t)(B + 255) / 256);
+  T.grad = take(MLP_NPARAM);
+  T.loss = take(1);
+  T.kpart = take((size_t)kMlpKsMax * MLP_W * MLP_W);
+  return T;
+}
+
+static int ensure_mlp_train(spt_ctx* c, int B) {
+  const size_t need = mlp_train_floats(B) + 64 * 32;
+  if (need > c->train_floats) {
+    if (c->d_train) (void)hipFree(c->d_train);
+    c->d_train = nullptr;
+    c->train_floats = 0;
+    HIPCHK(c, hipMalloc(&c->d_train, sizeof(float) * need));
+    c->train_floats = need;
+  }
+  return SPT_OK;
+}
+
+// Forward, loss and backward of one batch (X0, Tb already built); the
+// gradient lands in T.grad and the loss at loss_dst (device).
+template <bool SPLIT>
+static void mlp_step(hipStream_t s, const MlpTrainBufs& T, const float* p, int B, const float* w, float* loss_dst) {
+  mlp_gemm<0, 1, EPI_BIAS_SILU, SPLIT>(s, B, MLP_W, MLP_IN, T.X0, MLP_IN, p + MLP_OFF_W1, MLP_IN, T.H[0], MLP_W,
+                                       p + MLP_OFF_B1, nullptr, T.Z[0], MLP_W, nullptr);
+  for (int l = 1; l < MLP_DEPTH; ++l) {
+    const int64_t o = MLP_OFF_H + (l - 1) * MLP_HSTRIDE;
+    mlp_gemm<0, 1, EPI_BIAS_SILU, SPLIT>(s, B, MLP_W, MLP_W, T.H[l - 1], MLP_W, p + o, MLP_W, T.H[l], MLP_W,
+                                         p + o + (int64_t)MLP_W * MLP_W, nullptr, T.Z[l], MLP_W, nullptr);
+  }
+  mlp_gemm<0, 1, EPI_BIAS, SPLIT>(s, B, MLP_OUTP, MLP_W, T.H[MLP_DEPTH - 1], MLP_W, p + MLP_OFF_WO, MLP_W, T.Y,
+                                  MLP_OUTP, p + MLP_OFF_BO, nullptr, nullptr, 0, nullptr);
+  const int nb = (B + 255) / 256;
+  mlp_loss_kernel<<<nb, 256, 0, s>>>(T.Y, T.Tb, B, w[0], w[1], w[2], T.dY, T.part);
+  mlp_loss_final_kernel<<<1, 64, 0, s>>>(T.part, nb, B, loss_dst);
+  float* g = T.grad;
+  mlp_gemm<1, 0, EPI_STORE, SPLIT>(s, MLP_OUTP, MLP_W, B, T.dY, MLP_OUTP, T.H[MLP_DEPTH - 1], MLP_W, g + MLP_OFF_WO,
+                                   MLP_W, nullptr, nullptr, nullptr, 0, T.kpart);
+  mlp_colsum_kernel<<<MLP_OUTP, 256, 0, s>>>(T.dY, B, MLP_OUTP, g + MLP_OFF_BO);
+  mlp_gemm<0, 0, EPI_DSILU, SPLIT>(s, B, MLP_W, MLP_OUTP, T.dY, MLP_OUTP, p + MLP_OFF_WO, MLP_W, T.dZa, MLP_W, nullptr,
+                                   T.Z[MLP_DEPTH - 1], nullptr, MLP_W, nullptr);
+  float* da = T.dZa;
+  float* db = T.dZb;
+  for (int l = MLP_DEPTH - 1; l >= 1; --l) {
+    const int64_t o = MLP_OFF_H + (l - 1) * MLP_HSTRIDE;
+    mlp_gemm<1, 0, EPI_STORE, SPLIT>(s, MLP_W, MLP_W, B, da, MLP_W, T.H[l - 1], MLP_W, g + o, MLP_W, nullptr, nullptr,
+                                     nullptr, 0, T.kpart);
+    mlp_colsum_kernel<<<MLP_W, 256, 0, s>>>(da, B, MLP_W, g + o + (int64_t)MLP_W * MLP_W);
+    mlp_gemm<0, 0, EPI_DSILU, SPLIT>(s, B, MLP_W, MLP_W, da, MLP_W, p + o, MLP_W, db, MLP_W, nullptr, T.Z[l - 1],
+                                     nullptr, MLP_W, nullptr);
+    float* t = da;
+    da = db;
+    db = t;
+  }
+  mlp_gemm<1, 0, EPI_STORE, SPLIT>(s, MLP_W, MLP_IN, B, da, MLP_W, T.X0, MLP_IN, g + MLP_OFF_W1, MLP_IN, nullptr,
+                                   nullptr, nullptr, 0, T.kpart);
+  mlp_colsum_kernel<<<MLP_W, 256, 0, s>>>(da, B, MLP_W, g + MLP_OFF_B1);
+}
+
+static int check_mlp_cfg(spt_ctx* c, const spt_mlp_cfg* cfg) {
+  if (!cfg) return fail(c, SPT_EINVAL, "null surrogate config");
+  if (!(cfg->r_unit > 0.0) || !(cfg->env_r0 > 0.0) || !(cfg->env_pow >= 0.0))
+    return fail(c, SPT_EINVAL, "surrogate config needs r_unit > 0, env_r0 > 0, env_pow >= 0");
+  for (int i = 0; i < 3; ++i)
+    if (!(cfg->out_scale[i] != 0.0)) return fail(c, SPT_EINVAL, "surrogate out_scale must be nonzero");
+  return SPT_OK;
+}
+
+extern "C" {
+
+int spt_bem_potential(spt_ctx* c, const double* pos, int64_t n, double* phi) {
+  if (!c) return SPT_EINVAL;
+  if (!c->bem_nodes) return fail(c, SPT_ENOBEM, "no BEM table uploaded");
+  if (n < 0 || (n > 0 && (!pos || !phi))) return fail(c, SPT_EINVAL, "spt_bem_potential: bad arguments");
+  if (n == 0) return SPT_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  potential_kernel<<<(unsigned)((n + 127) / 128), 128, 0, c->stream>>>(c->d_prof, c->d_sigma, c->bem_nodes, 1.0, 20,
+                                                                       pos, n, phi);
+  HIPCHK(c, hipGetLastError());
+  return SPT_OK;
+}
+
+int spt_mlp_nparams(int64_t* n) {
+  if (!n) return SPT_EINVAL;
+  *n = MLP_NPARAM;
+  return SPT_OK;
+}
+
+int spt_mlp_grad(spt_ctx* c, const spt_mlp_cfg* cfg, const float* params, const double* x, const double* y,
+                 int32_t batch, const float* loss_weights, int32_t split, float* grad, float* loss) {
+  if (!c) return SPT_EINVAL;
+  int rc = check_mlp_cfg(c, cfg);
+  if (rc) return rc;
+  if (!params || !x || !y || !loss_weights || !grad || !loss) return fail(c, SPT_EINVAL, "spt_mlp_grad: null pointer");
+  if (batch < 32 || batch % 32) return fail(c, SPT_EINVAL, "batch must be a positive multiple of 32");
+  HIPCHK(c, hipSetDevice(c->device));
+  rc = ensure_mlp_train(c, batch);
+  if (rc) return rc;
+  const MlpTrainBufs T = mlp_train_bufs(c->d_train, batch);
+  hipStream_t s = c->stream;
+  mlp_batch_kernel<<<(batch + 255) / 256, 256, 0, s>>>(*cfg, x, y, batch, batch, 0ULL, 0, T.X0, T.Tb);
+  if (split)
+    mlp_step<true>(s, T, params, batch, loss_weights, T.loss);
+  else
+    mlp_step<false>(s, T, params, batch, loss_weights, T.loss);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(grad, T.grad, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(loss, T.loss, sizeof(float), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  return SPT_OK;
+}
+
+int spt_mlp_train(spt_ctx* c, const spt_mlp_cfg* cfg, float* params, float* adam_m, float* adam_v, int64_t adam_t0,
+                  const double* x, const double* y, int64_t n_samples, int32_t batch, int32_t steps, float lr,
+                  uint64_t seed, const float* loss_weights, int32_t split, float* loss_out) {
+  if (!c) return SPT_EINVAL;
+  int rc = check_mlp_cfg(c, cfg);
+  if (rc) return rc;
+  if (!params || !adam_m || !adam_v || !x || !y || !loss_weights || (steps > 0 && !loss_out))
+    return fail(c, SPT_EINVAL, "spt_mlp_train: null pointer");
+  if (batch < 32 || batch % 32) return fail(c, SPT_EINVAL, "batch must be a positive multiple of 32");
+  if (n_samples < 1 || steps < 0 || adam_t0 < 0) return fail(c, SPT_EINVAL, "spt_mlp_train: bad sizes");
+  if (steps == 0) return SPT_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  rc = ensure_mlp_train(c, batch);
+  if (rc) return rc;
+  const MlpTrainBufs T = mlp_train_bufs(c->d_train, batch);
+  hipStream_t s = c->stream;
+  float* d_loss = nullptr;
+  HIPCHK(c, hipMalloc(&d_loss, sizeof(float) * steps));
+  const double b1 = 0.9, b2 = 0.999;
+  for (int i = 0; i < steps; ++i) {
+    const int64_t t = adam_t0 + i + 1;
+    mlp_batch_kernel<<<(batch + 255) / 256, 256, 0, s>>>(*cfg, x, y, n_samples, batch, seed, t, T.X0, T.Tb);
+    if (split)
+      mlp_step<true>(s, T, params, batch, loss_weights, d_loss + i);
+    else
+      mlp_step<false>(s, T, params, batch, loss_weights, d_loss + i);
+    const float c1 = (float)(1.0 - pow(b1, (double)t)), c2 = (float)(1.0 - pow(b2, (double)t));
+    mlp_adam_kernel<<<(unsigned)((MLP_NPARAM + 255) / 256), 256, 0, s>>>(params, T.grad, adam_m, adam_v, MLP_NPARAM,
+                                                                         lr, (float)b1, (float)b2, 1e-8f, c1, c2);
+  }
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(loss_out, d_loss, sizeof(float) * steps, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(d_loss);
+  if (e != hipSuccess) return fail(c, SPT_EHIP, std::string("spt_mlp_train: ") + hipGetErrorString(e));
+  return SPT_OK;
+}
+
+int spt_mlp_infer(spt_ctx* c, const spt_mlp_cfg* cfg, const float* params, const double* pos, int64_t n,
+                  int32_t precision, double* out) {
+  if (!c) return SPT_EINVAL;
+  int rc = check_mlp_cfg(c, cfg);
+  if (rc) return rc;
+  if (!params || n < 0 || (n > 0 && (!pos || !out))) return fail(c, SPT_EINVAL, "spt_mlp_infer: bad arguments");
+  if (precision != SPT_PREC_FP64 && precision != SPT_PREC_MIXED) return fail(c, SPT_EINVAL, "unknown precision");
+  if (n == 0) return SPT_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  if (precision == SPT_PREC_FP64) {
+    if (!c->d_mlp_frag) HIPCHK(c, hipMalloc(&c->d_mlp_frag, sizeof(double) * MLP64_FRAG_N));
+    mlp_frag64_kernel<<<(unsigned)((MLP64_FRAG_N + 255) / 256), 256, 0, s>>>(params, c->d_mlp_frag);
+    Mlp64View V;
+    V.cfg = *cfg;
+    V.p = params;
+    V.frag = c->d_mlp_frag;
+    mlp_infer64_kernel<<<(unsigned)((n + MLP64_PTS - 1) / MLP64_PTS), 64, 0, s>>>(V, pos, n, out);
+  } else {
+    if (!c->d_mlp_split) HIPCHK(c, hipMalloc(&c->d_mlp_split, sizeof(__bf16) * 3 * MLP_NPARAM));
+    mlp_split_kernel<<<(unsigned)((MLP_NPARAM + 255) / 256), 256, 0, s>>>(params, MLP_NPARAM, c->d_mlp_split);
+    MlpView V;
+    V.cfg = *cfg;
+    V.p = params;
+    V.split = c->d_mlp_split;
+    mlp_infer_kernel<<<(unsigned)((n + MLP_WAVE_PTS - 1) / MLP_WAVE_PTS), 64, 0, s>>>(V, pos, n, out);
+  }
+  HIPCHK(c, hipGetLastError());
+  return SPT_OK;
+}
+
+int spt_surrogate_set(spt_ctx* c, const spt_mlp_cfg* cfg, const float* params) {
+  if (!c) return SPT_EINVAL;
+  if (!params) {
+    c->have_sur = false;
+    return SPT_OK;
+  }
+  int rc = check_mlp_cfg(c, cfg);
+  if (rc) return rc;
+  if (!(cfg->band >= 0.0)) return fail(c, SPT_EINVAL, "surrogate band must be >= 0");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!c->d_sur_p) HIPCHK(c, hipMalloc(&c->d_sur_p, sizeof(float) * MLP_NPARAM));
+  if (!c->d_sur_split) HIPCHK(c, hipMalloc(&c->d_sur_split, sizeof(__bf16) * 3 * MLP_NPARAM));
+  if (!c->d_sur_frag) HIPCHK(c, hipMalloc(&c->d_sur_frag, sizeof(double) * MLP64_FRAG_N));
+  HIPCHK(c, hipMemcpyAsync(c->d_sur_p, params, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToDevice, c->stream));
+  mlp_split_kernel<<<(unsigned)((MLP_NPARAM + 255) / 256), 256, 0, c->stream>>>(c->d_sur_p, MLP_NPARAM,
+                                                                                c->d_sur_split);
+  mlp_frag64_kernel<<<(unsigned)((MLP64_FRAG_N + 255) / 256), 256, 0, c->stream>>>(c->d_sur_p, c->d_sur_frag);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->sur_cfg = *cfg;
+  c->have_sur = true;
+  return SPT_OK;
 }
 
 }  // extern "C"

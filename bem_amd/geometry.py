@@ -85,8 +85,10 @@ class Geometry(ABC):
         return _pair_forces(self, position, charge, active, plummer_radius, want="image", precision=precision)
 
 
-def _precision_code(precision):
+def _precision_code(precision, trace=False):
     codes = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}
+    if trace:
+        codes["pairs32"] = _lib.PREC_PAIR32
     if precision not in codes:
         raise ValueError(f"precision must be one of {sorted(codes)}, got {precision!r}")
     return codes[precision]

@@ -16,7 +16,10 @@ field gather spreads each particle over many threads) or "auto" (lockstep
 when there are fewer groups than twice the compute units); and
 ``precision``: "fp64" (the plain method) or "mixed" (FP32 ring-kernel sums
 over segments at least two panel lengths away and FP32 pair forces, both
-Kahan-compensated; FP64 elsewhere).
+Kahan-compensated; FP64 elsewhere) or "pairs32" (FP64 field, FP32 Kahan
+pair forces; lockstep); and ``surrogate``: a trained
+bem_amd.surrogate.FieldSurrogate for the geometry, used in lockstep mode
+for the BEM perturbation wherever its judged band allows.
 """
 import warnings
 
@@ -42,7 +45,7 @@ class SpecificParticleTracer:
     def __init__(self, initial_particles, n_emit, geometry, screens, t_out=None,
                  plummer_radius=DEFAULT_PLUMMER_RADIUS, backend="mi355x", n_workers=1, rtol=None, atol=None,
                  t_max=None, z_max=None, group_sizes=None, block_threads=0, device=0, max_steps=10_000_000,
-                 mode="auto", precision="fp64"):
+                 mode="auto", precision="fp64", surrogate=None):
         if n_workers != 1:
             raise ValueError(f"n_workers is only supported for backend='cpu' (got backend={backend!r}, "
                              f"n_workers={n_workers!r})")
@@ -77,7 +80,15 @@ class SpecificParticleTracer:
         from .geometry import _precision_code
 
         self.precision = precision
-        self._precision = _precision_code(precision)
+        self._precision = _precision_code(precision, trace=True)
+        if precision == "pairs32" and mode == "persistent":
+            raise ValueError("precision 'pairs32' (FP64 field, FP32 pair forces) runs in lockstep mode only")
+        if surrogate is not None:
+            if mode == "persistent":
+                raise ValueError("a field surrogate runs in lockstep mode only (mode='lockstep' or 'auto')")
+            if surrogate.geometry is not geometry:
+                raise ValueError("the surrogate was trained for a different geometry object")
+        self.surrogate = surrogate
         self.max_steps = int(max_steps)
         self._setup_state(group_sizes)
         self.t_max = t_max if t_max is not None else self._default_t_max()
@@ -142,6 +153,8 @@ class SpecificParticleTracer:
         return np.argsort(-sizes, kind="stable").astype(np.int32)
 
     def _mode_code(self, ctx):
+        if self.surrogate is not None or self.precision == "pairs32":
+            return _lib.MODE_LOCKSTEP
         if self.mode == "persistent":
             return _lib.MODE_PERSISTENT
         if self.mode == "lockstep":
@@ -200,6 +213,10 @@ class SpecificParticleTracer:
         a.n_screens = ns
         a.mode = self._mode_code(ctx)
         a.precision = self._precision
+        a.use_surrogate = 0
+        if self.surrogate is not None:
+            self.surrogate.install(ctx)
+            a.use_surrogate = 1
         for i, z in enumerate(self.screen_positions):
             a.screens[i] = float(z)
         a.attempts, a.accepted, a.active_stages = _lib.ptr(attempts), _lib.ptr(accepted), _lib.ptr(act)

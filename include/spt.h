@@ -104,6 +104,36 @@ typedef struct spt_event {
  *    step control and the collocation solve. */
 #define SPT_PREC_FP64 0
 #define SPT_PREC_MIXED 1
+#define SPT_PREC_PAIR32 2 /* spt_trace, LOCKSTEP: FP64 field, FP32 + Kahan pairs */
+
+/* Fourier-feature MLP field surrogate (no counterpart in the reference; the
+ * reference evaluates the BEM field directly). Fixed architecture: 32
+ * frequencies -> 64 features -> 4 x (128, SiLU) -> 3 outputs (phi, E_rho,
+ * E_z of the BEM perturbation), parameters FP32, flat layout [W1 (128x64),
+ * b1, W2, b2, W3, b3, W4, b4, Wout (32x128, rows 3.. zero), bout (32)].
+ *
+ * Inputs: polar coordinates about (rho, z) = (0, center_z):
+ * r = |(rho, z - center_z)|, u = (log(r / r_unit), (z - center_z) / r),
+ * normalised un = (u - u_offset) * u_scale; features sin/cos(2 pi B un).
+ * Outputs: physical = network output * out_scale * env(r),
+ * env(r) = (1 + r / env_r0)^-env_pow. */
+#define SPT_MLP_FOURIER 32
+#define SPT_MLP_WIDTH 128
+#define SPT_MLP_DEPTH 4
+
+typedef struct spt_mlp_cfg {
+  float fourier[2 * SPT_MLP_FOURIER]; /* frequency matrix B (32 x 2), row-major  */
+  double center_z, r_unit;
+  double u_offset[2], u_scale[2];
+  double out_scale[3];
+  double env_r0, env_pow;
+  double rho_max, z_min, z_max;       /* box where spt_trace may use it,         */
+  double cos_min;                     /* cone (z - center_z) / r >= cos_min,     */
+  double band;                        /* and at least this far from the          */
+                                      /* conductor surface (capsule of the       */
+                                      /* geometry); blended to the BEM between   */
+                                      /* band and 2 band                         */
+} spt_mlp_cfg;
 
 /* Group status words written by spt_trace. */
 #define SPT_STATUS_DONE 0
@@ -131,7 +161,9 @@ typedef struct spt_trace_args {
   int32_t n_screens;
   int32_t mode;                 /* SPT_MODE_*                               */
   int32_t precision;            /* SPT_PREC_*                               */
-  int32_t pad0;
+  int32_t use_surrogate;        /* 1: field from the surrogate set with     */
+                                /* spt_surrogate_set where it applies       */
+                                /* (LOCKSTEP mode only), BEM elsewhere      */
   double screens[SPT_MAX_SCREENS];
   /* outputs, device */
   int64_t* attempts;            /* per group                                */
@@ -175,6 +207,13 @@ int spt_bem_solve(spt_ctx* ctx, const double* profile_rz, int32_t n_nodes, doubl
  * uploaded table: perturbation field at n device points into device E. */
 int spt_bem_pert_field(spt_ctx* ctx, const double* pos, int64_t n, double* E, int32_t precision);
 
+/* AxisymmetricBEMSolution.potential (reference bem/axisymmetric.py:445-462,
+ * 275-302): perturbation potential of the uploaded BEM table at n device
+ * points into device phi; per segment, adaptive bisection (split while
+ * length > distance to the midpoint, depth <= 20) with Gauss-4 leaves and
+ * sigma linear along the segment. Surrogate training labels. */
+int spt_bem_potential(spt_ctx* ctx, const double* pos, int64_t n, double* phi);
+
 /* Geometry.field (reference geometry.py:144-145,227-228;
  * bem/geometry.py:319-324): full external field [V/m] at device points. */
 int spt_geometry_field(spt_ctx* ctx, const spt_geometry* geom, const double* pos, int64_t n, double* E,
@@ -191,6 +230,36 @@ int spt_pair_forces(spt_ctx* ctx, const spt_geometry* geom, const double* pos, c
 /* Geometry.kill_mask (+ z_max, reference geometry.py:304-322): one byte per
  * device point. */
 int spt_kill_mask(spt_ctx* ctx, const spt_geometry* geom, const double* pos, int64_t n, uint8_t* out);
+
+/* Surrogate training and use (no counterpart in the reference). Array
+ * arguments are device pointers unless marked host.
+ *  spt_mlp_nparams: number of FP32 parameters of the flat layout.
+ *  spt_mlp_grad: loss and gradient (same layout as params) of the batch
+ *    x (batch x 2: rho, z; FP64), y (batch x 3 physical targets: phi, E_rho,
+ *    E_z; FP64); loss = mean over rows of sum_c w_c (net_c - y_c / scale_c)^2
+ *    with scale_c = out_scale_c * env(r); batch a multiple of 32. Products
+ *    on BF16 MFMA with FP32 accumulation: split = 0 one BF16 term per
+ *    operand, split = 1 a three-term BF16 split (six products). host:
+ *    loss_weights[3], loss.
+ *  spt_mlp_train: `steps` Adam steps (lr, beta1 0.9, beta2 0.999, eps 1e-8,
+ *    step t = adam_t0 + 1 ...) on batches drawn from the n samples by a
+ *    counter-based hash of (seed, step); host: loss_weights[3], per-step
+ *    losses loss_out[steps].
+ *  spt_mlp_infer: (phi, E_rho, E_z) at n points (x, y, z) in physical units;
+ *    precision SPT_PREC_FP64: FP64 MFMA products; SPT_PREC_MIXED: ~FP32
+ *    accuracy from a three-term BF16 split (six MFMA products per tile).
+ *    spt_trace uses the form matching its own precision.
+ *  spt_surrogate_set: make params the surrogate that spt_trace uses when
+ *    use_surrogate is set (a private copy is kept); NULL clears it. */
+int spt_mlp_nparams(int64_t* n);
+int spt_mlp_grad(spt_ctx* ctx, const spt_mlp_cfg* cfg, const float* params, const double* x, const double* y,
+                 int32_t batch, const float* loss_weights, int32_t split, float* grad, float* loss);
+int spt_mlp_train(spt_ctx* ctx, const spt_mlp_cfg* cfg, float* params, float* adam_m, float* adam_v,
+                  int64_t adam_t0, const double* x, const double* y, int64_t n_samples, int32_t batch, int32_t steps,
+                  float lr, uint64_t seed, const float* loss_weights, int32_t split, float* loss_out);
+int spt_mlp_infer(spt_ctx* ctx, const spt_mlp_cfg* cfg, const float* params, const double* pos, int64_t n,
+                  int32_t precision, double* out);
+int spt_surrogate_set(spt_ctx* ctx, const spt_mlp_cfg* cfg, const float* params);
 
 /* batched_rk45.integrate + make_accel_fn + find_crossings_grouped
  * (reference batched_rk45.py:55-401, geometry.py:265-301, screens.py:43-168,

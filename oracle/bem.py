@@ -244,6 +244,64 @@ def assemble(profile, refine_ratio=1.0, max_depth=20):
     return A
 
 
+def potential(profile, sigma, points, refine_ratio=1.0, max_depth=20):
+    """Perturbation potential at `points` (..., 3) (reference
+    ``bem/axisymmetric.py:445-462``): per segment, adaptive bisection
+    (split while length > refine_ratio * distance to the midpoint, depth <=
+    max_depth) with Gauss-4 leaves and sigma linear along the segment
+    (reference ``bem/axisymmetric.py:275-302``).
+
+    Vectorised breadth-first over all (point, segment) pairs, like
+    ``assemble``; the leaf sum order differs from the reference's recursion
+    only in rounding."""
+    profile = np.asarray(profile, dtype=float)
+    sigma = np.asarray(sigma, dtype=float)
+    pts = np.asarray(points, dtype=float)
+    shape = pts.shape[:-1]
+    flat = pts.reshape(-1, 3)
+    rho_f = np.hypot(flat[:, 0], flat[:, 1])
+    z_f = flat[:, 2]
+    n = len(profile)
+    ii, jj = np.meshgrid(np.arange(flat.shape[0]), np.arange(n - 1), indexing="ij")
+    ii, jj = ii.ravel(), jj.ravel()
+    fr, fz = rho_f[ii], z_f[ii]
+    a0 = profile[jj].copy()
+    a1 = profile[jj + 1].copy()
+    s0 = sigma[jj].copy()
+    s1 = sigma[jj + 1].copy()
+    acc = np.zeros(ii.size)
+    owner = np.arange(ii.size)
+    depth = 0
+    while owner.size:
+        mid = 0.5 * (a0 + a1)
+        length = np.linalg.norm(a1 - a0, axis=-1)
+        dist = np.hypot(fr[owner] - mid[:, 0], fz[owner] - mid[:, 1])
+        split = (depth < max_depth) & (length > refine_ratio * dist)
+        leaf = ~split
+        if leaf.any():
+            o = owner[leaf]
+            b0, b1, t0, t1, ln = a0[leaf], a1[leaf], s0[leaf], s1[leaf], length[leaf]
+            tot = np.zeros(o.size)
+            for xi, wi in zip(_GX4, _GW4):
+                t = 0.5 * (xi + 1.0)
+                rs = b0[:, 0] + t * (b1[:, 0] - b0[:, 0])
+                zs = b0[:, 1] + t * (b1[:, 1] - b0[:, 1])
+                ss = t0 + t * (t1 - t0)
+                tot += ss * 2.0 * np.pi * rs * ring_potential(fr[o], fz[o] - zs, rs) * (ln * 0.5 * wi)
+            np.add.at(acc, o, tot)
+        if split.any():
+            pm = mid[split]
+            sm = 0.5 * (s0[split] + s1[split])
+            a0 = np.concatenate([a0[split], pm])
+            a1 = np.concatenate([pm, a1[split]])
+            s0, s1 = np.concatenate([s0[split], sm]), np.concatenate([sm, s1[split]])
+            owner = np.concatenate([owner[split], owner[split]])
+        else:
+            owner = owner[:0]
+        depth += 1
+    return acc.reshape(flat.shape[0], n - 1).sum(axis=1).reshape(shape)
+
+
 def solve(profile, dirichlet, refine_ratio=1.0, max_depth=20):
     """sigma with A sigma = g at every node (reference
     ``bem/axisymmetric.py:411-443``); ``dirichlet(rho, z)`` vectorised."""

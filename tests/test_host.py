@@ -45,6 +45,37 @@ def test_struct_layouts():
     assert _lib.TraceArgs_t.screens.offset % 8 == 0
 
 
+def test_struct_layouts_match_the_c_header(tmp_path):
+    """sizeof/offsetof of every ABI struct, compiled from include/spt.h by
+    the host C compiler, against the ctypes mirrors."""
+    import shutil
+    import subprocess
+
+    from bem_amd import _lib
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no host C compiler")
+    structs = {"spt_geometry": _lib.Geometry_t, "spt_event": _lib.Event_t, "spt_mlp_cfg": _lib.MlpCfg_t,
+               "spt_trace_args": _lib.TraceArgs_t}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "spt.h"', "int main(void) {"]
+    for cname, ct in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in ct._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, ct in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(ct), cname
+        for fname, _ in ct._fields_:
+            assert got[(cname, fname)] == getattr(ct, fname).offset, (cname, fname)
+
+
 def _coeffs(name):
     src = open(os.path.join(ROOT, "bem_amd", "csrc", "ellip_coeffs.h")).read()
     return [float(x) for x in re.search(r"SPT_ELLIP_%s \{ (.*) \}" % name, src).group(1).split(",")]
