@@ -1,19 +1,31 @@
-"""Benchmark: particle-RK-stages/s of the MI355X tracing path.
+"""Benchmark: particle-RK-stages/s of the MI355X tracing path, surrogate +
+batched against plain, at a surrogate field error <= 1e-6.
 
 Workload (one "step" = one full trace, every group from birth to t_max):
 hemisphere-capped post, R = 1 um on a 10 um post, 400-panel mirror-doubled
 axisymmetric BEM (12,800 field quadrature nodes), uniform -1e8 V/m (anode
-+10 kV at 100 um; particles absorbed there), 1024 electrons emitted from the
-cap in 64 groups of 16 (polar angle U[0, 60 deg], KE ~ Exp(0.2 eV),
-cos-weighted), FP64, RK45 rtol 1e-9, atol 1e-12, screen at 90 um, seed 0.
++10 kV at 100 um; particles absorbed there), 262,144 electrons emitted from
+the cap in 16,384 groups of 16 (polar angle U[0, 60 deg], KE ~ Exp(0.2 eV),
+cos-weighted, numpy default_rng(0)), RK45 rtol 1e-8, atol 1e-12, screen at
+90 um; FP64 field and step control, FP32 + Kahan pair forces; lockstep
+schedule (every running group one attempt per graph-replayed iteration).
 
-value = 7 x sum_g(size_g x attempts_g) / time (all particle slots of
+The surrogate (32 Fourier features, 4 x 128 SiLU, outputs phi, E_rho, E_z)
+is trained on the device before the timed region (2^20 BEM samples,
+log-uniform in distance from the surface), judged against the BEM on
+fresh points, and used only beyond the distance band where its worst
+field error relative to |E| is <= 1e-6 (and never within 3 panel lengths).
+The plain method (no surrogate) is timed on the same workload; so are the
+surrogate bands for looser tolerances (1e-4, 1e-3).
+
+value = 7 x sum_g(size_g x attempts_g) / wall time (all particle slots of
 running groups, the reference's accel_fn shape). The roofline entry is for
-the dominant kernel (the persistent trace kernel, one launch per step):
-algorithmic FP64 flops = active particle-stages x 12,800 nodes x 120 flops
-per node + pair flops, over its device time from HIP events on the launch
-stream. cpu_baseline: the oracle (NumPy FP64 restatement of the reference)
-on a time-bounded sample of the same workload, one process.
+the dominant kernel, the BEM field gather (99% of device time in a plain
+trace): algorithmic FP64 flops = points x 12,800 nodes x 120 flops per
+node, over its average launch time from HIP events on the launch stream,
+on 262,144 points 10 nm .. 10 um off the cap. cpu_baseline: the oracle
+(NumPy FP64 restatement of the reference) on a time-bounded sample of the
+same workload, one process.
 """
 import argparse
 import json
@@ -29,88 +41,94 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-RK-stages/sec, 1 GPU, surrogate+batched vs plain at field err ≤1e-6"
 PEAK_FP64_TFLOPS = 78.6
 C_RING = 120          # FP64 flops per ring-kernel node (K, E, log, kernel algebra, accumulation)
-W_PAIR_PER_PARTNER = 20
+TRAFFIC_BYTES = 43.5e6  # HBM bytes per 262,144-point field-gather launch, from the rocprofv3 counter runs
 
 
-def workload(n_groups=64, group=16, n_segments=400, seed=0):
+def workload(n_el=262144, group=16, n_segments=400, seed=0, rtol=1e-8):
     from bem_amd.emission import cap_source
 
     R, H = 1e-6, 10e-6
-    src = cap_source(n_groups * group, R, H, theta_max_deg=60.0, mean_ke_ev=0.2, seed=seed)
+    src = cap_source(n_el, R, H, theta_max_deg=60.0, mean_ke_ev=0.2, seed=seed)
     return dict(R=R, H=H, E_gun=-1e8, n_segments=n_segments, src=src, group=group, screens=[90e-6], z_max=100e-6,
-                rtol=1e-9, atol=1e-12)
+                rtol=rtol, atol=1e-12, n_el=n_el)
 
 
-def gpu_bench(wl, steps, warmup, block_threads, mode="auto", precision="fp64"):
-    import torch
-
+def make_tracer(wl, geo, surrogate=None, precision="pairs32", mode="lockstep"):
     from bem_amd import SpecificParticleTracer
-    from bem_amd.bem.geometry import PostTipBEMGeometry
     from bem_amd.particles import ParticleGroup
 
     src = wl["src"]
     n = len(src["x"])
     pg = ParticleGroup(dict(src, weight=np.full(n, 1.602176634e-19), status=np.ones(n), species="electron"))
-    geo = PostTipBEMGeometry(wl["E_gun"], wl["R"], wl["H"], n_segments=wl["n_segments"])
-    tr = SpecificParticleTracer(pg, wl["group"], geo, wl["screens"], rtol=wl["rtol"], atol=wl["atol"],
-                                z_max=wl["z_max"], block_threads=block_threads, mode=mode, precision=precision)
+    return SpecificParticleTracer(pg, wl["group"], geo, wl["screens"], rtol=wl["rtol"], atol=wl["atol"],
+                                  z_max=wl["z_max"], mode=mode, precision=precision, surrogate=surrogate)
+
+
+def timed_traces(tr, steps, warmup):
+    import torch
+
     for _ in range(warmup):
         tr.trace()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     outs = [tr.trace() for _ in range(steps)]
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    return tr, outs, wall
+    return outs, time.perf_counter() - t0
 
 
-def field_gather_timing(wl, reps=20):
-    """Average duration of the BEM field gather (the trace's dominant
-    kernel; the standalone entry point runs the same device code and launch
-    shape) over the workload's particles displaced 10 nm .. 10 um off the
-    cap, timed with HIP events on the launch stream."""
+def screen_time_diff(a, b):
+    ea, eb = a["events"], b["events"]
+    if len(ea) != len(eb):
+        return None
+    ka = np.lexsort((ea["particle"], ea["group"]))
+    kb = np.lexsort((eb["particle"], eb["group"]))
+    dt = np.abs(ea["t"][ka] - eb["t"][kb]) / ea["t"][ka]
+    return dict(max=float(dt.max()), median=float(np.median(dt)))
+
+
+def field_gather_timing(wl, geo, n_pts=262144, reps=10):
+    """Average launch duration of the BEM field gather over n_pts points
+    displaced 10 nm .. 10 um off the cap, timed with HIP events on the
+    stream the kernel is launched on (the library is bound to torch's
+    current stream)."""
     import torch
 
     from bem_amd import _lib
-    from bem_amd.bem.geometry import PostTipBEMGeometry
+    from bem_amd.emission import cap_source
 
-    src = wl["src"]
-    n = len(src["x"])
+    src = cap_source(n_pts, wl["R"], wl["H"], theta_max_deg=60.0, mean_ke_ev=0.2, seed=1)
     R, H = wl["R"], wl["H"]
     p = np.stack([src["x"], src["y"], src["z"] - H], -1)
     rc = np.linalg.norm(p, axis=1, keepdims=True)
-    d = 10 ** np.random.default_rng(1).uniform(-8, -5, (n, 1))
+    d = 10 ** np.random.default_rng(1).uniform(-8, -5, (n_pts, 1))
     pts = p / rc * (R + d)
     pts[:, 2] += H
-    geo = PostTipBEMGeometry(wl["E_gun"], R, H, n_segments=wl["n_segments"])
-    sol = geo.solution
-    sol.field(pts[:4])  # upload
-    ctx = _lib.Context.get(0)
-    pt = torch.as_tensor(pts, device="cuda").contiguous()
+    ctx = geo._ctx()
+    pt = torch.as_tensor(pts, device=ctx.torch_device).contiguous()
     out = torch.empty_like(pt)
     ctx.bind_stream()
     for _ in range(2):
-        ctx.check(ctx.lib.spt_bem_pert_field(ctx.h, _lib.ptr(pt), n, _lib.ptr(out), 0), "field")
+        ctx.check(ctx.lib.spt_bem_pert_field(ctx.h, _lib.ptr(pt), n_pts, _lib.ptr(out), _lib.PREC_FP64), "field")
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(reps):
-        ctx.check(ctx.lib.spt_bem_pert_field(ctx.h, _lib.ptr(pt), n, _lib.ptr(out), 0), "field")
+        ctx.check(ctx.lib.spt_bem_pert_field(ctx.h, _lib.ptr(pt), n_pts, _lib.ptr(out), _lib.PREC_FP64), "field")
     e1.record()
     torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / reps, n
+    return e0.elapsed_time(e1) / reps, n_pts
 
 
 def cpu_baseline(wl, seconds=15.0):
     """Oracle on the first group of the workload until `seconds` of work."""
     from oracle import bem as ob
+    from oracle import rk45
     from oracle import tracker as otr
+    from bem_amd.bem.mesh import post_profile
 
     src = wl["src"]
     g = wl["group"]
     R, H = wl["R"], wl["H"]
-    from bem_amd.bem.mesh import post_profile
-
     prof = post_profile(R, H, wl["n_segments"])
     sigma = ob.solve(prof, lambda r, z: wl["E_gun"] * z)
     st = otr.setup(*(src[k][:g] for k in ("x", "y", "z", "px", "py", "pz", "t", "id")), g)
@@ -131,8 +149,6 @@ def cpu_baseline(wl, seconds=15.0):
         count[0] += pos.shape[0] * pos.shape[1]
         return accel(pos, vel, active, gidx)
 
-    from oracle import rk45
-
     t_max = otr.default_t_max(st, wl["E_gun"], wl["screens"])
     try:
         rk45.integrate(timed, st["pos"], st["vel"], st["t_birth"], t_max, wl["rtol"], wl["atol"],
@@ -145,34 +161,90 @@ def cpu_baseline(wl, seconds=15.0):
                        f"({count[0]} particle-stages), 1 process")
 
 
+def c2_plain_rate(steps=3):
+    """Plain FP64 method on 1024 electrons (64 groups of 16, rtol 1e-9), the
+    configuration the parity tests pin against the oracle."""
+    from bem_amd.bem.geometry import PostTipBEMGeometry
+
+    wl = workload(n_el=1024, rtol=1e-9)
+    geo = PostTipBEMGeometry(wl["E_gun"], wl["R"], wl["H"], n_segments=wl["n_segments"])
+    tr = make_tracer(wl, geo, precision="fp64", mode="lockstep")
+    outs, wall = timed_traces(tr, steps, 1)
+    return dict(value=sum(o["particle_stages"] for o in outs) / wall, unit="particle-RK-stages/s",
+                workload="1024 e- in 64 groups of 16, rtol 1e-9, FP64 field and pairs, lockstep",
+                attempts_mean=float(outs[-1]["attempts"].mean()), ms_per_trace=wall / steps * 1e3)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--block-threads", type=int, default=0)
-    ap.add_argument("--mode", default="auto", choices=["auto", "persistent", "lockstep"])
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--electrons", type=int, default=262144)
+    ap.add_argument("--tol", type=float, default=1e-6, help="surrogate field-error tolerance for the headline value")
+    ap.add_argument("--train-steps", type=int, default=60000)
+    ap.add_argument("--no-sweep", action="store_true", help="skip the looser-tolerance runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--field-only", action="store_true",
+                    help="only the field-gather timing and a 256 MiB device copy (for hardware-counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the tracing path is benchmarked per GPU (groups shard across GPUs "
               "with no exchange during integration); --gpus above 1 is not supported.", file=sys.stderr)
         sys.exit(2)
-    wl = workload()
-    tr, outs, wall = gpu_bench(wl, args.steps, args.warmup, args.block_threads, args.mode, args.precision)
+    from bem_amd import surrogate
+    from bem_amd.bem.geometry import PostTipBEMGeometry
+
+    def say(*a):
+        print(*a, file=sys.stderr, flush=True)
+
+    wl = workload(n_el=args.electrons)
+    t0 = time.perf_counter()
+    geo = PostTipBEMGeometry(wl["E_gun"], wl["R"], wl["H"], n_segments=wl["n_segments"])
+    t_solve = time.perf_counter() - t0
+    if args.field_only:
+        import torch
+
+        field_ms, n_pts = field_gather_timing(wl, geo, reps=3)
+        src = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps(dict(field_ms=field_ms, points=n_pts, copy_bytes_read=src.numel() * 4,
+                              copy_bytes_written=dst.numel() * 4)))
+        return
+    say(f"bench: BEM solve {t_solve:.2f} s; training the surrogate ({args.train_steps} steps)")
+    sur = surrogate.train(geo, n_samples=1 << 20, steps=args.train_steps, batch=8192, tol=args.tol, chunk=10000)
+    rep = sur.report
+    say(f"bench: trained in {rep['t_train_s']:.1f} s; band at tol {args.tol:g}: {rep['band']:.3e} m")
+    tr = make_tracer(wl, geo, surrogate=sur)
+    outs, wall = timed_traces(tr, args.steps, args.warmup)
+    say(f"bench: {args.steps} surrogate traces in {wall:.2f} s")
     stages = sum(o["particle_stages"] for o in outs)
     act = sum(int(o["active_stages"].sum()) for o in outs)
-    dev_ms = np.array([o["device_ms"] for o in outs])
+    plain_outs, plain_wall = timed_traces(make_tracer(wl, geo), 1, 0)
+    plain_value = plain_outs[0]["particle_stages"] / plain_wall
+    say(f"bench: plain trace {plain_wall:.2f} s")
+    sweep = []
+    if not args.no_sweep:
+        for tol in (1e-4, 1e-3):
+            s = surrogate.with_tolerance(sur, tol)
+            if not np.isfinite(s.cfg["band"]):
+                sweep.append(dict(tol=tol, band_m=None))
+                continue
+            o, w = timed_traces(make_tracer(wl, geo, surrogate=s), 1, 0)
+            sweep.append(dict(tol=tol, band_m=s.cfg["band"], value=o[0]["particle_stages"] / w,
+                              wall_s=w, gain_wall=plain_wall / w, attempts_mean=float(o[0]["attempts"].mean()),
+                              screen_t_rel_diff_vs_plain=screen_time_diff(plain_outs[0], o[0])))
+            say(f"bench: tol {tol:g} trace {w:.2f} s")
+    field_ms, n_pts = field_gather_timing(wl, geo)
     n_quad = wl["n_segments"] * 8 * 4
-    flops = act * (n_quad * C_RING + W_PAIR_PER_PARTNER * wl["group"])
-    trace_tflops = flops / (dev_ms.sum() / 1e3) / 1e12
-    field_ms, n_pts = field_gather_timing(wl)
     achieved = n_pts * n_quad * C_RING / (field_ms / 1e3) / 1e12
+    value = stages / wall
     res = {
         "metric": METRIC,
-        "value": stages / wall,
+        "value": value,
         "unit": "particle-RK-stages/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -181,26 +253,39 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f64/f32-kahan",
-        "data": "synthetic (seeded cap emission, numpy default_rng(0))",
-        "config": {"workload": "post-tip 1 um on 10 um post, 400-panel BEM, 1024 e- in 64 groups of 16, fp64 plain, "
-                               "rtol 1e-9, screen 90 um, anode kill 100 um",
-                   "form": f"{'plain FP64' if args.precision == 'fp64' else 'mixed FP32-Kahan'} BEM field, "
-                           f"{args.mode} schedule",
-                   "n_quad": n_quad, "block_threads": args.block_threads or 256},
-        "active_particle_stages_per_s": act / wall,
+        "dtype": "f64",
+        "data": "synthetic (seeded cap emission, numpy default_rng(0)); surrogate trained on BEM samples",
+        "config": {"workload": f"post-tip 1 um on 10 um post, 400-panel BEM, {wl['n_el']} e- in "
+                               f"{wl['n_el'] // wl['group']} groups of 16, rtol 1e-8, screen 90 um, anode kill 100 um",
+                   "form": f"surrogate + batched lockstep, band at field err <= {args.tol:g}; FP64 field and step "
+                           "control, FP32 + Kahan pair forces",
+                   "n_quad": n_quad},
+        "plain_value": plain_value,
+        "gain_vs_plain": value / plain_value,
+        "surrogate": {"tol": args.tol, "band_m": rep["band"], "band_panels": rep["band_panels"],
+                      "field_err_max": rep["field_err_max"], "field_err_median": rep["field_err_median"],
+                      "field_err_p99": rep["field_err_p99"], "phi_err_max": rep["phi_err_max"],
+                      "train_s": rep["t_train_s"], "labels_s": rep["t_labels_s"], "train_steps": rep["steps"],
+                      "batch": rep["batch"], "n_samples": rep["n_samples"],
+                      "screen_t_rel_diff_vs_plain": screen_time_diff(plain_outs[0], outs[-1])},
+        "tolerance_sweep": sweep,
         "attempts_per_group_mean": float(np.mean(outs[-1]["attempts"])),
+        "plain_attempts_per_group_mean": float(np.mean(plain_outs[0]["attempts"])),
+        "active_particle_stages_per_s": act / wall,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": "bem_pert_kernel (BEM field gather; same device code and launch shape as the "
-                               "trace's ls_field_kernel)",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_BYTES,
+                     "traffic_source": "FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB copy), "
+                                       "profiles/bem_field_gather_post400_262k.md",
+                     "kernel": "bem_pert_kernel (BEM field gather; the same device code as the trace's "
+                               "ls_field_kernel)",
                      "kernel_ms_mean": field_ms, "points_per_launch": n_pts,
                      "flops_per_launch": n_pts * n_quad * C_RING,
-                     "whole_trace_tflops": trace_tflops, "trace_ms_mean": float(dev_ms.mean()),
                      "note": "FP64 VALU-bound; MI355X FP64 vector peak = FP64 MFMA peak = 78.6 TFLOP/s"},
+        "c2_plain_fp64": c2_plain_rate(),
     }
     if not args.no_cpu_baseline:
-        res["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
+        wl_cpu = dict(wl)
+        res["cpu_baseline"] = cpu_baseline(wl_cpu, args.cpu_seconds)
     print(json.dumps(res))
 
 
