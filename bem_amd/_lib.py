@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(_HERE, "libspt_hip.so")
 
 SPT_OK, SPT_EINVAL, SPT_ENOMEM, SPT_EHIP, SPT_ENOBEM, SPT_ESTEPCAP = 0, -1, -2, -3, -4, -5
 SPT_MAX_SCREENS = 16
-GEOM_FLAT, GEOM_TIP, GEOM_BEM_POST = 0, 1, 2
+GEOM_FLAT, GEOM_TIP, GEOM_BEM_POST, GEOM_BEM_WELL = 0, 1, 2, 3
 IMAGE_NONE, IMAGE_PLANE, IMAGE_TIP3 = 0, 1, 2
 STATUS_DONE, STATUS_STEPCAP = 0, 1
 MODE_PERSISTENT, MODE_LOCKSTEP = 0, 1
@@ -41,6 +41,7 @@ class Geometry_t(ctypes.Structure):
         ("z0", ctypes.c_double), ("z_handoff", ctypes.c_double),
         ("has_kill_z", ctypes.c_int32), ("has_z_max", ctypes.c_int32),
         ("kill_z_below", ctypes.c_double), ("z_max", ctypes.c_double),
+        ("bottom_fillet", ctypes.c_double), ("rim_fillet", ctypes.c_double),
     ]
 
 

@@ -1,7 +1,8 @@
-"""BEM-backed geometries (reference bem/geometry.py:58-507).
+"""BEM-backed geometries (reference bem/geometry.py:58-893).
 
 The static field is the uniform asymptotic field plus the perturbation of
-an axisymmetric BEM solve on a mirror-doubled closed profile, zero inside
+an axisymmetric BEM solve (a mirror-doubled closed profile for
+protrusions, the real truncated-plane surface for the well), zero inside
 the conductor, blended with a cubic smoothstep to the plain flat-cathode
 field between z_handoff/2 and z_handoff (reference bem/geometry.py:44-55,
 319-324). The collocation solve runs on the device at construction.
@@ -14,7 +15,7 @@ import numpy as np
 
 from .. import _lib
 from .. import geometry as geometry_module
-from .mesh import n_for_length, post_profile, sphere_profile
+from .mesh import cylindrical_well_profile, n_for_length, post_profile, sphere_profile
 from .solver import AxisymmetricBEMSolution
 
 
@@ -135,3 +136,51 @@ class PostTipBEMGeometry(_AxisymmetricBEMGeometry):
 
 geometry_module._WORKER_REGISTRY["hemispherical_tip_bem"] = HemisphericalTipBEMGeometry
 geometry_module._WORKER_REGISTRY["post_tip_bem"] = PostTipBEMGeometry
+
+
+class CylindricalWellBEMGeometry(_AxisymmetricBEMGeometry):
+    """Grounded cylindrical well of radius R and depth H cut into a flat
+    cathode (reference bem/geometry.py:510-812, bem/fields.py:240-352): BEM
+    on the real surface (bottom disk, wall, graded plane out to
+    ``field_plane_radius``, optional rounded corners), zero field inside
+    the conductor, the same flat-cathode blend; particles are killed only
+    when they re-enter the conductor. Default ``z_handoff`` = 20 max(R, H),
+    ``field_plane_radius`` = z_handoff, ``field_max_length`` = min(R, H)/19
+    (the reference's defaults)."""
+
+    def __init__(self, E_gun, R, H, field_plane_radius=None, field_max_length=None, n_subdiv=8,
+                 bottom_fillet_radius=None, rim_fillet_radius=None, z0=None, z_handoff=None, sigma=None, device=0):
+        _no_image(z0)
+        self.z0 = None
+        self.H = float(H)
+        R = float(R)
+        zh = 20.0 * max(R, self.H) if z_handoff is None else float(z_handoff)
+        self.field_plane_radius = zh if field_plane_radius is None else float(field_plane_radius)
+        self.field_max_length = field_max_length
+        self.bottom_fillet_radius = bottom_fillet_radius
+        self.rim_fillet_radius = rim_fillet_radius
+        max_length = min(R, self.H) / 19.0 if field_max_length is None else float(field_max_length)
+        profile = cylindrical_well_profile(R, self.H, self.field_plane_radius, max_length,
+                                           bottom_fillet_radius=bottom_fillet_radius,
+                                           rim_fillet_radius=rim_fillet_radius)
+        self._setup(E_gun, R, profile, n_subdiv, sigma, zh, None, device)
+
+    def spt_geometry(self, z_max=None):
+        g = super().spt_geometry(z_max)
+        g.kind = _lib.GEOM_BEM_WELL
+        g.bottom_fillet = float(self.bottom_fillet_radius or 0.0)
+        g.rim_fillet = float(self.rim_fillet_radius or 0.0)
+        return g
+
+    def worker_args(self):
+        return ("cylindrical_well_bem", self.E_gun, self.R, self.H, self.field_plane_radius, self.field_max_length,
+                self.n_subdiv, self.bottom_fillet_radius, self.rim_fillet_radius, None, self.z_handoff)
+
+    @classmethod
+    def _from_worker_args(cls, args):
+        E_gun, R, H, fpr, fml, n_subdiv, bf, rf, z0, z_handoff = args
+        return cls(E_gun, R, H, field_plane_radius=fpr, field_max_length=fml, n_subdiv=n_subdiv,
+                   bottom_fillet_radius=bf, rim_fillet_radius=rf, z0=z0, z_handoff=z_handoff)
+
+
+geometry_module._WORKER_REGISTRY["cylindrical_well_bem"] = CylindricalWellBEMGeometry

@@ -1,8 +1,9 @@
 """Generating profiles (rho, z) for the axisymmetric BEM solves.
 
-``sphere_profile`` and ``n_for_length`` keep the reference's definitions
-(``bem/mesh.py:65-73,134-150``). ``post_profile`` is new: the reference has
-no post geometry."""
+``sphere_profile``, ``n_for_length`` and the cylindrical-well profile keep
+the reference's definitions (``bem/mesh.py:65-170,317-409``): the BEM
+density is solved on these exact nodes, so the node placement is part of
+the result. ``post_profile`` is new: the reference has no post geometry."""
 import numpy as np
 
 
@@ -57,3 +58,81 @@ def post_profile(R, H, n_segments):
     prof[-1, 0] = 0.0
     prof[0, 0] = 0.0
     return prof
+
+
+MIN_FILLET_SEGMENTS = 5  # every 90-degree fillet gets at least this many segments
+
+
+def region_max_length(max_length, region_max_length):
+    """A per-region cap can only tighten the global one."""
+    return max_length if region_max_length is None else min(max_length, region_max_length)
+
+
+def flat_annulus(r_inner, r_outer, z, n):
+    r = np.linspace(r_inner, r_outer, n)
+    return np.column_stack([r, np.full(n, float(z))])
+
+
+def graded_annulus(r_inner, r_outer, z, first_step, growth=1.15):
+    """Nodes from r_inner to r_outer, the spacing starting at first_step and
+    growing by `growth` per step; a last gap shorter than half the previous
+    step is absorbed by moving the last interior node onto r_outer."""
+    r = [float(r_inner)]
+    step = float(first_step)
+    while r[-1] + step < r_outer:
+        r.append(r[-1] + step)
+        step *= growth
+    if len(r) > 1 and r_outer - r[-1] < 0.5 * (r[-1] - r[-2]):
+        r[-1] = float(r_outer)
+    else:
+        r.append(float(r_outer))
+    r = np.asarray(r)
+    return np.column_stack([r, np.full(len(r), float(z))])
+
+
+def arc(center_r, radius, phi0, phi1, n):
+    """(center_r + radius cos phi, radius sin phi) for phi from phi0 to phi1."""
+    ph = np.linspace(phi0, phi1, n)
+    return np.column_stack([center_r + radius * np.cos(ph), radius * np.sin(ph)])
+
+
+def wall(rho, z0, z1, n):
+    z = np.linspace(z0, z1, n)
+    return np.column_stack([np.full(n, float(rho)), z])
+
+
+def cylindrical_well_profile(R, H, plane_radius, max_length, bottom_fillet_radius=None, rim_fillet_radius=None,
+                             bottom_max_length=None, bottom_fillet_max_length=None, wall_max_length=None,
+                             rim_fillet_max_length=None, plane_max_length=None, plane_growth=1.15):
+    """The real surface of a cylindrical well of radius R and depth H cut
+    into the z = 0 cathode, from the axis outwards: bottom disk (z = -H),
+    optional bottom fillet, wall (rho = R), optional rim fillet, then the
+    plane graded out to plane_radius (reference ``bem/mesh.py:317-409``)."""
+    rb = 0.0 if bottom_fillet_radius is None else float(bottom_fillet_radius)
+    rr = 0.0 if rim_fillet_radius is None else float(rim_fillet_radius)
+    if rb < 0.0:
+        raise ValueError(f"bottom_fillet_radius must be >= 0 (got {bottom_fillet_radius!r})")
+    if rr < 0.0:
+        raise ValueError(f"rim_fillet_radius must be >= 0 (got {rim_fillet_radius!r})")
+    if rb + rr >= H:
+        raise ValueError(f"bottom_fillet_radius + rim_fillet_radius must be < H (got {rb!r} + {rr!r} >= {H!r})")
+    parts = []
+    n = n_for_length(R - rb, region_max_length(max_length, bottom_max_length))
+    parts.append(flat_annulus(0.0, R - rb, -H, n))
+    if rb > 0.0:
+        n = n_for_length(0.5 * np.pi * rb, region_max_length(max_length, bottom_fillet_max_length),
+                         min_points=MIN_FILLET_SEGMENTS + 1)
+        a = arc(R - rb, rb, -0.5 * np.pi, 0.0, n)[1:]
+        a[:, 1] += -H + rb
+        parts.append(a)
+    n = n_for_length(H - rr - rb, region_max_length(max_length, wall_max_length))
+    parts.append(wall(R, -H + rb, -rr, n)[1:])
+    if rr > 0.0:
+        n = n_for_length(0.5 * np.pi * rr, region_max_length(max_length, rim_fillet_max_length),
+                         min_points=MIN_FILLET_SEGMENTS + 1)
+        a = arc(R + rr, rr, np.pi, 0.5 * np.pi, n)[1:]
+        a[:, 1] += -rr
+        parts.append(a)
+    parts.append(graded_annulus(R + rr, plane_radius, 0.0, region_max_length(max_length, plane_max_length),
+                                growth=plane_growth)[1:])
+    return np.concatenate(parts, axis=0)

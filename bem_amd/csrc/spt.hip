@@ -345,11 +345,42 @@ __device__ __forceinline__ double post_r2(double x, double y, double z, double H
   return x * x + y * y + dz * dz;
 }
 
+__device__ __forceinline__ bool is_bem(const spt_geometry& G) {
+  return G.kind == SPT_GEOM_BEM_POST || G.kind == SPT_GEOM_BEM_WELL;
+}
+
+// Inside (or on) the real cylindrical-well conductor (reference
+// bem/fields.py:89-135): below the plane outside the well radius, below
+// the bottom inside it; within a fillet's square the distance to the
+// fillet centre decides, biased 1e-6 R towards inside.
+__device__ __forceinline__ bool well_contains(const spt_geometry& G, double rho, double z) {
+  const double R = G.R, H = G.H;
+  const double eps = 1e-6 * R;
+  bool m = rho >= R * (1.0 - 1e-6) ? (z <= 0.0) : (z <= -H * (1.0 + 1e-6));
+  const double rf = G.rim_fillet, bf = G.bottom_fillet;
+  if (rf > 0.0 && rho >= R - eps && rho <= R + rf + eps && z >= -rf - eps && z <= eps) {
+    const double dr = rho - (R + rf), dz = z + rf;
+    m = sqrt(dr * dr + dz * dz) <= rf;
+  }
+  if (bf > 0.0 && rho <= R + eps && rho >= R - bf - eps && z <= -H + bf + eps && z >= -H - eps) {
+    const double dr = rho - (R - bf), dz = z - (-H + bf);
+    m = sqrt(dr * dr + dz * dz) >= bf;
+  }
+  return m;
+}
+
+// Outside the conductor, where the BEM field is defined (reference
+// bem/fields.py:200-211 tip, 338-352 well; the post measures r from its
+// axis segment).
+__device__ __forceinline__ bool bem_valid(const spt_geometry& G, double x, double y, double z) {
+  if (G.kind == SPT_GEOM_BEM_WELL) return !well_contains(G, sqrt(x * x + y * y), z);
+  const double r = sqrt(post_r2(x, y, z, G.H));
+  return (r >= G.R * (1.0 - 1e-6)) && (z >= 0.0);
+}
+
 // Does this point need the BEM perturbation? (valid region and w > 0)
 __device__ __forceinline__ bool bem_needed(const spt_geometry& G, double x, double y, double z) {
-  const double r = sqrt(post_r2(x, y, z, G.H));
-  const bool valid = (r >= G.R * (1.0 - 1e-6)) && (z >= 0.0);
-  return valid && blend_weight(z, 0.5 * G.z_handoff, G.z_handoff) > 0.0;
+  return bem_valid(G, x, y, z) && blend_weight(z, 0.5 * G.z_handoff, G.z_handoff) > 0.0;
 }
 
 // Full external field at one point; (qx, qy, qz) is the BEM perturbation
@@ -379,9 +410,9 @@ __device__ void geometry_field_point(const spt_geometry& G, double x, double y, 
     Ez = G.E_gun * (1.0 - R3 / r3 + 3.0 * R3 * z * z / r5);
     return;
   }
-  // BEM post / hemisphere: reference bem/fields.py:200-211, bem/geometry.py:319-324
-  const double r = sqrt(post_r2(x, y, z, G.H));
-  const bool valid = (r >= G.R * (1.0 - 1e-6)) && (z >= 0.0);
+  // BEM post / hemisphere / well: reference bem/fields.py:200-211,338-352,
+  // bem/geometry.py:319-324,779-784
+  const bool valid = bem_valid(G, x, y, z);
   const double w = blend_weight(z, 0.5 * G.z_handoff, G.z_handoff);
   double nx = 0.0, ny = 0.0, nz = 0.0;
   if (valid && w > 0.0) {
@@ -399,7 +430,9 @@ __device__ void geometry_field_point(const spt_geometry& G, double x, double y, 
 // 238-248,304-322; bem/geometry.py:429-436).
 __device__ __forceinline__ bool kill_point(const spt_geometry& G, double x, double y, double z) {
   bool k = false;
-  if (G.kind != SPT_GEOM_FLAT) {
+  if (G.kind == SPT_GEOM_BEM_WELL) {
+    k = well_contains(G, sqrt(x * x + y * y), z);
+  } else if (G.kind != SPT_GEOM_FLAT) {
     const double rk = G.R * (1.0 - 1e-6);
     k = post_r2(x, y, z, G.kind == SPT_GEOM_BEM_POST ? G.H : 0.0) <= rk * rk;
   }
@@ -679,7 +712,7 @@ struct TraceShared {
 template <int T, int NMAX, int PR>
 __device__ void stage_field(const TraceParams& P, int n, TraceShared<T, NMAX>& sh) {
   const int tid = threadIdx.x;
-  if (P.G.kind != SPT_GEOM_BEM_POST) {
+  if (!is_bem(P.G)) {
     if (tid < n && sh.sact[tid])
       geometry_field_point(P.G, sh.sx[tid], sh.sy[tid], sh.sz[tid], 0, 0, 0, sh.fx[tid], sh.fy[tid], sh.fz[tid]);
     return;
@@ -939,7 +972,7 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void geometry_field_kernel(
     sx[tid] = pos[3 * (base + tid)];
     sy[tid] = pos[3 * (base + tid) + 1];
     sz[tid] = pos[3 * (base + tid) + 2];
-    sh.need[tid] = G.kind == SPT_GEOM_BEM_POST && bem_needed(G, sx[tid], sy[tid], sz[tid]);
+    sh.need[tid] = is_bem(G) && bem_needed(G, sx[tid], sy[tid], sz[tid]);
   }
   __syncthreads();
   if (tid == 0) {
@@ -1096,7 +1129,7 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_field_kernel(LockPa
     sy[tid] = LKB(49);
     sz[tid] = LKB(50);
     if (L.sur && L.sur[gi] == 1) act[tid] = 0;  // written by ls_surrogate_kernel
-    sh.need[tid] = act[tid] && G.kind == SPT_GEOM_BEM_POST && bem_needed(G, sx[tid], sy[tid], sz[tid]);
+    sh.need[tid] = act[tid] && is_bem(G) && bem_needed(G, sx[tid], sy[tid], sz[tid]);
   }
   __syncthreads();
   if (tid == 0) {
@@ -1835,11 +1868,18 @@ int spt_bem_pert_field(spt_ctx* c, const double* pos, int64_t n, double* E, int3
 
 static int check_geom(spt_ctx* c, const spt_geometry* g) {
   if (!g) return fail(c, SPT_EINVAL, "null geometry");
-  if (g->kind < SPT_GEOM_FLAT || g->kind > SPT_GEOM_BEM_POST) return fail(c, SPT_EINVAL, "unknown geometry kind");
+  if (g->kind < SPT_GEOM_FLAT || g->kind > SPT_GEOM_BEM_WELL) return fail(c, SPT_EINVAL, "unknown geometry kind");
   if (g->image < SPT_IMAGE_NONE || g->image > SPT_IMAGE_TIP3) return fail(c, SPT_EINVAL, "unknown image model");
-  if (g->kind == SPT_GEOM_BEM_POST) {
+  const bool bem = g->kind == SPT_GEOM_BEM_POST || g->kind == SPT_GEOM_BEM_WELL;
+  if (bem) {
     if (!c->bem_nodes) return fail(c, SPT_ENOBEM, "BEM geometry without an uploaded table");
     if (!(g->z_handoff > 0.0)) return fail(c, SPT_EINVAL, "z_handoff must be positive");
+  }
+  if (g->kind == SPT_GEOM_BEM_WELL) {
+    if (!(g->H > 0.0)) return fail(c, SPT_EINVAL, "well depth H must be positive");
+    if (!(g->bottom_fillet >= 0.0 && g->rim_fillet >= 0.0 && g->bottom_fillet + g->rim_fillet < g->H))
+      return fail(c, SPT_EINVAL, "well fillet radii must be >= 0 with sum < H");
+    if (g->image != SPT_IMAGE_NONE) return fail(c, SPT_EINVAL, "image charges on the well are not supported");
   }
   if (g->kind != SPT_GEOM_FLAT && !(g->R > 0.0)) return fail(c, SPT_EINVAL, "R must be positive");
   return SPT_OK;
@@ -1853,7 +1893,8 @@ int spt_geometry_field(spt_ctx* c, const spt_geometry* g, const double* pos, int
   if (n < 0 || (n > 0 && (!pos || !E))) return fail(c, SPT_EINVAL, "spt_geometry_field: bad arguments");
   if (n == 0) return SPT_OK;
   HIPCHK(c, hipSetDevice(c->device));
-  const int PB = g->kind == SPT_GEOM_BEM_POST ? field_points_per_block(c, n) : FIELD_NMAX;
+  const int PB = (g->kind == SPT_GEOM_BEM_POST || g->kind == SPT_GEOM_BEM_WELL) ? field_points_per_block(c, n)
+                                                                                : FIELD_NMAX;
   const unsigned nb = (unsigned)((n + PB - 1) / PB);
   if (precision == SPT_PREC_MIXED)
     geometry_field_kernel<FIELD_T, FIELD_NMAX, SPT_PREC_MIXED><<<nb, FIELD_T, 0, c->stream>>>(*g, bem_view(c), pos, n,
@@ -2063,7 +2104,8 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
   if (a->use_surrogate) {
     if (a->mode != SPT_MODE_LOCKSTEP) return fail(c, SPT_EINVAL, "the field surrogate runs in lockstep mode only");
     if (!c->have_sur) return fail(c, SPT_EINVAL, "use_surrogate set but no surrogate (spt_surrogate_set)");
-    if (a->geom.kind != SPT_GEOM_BEM_POST) return fail(c, SPT_EINVAL, "the field surrogate is for BEM geometries");
+    if (a->geom.kind != SPT_GEOM_BEM_POST)
+      return fail(c, SPT_EINVAL, "the field surrogate is built for the post-tip geometry");
   }
   if (a->precision == SPT_PREC_MIXED && a->mode == SPT_MODE_PERSISTENT && T != 256)
     return fail(c, SPT_EINVAL, "mixed precision in persistent mode is built for 256-thread workgroups only");
@@ -2136,7 +2178,8 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
       L.V64.p = c->d_sur_p;
       L.V64.frag = c->d_sur_frag;
     }
-    const FieldShape fs = field_shape(c, a->n_particles, a->geom.kind == SPT_GEOM_BEM_POST);
+    const FieldShape fs = field_shape(c, a->n_particles,
+                                      a->geom.kind == SPT_GEOM_BEM_POST || a->geom.kind == SPT_GEOM_BEM_WELL);
     HIPCHK(c, hipEventRecord(c->ev_in, c->stream));
     HIPCHK(c, hipStreamWaitEvent(c->own, c->ev_in, 0));
     int rc3;

@@ -92,3 +92,22 @@ def cap_source(n, R, center_z, theta_max_deg=60.0, mean_ke_ev=0.2, seed=0, t_spr
     t = t_spread * rng.random(n) if t_spread > 0 else np.zeros(n)
     return dict(x=pos[:, 0], y=pos[:, 1], z=pos[:, 2], px=p[:, 0], py=p[:, 1], pz=p[:, 2], t=t,
                 id=np.arange(1, n + 1))
+
+
+def well_bottom_source(group_sizes, R, H, mean_ke_ev=0.1, t_offset_max=10e-12, radius_fraction=0.9, seed=0):
+    """Emission from the bottom disk of a cylindrical well (z = -H),
+    uniform in area within radius_fraction * R, KE ~ Exp(mean),
+    cos-weighted direction about +z; every group gets its own emission-time
+    offset ~ U[0, t_offset_max] (all its members are born then). Returns the
+    source dict with ids 1..n in group order."""
+    sizes = np.asarray(group_sizes, dtype=np.int64)
+    n = int(sizes.sum())
+    rng = np.random.default_rng(seed)
+    r = radius_fraction * R * np.sqrt(rng.random(n))
+    ph = 2.0 * np.pi * rng.random(n)
+    ke = rng.exponential(mean_ke_ev, n)
+    d = _cos_weighted(rng, n)
+    p = _momentum_ev_c(ke)[:, None] * d
+    t = np.repeat(t_offset_max * rng.random(len(sizes)), sizes)
+    return dict(x=r * np.cos(ph), y=r * np.sin(ph), z=np.full(n, -float(H)), px=p[:, 0], py=p[:, 1], pz=p[:, 2],
+                t=t, id=np.arange(1, n + 1))

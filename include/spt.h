@@ -46,6 +46,11 @@ extern "C" {
                             /* + uploaded BEM perturbation, smoothstep blend */
                             /* to the flat cathode between z_handoff/2 and   */
                             /* z_handoff                                     */
+#define SPT_GEOM_BEM_WELL 3 /* CylindricalWellBEMGeometry: well of radius R, */
+                            /* depth H (bottom at z = -H), optional rounded  */
+                            /* bottom/rim corners; uploaded BEM perturbation */
+                            /* of the real truncated-plane surface, same     */
+                            /* blend; killed only inside the conductor       */
 
 /* Image-charge models (reference forces.py:60-191). */
 #define SPT_IMAGE_NONE 0
@@ -58,15 +63,17 @@ typedef struct spt_geometry {
   int32_t kind;        /* SPT_GEOM_*                                         */
   int32_t image;       /* SPT_IMAGE_*                                        */
   double E_gun;        /* asymptotic Ez [V/m]                                */
-  double R;            /* tip / post radius [m]                              */
+  double R;            /* tip / post / well radius [m]                       */
   double H;            /* post height [m] (cap centre at z = H); 0 = bare    */
-                       /* hemisphere                                         */
+                       /* hemisphere; well depth [m]                         */
   double z0;           /* image-plane recession [m]                          */
-  double z_handoff;    /* BEM-to-flat blend height [m] (SPT_GEOM_BEM_POST)   */
+  double z_handoff;    /* BEM-to-flat blend height [m] (BEM kinds)           */
   int32_t has_kill_z;  /* kill at z <= kill_z_below                          */
   int32_t has_z_max;   /* kill at z >= z_max                                 */
   double kill_z_below;
   double z_max;
+  double bottom_fillet; /* well corner rounding radii [m] (0 = sharp)       */
+  double rim_fillet;
 } spt_geometry;
 
 /* One screen crossing (reference screens.py:18-33,96-115). `batch` is the

@@ -348,3 +348,35 @@ def post_bem_field(pos, Ez, R, H, profile, sigma, z_handoff, n_subdiv=8):
     near = np.where(valid[..., None], near, 0.0)
     from .fields import gun_field, blend
     return blend(near, gun_field(pos, Ez), pos[..., 2], 0.5 * z_handoff, z_handoff)
+
+
+def well_contains(rho, z, R, H, bottom_fillet=0.0, rim_fillet=0.0):
+    """Inside (or on) the real cylindrical-well conductor (reference
+    ``bem/fields.py:89-135``): below the plane outside the well radius,
+    below the bottom inside it, the fillet regions decided by the distance
+    to the fillet centre (biased 1e-6 R towards inside)."""
+    eps = 1e-6 * R
+    outside = rho >= R * (1.0 - 1e-6)
+    m = np.where(outside, z <= 0.0, z <= -H * (1.0 + 1e-6))
+    if rim_fillet > 0.0:
+        reg = (rho >= R - eps) & (rho <= R + rim_fillet + eps) & (z >= -rim_fillet - eps) & (z <= eps)
+        m = np.where(reg, np.hypot(rho - (R + rim_fillet), z + rim_fillet) <= rim_fillet, m)
+    if bottom_fillet > 0.0:
+        reg = (rho <= R + eps) & (rho >= R - bottom_fillet - eps) & (z <= -H + bottom_fillet + eps) & (z >= -H - eps)
+        m = np.where(reg, np.hypot(rho - (R - bottom_fillet), z - (-H + bottom_fillet)) >= bottom_fillet, m)
+    return m
+
+
+def well_bem_field(pos, Ez, R, H, profile, sigma, z_handoff, bottom_fillet=0.0, rim_fillet=0.0, n_subdiv=8):
+    """CylindricalWellBEMGeometry.field (reference ``bem/fields.py:324-352``,
+    ``bem/geometry.py:779-784``): uniform + perturbation, zero inside the
+    conductor, blended to the flat-cathode field over [z_handoff/2,
+    z_handoff]."""
+    rho = np.hypot(pos[..., 0], pos[..., 1])
+    valid = ~well_contains(rho, pos[..., 2], R, H, bottom_fillet, rim_fillet)
+    near = np.zeros_like(pos)
+    near[..., 2] = Ez
+    near = near + evaluate_field(profile, sigma, pos, n_subdiv)
+    near = np.where(valid[..., None], near, 0.0)
+    from .fields import gun_field, blend
+    return blend(near, gun_field(pos, Ez), pos[..., 2], 0.5 * z_handoff, z_handoff)

@@ -5,6 +5,8 @@ Geometries are plain dicts:
 
 * ``{"kind": "flat", "E_gun", "z0" (or None), "kill_z_below" (or None)}``
 * ``{"kind": "tip", "E_gun", "R", "z0", "kill_z_below"}`` (analytic)
+* ``{"kind": "well_bem", "E_gun", "R", "H", "profile", "sigma", "z_handoff",
+  "bottom_fillet", "rim_fillet"}`` (cylindrical well, reference bem/geometry.py:510-812)
 * ``{"kind": "tip_bem" | "post_bem", "E_gun", "R", "H" (post only), "profile",
   "sigma", "n_subdiv", "z_handoff", "kill_z_below"}`` (image force off, the
   reference BEM default ``z0=None``)
@@ -39,13 +41,17 @@ def field(geom, pos):
     if k == "post_bem":
         return bem.post_bem_field(pos, geom["E_gun"], geom["R"], geom["H"], geom["profile"], geom["sigma"],
                                   geom["z_handoff"], geom.get("n_subdiv", 8))
+    if k == "well_bem":
+        return bem.well_bem_field(pos, geom["E_gun"], geom["R"], geom["H"], geom["profile"], geom["sigma"],
+                                  geom["z_handoff"], geom.get("bottom_fillet", 0.0), geom.get("rim_fillet", 0.0),
+                                  geom.get("n_subdiv", 8))
     raise ValueError(k)
 
 
 def image_force(geom, pos, q, active, a):
     k = geom["kind"]
     z0 = geom.get("z0")
-    if z0 is None or k in ("tip_bem", "post_bem"):
+    if z0 is None or k in ("tip_bem", "post_bem", "well_bem"):
         return np.zeros_like(pos)
     if k == "flat":
         return forces.flat_image(pos, q, active, z0, a)
@@ -59,6 +65,9 @@ def kill_mask(geom, pos):
     kz = geom.get("kill_z_below", 0.0)
     if k == "flat":
         m = np.zeros(pos.shape[:-1], dtype=bool) if kz is None else pos[..., 2] <= kz
+    elif k == "well_bem":  # reference bem/geometry.py:803-812: the conductor only
+        m = bem.well_contains(np.hypot(pos[..., 0], pos[..., 1]), pos[..., 2], geom["R"], geom["H"],
+                              geom.get("bottom_fillet", 0.0), geom.get("rim_fillet", 0.0))
     else:
         R = geom["R"]
         rk = R * (1.0 - 1e-6)

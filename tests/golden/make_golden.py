@@ -16,6 +16,9 @@ captured here:
   solved by the reference's AxisymmetricBEMSolution.solve;
 * bem_post400.npz -- sigma for the 400-segment post profile of the
   benchmark workload;
+* bem_well.npz    -- reference CylindricalWellBEMGeometry (sharp and
+  rounded corners): profile, sigma, perturbation and blended field at
+  sample points, kill mask; the profile generator with per-region caps;
 * run_*.npz       -- full tracker runs: per-group attempts/accepted, final
   state and screen events in the reference's own recording order.
 """
@@ -119,6 +122,38 @@ def capture_sphere():
     np.savez(os.path.join(HERE, "bem_sphere.npz"), Ez=Ez, R=R, profile=sol.profile, sigma=sol.sigma,
              points=pts, pert=pert, gpoints=gpts, gfield=gfield, z_handoff=geom.z_handoff)
     print(f"bem_sphere: {len(sol.profile)} nodes, {len(pts)} points, {time.time() - t0:.1f} s", flush=True)
+
+
+def capture_well():
+    """Reference CylindricalWellBEMGeometry (z0 None): sharp corners, and
+    rounded corners; profile, sigma, perturbation and blended geometry
+    field at sample points, kill mask."""
+    from specific_particle_tracer.bem import mesh as ref_mesh
+
+    Ez, R, H = -1e8, 50e-9, 30e-9
+    out = {}
+    for tag, kw in (("sharp", dict(field_max_length=3e-9)),
+                    ("fillet", dict(field_max_length=4e-9, bottom_fillet_radius=4e-9, rim_fillet_radius=6e-9))):
+        t0 = time.time()
+        geo = ref_bem_geometry.CylindricalWellBEMGeometry(Ez, R, H, **kw)
+        sol = geo._field.solution
+        rng = np.random.default_rng(11)
+        pts = field_points(sol.profile, R, rng)
+        inside = np.stack([0.5 * R * rng.random(20), np.zeros(20), -H * rng.random(20)], -1)  # in the well
+        pts = np.concatenate([pts, inside])
+        out.update({f"{tag}_profile": sol.profile, f"{tag}_sigma": sol.sigma, f"{tag}_points": pts,
+                    f"{tag}_pert": sol.field(pts), f"{tag}_gfield": geo.field(pts),
+                    f"{tag}_kill": geo.kill_mask(pts), f"{tag}_z_handoff": geo.z_handoff,
+                    f"{tag}_plane_radius": geo.field_plane_radius, f"{tag}_max_length": kw["field_max_length"],
+                    f"{tag}_bottom_fillet": kw.get("bottom_fillet_radius", 0.0),
+                    f"{tag}_rim_fillet": kw.get("rim_fillet_radius", 0.0)})
+        # profile generator alone, with per-region caps
+        out[f"{tag}_profile_regions"] = ref_mesh.cylindrical_well_real_profile(
+            R, H, 0.4e-6, 5e-9, bottom_fillet_radius=kw.get("bottom_fillet_radius"),
+            rim_fillet_radius=kw.get("rim_fillet_radius"), bottom_max_length=3e-9, wall_max_length=2e-9,
+            plane_max_length=4e-9, bottom_fillet_max_length=1e-9, rim_fillet_max_length=1.5e-9, plane_growth=1.2)
+        print(f"bem_well {tag}: {len(sol.profile)} nodes, {time.time() - t0:.1f} s", flush=True)
+    np.savez(os.path.join(HERE, "bem_well.npz"), Ez=Ez, R=R, H=H, **out)
 
 
 def capture_post(n_seg, with_field):
@@ -233,6 +268,13 @@ def main(which):
         capture_post(80, True)
     if "post400" in which:
         capture_post(400, False)
+    if "well" in which:
+        capture_well()
+    if "wellrun" in which:
+        R, H = 50e-9, 30e-9
+        src = emission.well_bottom_source([4, 4, 4], R, H, t_offset_max=10e-15, seed=6)
+        geo = ref_bem_geometry.CylindricalWellBEMGeometry(-1e8, R, H, field_max_length=3e-9)
+        traced_run("well", src, 4, geo, [20e-9, 100e-9], rtol=1e-8, atol=1e-12)
     if "c1" in which:
         src = emission.flat_gaussian_source(64, seed=0)
         traced_run("c1", src, 64, FlatCathode(-1e7, z0=3e-9), [100e-6], rtol=1e-8, atol=1e-12, t_max=1e-9)
@@ -256,4 +298,4 @@ def main(which):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or ["sphere", "post80", "c1", "c1small", "tipbem", "post", "post400"])
+    main(sys.argv[1:] or ["sphere", "post80", "c1", "c1small", "tipbem", "post", "post400", "well", "wellrun"])

@@ -149,6 +149,12 @@ def _tracker_for(name, block_threads=0, mode="auto"):
         R = 1e-6
         prof = ob.sphere_profile(R, ob.n_for_length(R * np.pi, R * np.pi / 39))
         geo = HemisphericalTipBEMGeometry(-1e8, R, sigma=ob.solve(prof, lambda r, z: -1e8 * z))
+    elif name == "well":
+        from bem_amd.bem.geometry import CylindricalWellBEMGeometry
+
+        w = golden("bem_well.npz")
+        geo = CylindricalWellBEMGeometry(float(w["Ez"]), float(w["R"]), float(w["H"]),
+                                         field_max_length=float(w["sharp_max_length"]), sigma=w["sharp_sigma"])
     else:
         p = golden("bem_post80.npz")
         geo = PostTipBEMGeometry(-1e8, float(p["R"]), float(p["H"]), n_segments=80, sigma=p["sigma"])
@@ -159,7 +165,7 @@ def _tracker_for(name, block_threads=0, mode="auto"):
 
 
 @pytest.mark.parametrize("mode", ["persistent", "lockstep"])
-@pytest.mark.parametrize("name", ["c1small", "tipbem", "post80", "c1"])
+@pytest.mark.parametrize("name", ["c1small", "tipbem", "post80", "c1", "well"])
 def test_trace_matches_reference_runs(torch_cuda, name, mode):
     _say("trace", name, mode)
     d, tr = _tracker_for(name, mode=mode)

@@ -39,9 +39,9 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts():
     from bem_amd import _lib
 
-    assert ctypes.sizeof(_lib.Geometry_t) == 72
+    assert ctypes.sizeof(_lib.Geometry_t) == 88
     assert ctypes.sizeof(_lib.Event_t) == 80
-    assert _lib.TraceArgs_t.n_groups.offset == 72
+    assert _lib.TraceArgs_t.n_groups.offset == 88
     assert _lib.TraceArgs_t.screens.offset % 8 == 0
 
 

@@ -34,6 +34,31 @@ def test_bem_solve_matches_reference_sigma(name):
     np.testing.assert_allclose(s, d["sigma"], rtol=1e-11, atol=1e-12 * np.abs(d["sigma"]).max())
 
 
+@pytest.mark.parametrize("tag", ["sharp", "fillet"])
+def test_well_profile_field_and_kill_match_reference_golden(tag):
+    from bem_amd.bem.mesh import cylindrical_well_profile
+
+    d = golden("bem_well.npz")
+    R, H, Ez = float(d["R"]), float(d["H"]), float(d["Ez"])
+    bf, rf = float(d[f"{tag}_bottom_fillet"]), float(d[f"{tag}_rim_fillet"])
+    prof = cylindrical_well_profile(R, H, float(d[f"{tag}_plane_radius"]), float(d[f"{tag}_max_length"]),
+                                    bottom_fillet_radius=bf or None, rim_fillet_radius=rf or None)
+    np.testing.assert_array_equal(prof, d[f"{tag}_profile"])
+    prof2 = cylindrical_well_profile(R, H, 0.4e-6, 5e-9, bottom_fillet_radius=bf or None,
+                                     rim_fillet_radius=rf or None, bottom_max_length=3e-9, wall_max_length=2e-9,
+                                     plane_max_length=4e-9, bottom_fillet_max_length=1e-9,
+                                     rim_fillet_max_length=1.5e-9, plane_growth=1.2)
+    np.testing.assert_array_equal(prof2, d[f"{tag}_profile_regions"])
+    pts = d[f"{tag}_points"]
+    np.testing.assert_array_equal(ob.evaluate_field(d[f"{tag}_profile"], d[f"{tag}_sigma"], pts), d[f"{tag}_pert"])
+    g = ob.well_bem_field(pts, Ez, R, H, d[f"{tag}_profile"], d[f"{tag}_sigma"], float(d[f"{tag}_z_handoff"]), bf, rf)
+    np.testing.assert_array_equal(g, d[f"{tag}_gfield"])
+    k = ob.well_contains(np.hypot(pts[:, 0], pts[:, 1]), pts[:, 2], R, H, bf, rf)
+    np.testing.assert_array_equal(k, d[f"{tag}_kill"])
+    sig = ob.solve(d[f"{tag}_profile"], lambda r, z: Ez * z)
+    assert np.abs(sig - d[f"{tag}_sigma"]).max() / np.abs(d[f"{tag}_sigma"]).max() < 1e-13
+
+
 def test_grounded_sphere_potential_and_field():
     """Solved sphere reproduces the closed form E0 a^3 z / r^3 perturbation
     (reference tests/test_bem_axisymmetric.py:118-134,144-221)."""
@@ -169,6 +194,10 @@ def _geom_for(name):
         prof = ob.sphere_profile(R, ob.n_for_length(R * np.pi, R * np.pi / 39))
         return dict(kind="tip_bem", E_gun=-1e8, R=R, profile=prof, sigma=ob.solve(prof, lambda r, z: -1e8 * z),
                     kill_z_below=0.0)
+    if name == "well":
+        w = golden("bem_well.npz")
+        return dict(kind="well_bem", E_gun=float(w["Ez"]), R=float(w["R"]), H=float(w["H"]),
+                    profile=w["sharp_profile"], sigma=w["sharp_sigma"], z_handoff=float(w["sharp_z_handoff"]))
     p = golden("bem_post80.npz")
     R, H = float(p["R"]), float(p["H"])
     return dict(kind="post_bem", E_gun=-1e8, R=R, H=H, profile=p["profile"], sigma=p["sigma"],
@@ -184,7 +213,7 @@ def run_oracle(name):
     return d, st, out
 
 
-@pytest.mark.parametrize("name", ["c1small", "tipbem", "post80"])
+@pytest.mark.parametrize("name", ["c1small", "tipbem", "post80", "well"])
 def test_oracle_integrator_matches_reference_run(name):
     d, st, out = run_oracle(name)
     np.testing.assert_array_equal(out["attempts"], d["attempts"])
