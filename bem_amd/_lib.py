@@ -29,7 +29,7 @@ EXPORTS = (
     "spt_create", "spt_destroy", "spt_last_error", "spt_set_stream", "spt_sync", "spt_device_cus",
     "spt_bem_upload", "spt_bem_assemble", "spt_bem_solve", "spt_bem_pert_field", "spt_geometry_field",
     "spt_pair_forces", "spt_kill_mask", "spt_trace", "spt_bem_potential", "spt_mlp_nparams", "spt_mlp_grad",
-    "spt_mlp_train", "spt_mlp_infer", "spt_surrogate_set",
+    "spt_mlp_train", "spt_mlp_infer", "spt_surrogate_set", "spt_toroidal_q", "spt_ring_modes",
 )
 MLP_FOURIER, MLP_WIDTH, MLP_DEPTH = 32, 128, 4
 
@@ -133,6 +133,8 @@ def load():
                                              ctypes.c_int32, _p]),
             "spt_mlp_infer": (ctypes.c_int, [_p, P(MlpCfg_t), _p, _p, ctypes.c_int64, ctypes.c_int32, _p]),
             "spt_surrogate_set": (ctypes.c_int, [_p, P(MlpCfg_t), _p]),
+            "spt_toroidal_q": (ctypes.c_int, [_p, _p, ctypes.c_int64, ctypes.c_int32, _p, _p]),
+            "spt_ring_modes": (ctypes.c_int, [_p, _p, _p, _p, _p, ctypes.c_int64, ctypes.c_int32, _p, _p, _p, _p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(lib, name)

@@ -1009,6 +1009,7 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void geometry_field_kernel(
 }
 
 #include "spt_mlp.inc"
+#include "spt_toroidal.inc"
 
 // ============================================================ lockstep mode
 // The reference's schedule (batched_rk45.py:219-319): every running group
@@ -2473,6 +2474,34 @@ int spt_surrogate_set(spt_ctx* c, const spt_mlp_cfg* cfg, const float* params) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->sur_cfg = *cfg;
   c->have_sur = true;
+  return SPT_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int spt_toroidal_q(spt_ctx* c, const double* chi, int64_t n, int32_t n_max, double* q, double* dq) {
+  if (!c) return SPT_EINVAL;
+  if (n < 0 || n_max < 0 || n_max > SPT_TOROIDAL_MAX_M || (n > 0 && (!chi || !q || !dq)))
+    return fail(c, SPT_EINVAL, "spt_toroidal_q: need 0 <= n_max <= 64 and device arrays");
+  if (n == 0) return SPT_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  toroidal_q_kernel<<<(unsigned)((n + 127) / 128), 128, 0, c->stream>>>(chi, n, n_max, q, dq);
+  HIPCHK(c, hipGetLastError());
+  return SPT_OK;
+}
+
+int spt_ring_modes(spt_ctx* c, const double* rho, const double* z, const double* a, const double* za, int64_t n,
+                   int32_t n_max, double* phi, double* dphi_drho, double* dphi_dz, double* g) {
+  if (!c) return SPT_EINVAL;
+  if (n < 0 || n_max < 0 || n_max > SPT_TOROIDAL_MAX_M || (n > 0 && (!rho || !z || !a || !za)))
+    return fail(c, SPT_EINVAL, "spt_ring_modes: need 0 <= n_max <= 64 and device arrays");
+  if (n == 0) return SPT_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  ring_modes_kernel<<<(unsigned)((n + 127) / 128), 128, 0, c->stream>>>(rho, z, a, za, n, n_max, phi, dphi_drho,
+                                                                       dphi_dz, g);
+  HIPCHK(c, hipGetLastError());
   return SPT_OK;
 }
 

@@ -238,6 +238,19 @@ int spt_pair_forces(spt_ctx* ctx, const spt_geometry* geom, const double* pos, c
  * device point. */
 int spt_kill_mask(spt_ctx* ctx, const spt_geometry* geom, const double* pos, int64_t n, uint8_t* out);
 
+/* toroidal_Q (reference bem/toroidal.py:188-219): Q_{m-1/2}(chi) and
+ * dQ/dchi for m = 0..n_max (n_max <= 64) at n device values chi >= 1 into
+ * device q, dq, each (n_max + 1) x n, mode-major. */
+int spt_toroidal_q(spt_ctx* ctx, const double* chi, int64_t n, int32_t n_max, double* q, double* dq);
+
+/* ring_field_modes and point_charge_potential_modes (reference
+ * bem/ring_modes.py:92-158) for n (field, source) pairs (rho, z; a, za):
+ * Phi_m, dPhi_m/drho, dPhi_m/dz of a cos(m phi') ring and the point-charge
+ * coefficient g_m, m = 0..n_max, each (n_max + 1) x n mode-major; any
+ * output may be NULL. */
+int spt_ring_modes(spt_ctx* ctx, const double* rho, const double* z, const double* a, const double* za, int64_t n,
+                   int32_t n_max, double* phi, double* dphi_drho, double* dphi_dz, double* g);
+
 /* Surrogate training and use (no counterpart in the reference). Array
  * arguments are device pointers unless marked host.
  *  spt_mlp_nparams: number of FP32 parameters of the flat layout.
