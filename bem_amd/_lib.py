@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(_HERE, "libspt_hip.so")
 SPT_OK, SPT_EINVAL, SPT_ENOMEM, SPT_EHIP, SPT_ENOBEM, SPT_ESTEPCAP = 0, -1, -2, -3, -4, -5
 SPT_MAX_SCREENS = 16
 GEOM_FLAT, GEOM_TIP, GEOM_BEM_POST, GEOM_BEM_WELL = 0, 1, 2, 3
-IMAGE_NONE, IMAGE_PLANE, IMAGE_TIP3 = 0, 1, 2
+IMAGE_NONE, IMAGE_PLANE, IMAGE_TIP3, IMAGE_BEM = 0, 1, 2, 3
 STATUS_DONE, STATUS_STEPCAP = 0, 1
 MODE_PERSISTENT, MODE_LOCKSTEP = 0, 1
 PREC_FP64, PREC_MIXED, PREC_PAIR32 = 0, 1, 2
@@ -30,6 +30,7 @@ EXPORTS = (
     "spt_bem_upload", "spt_bem_assemble", "spt_bem_solve", "spt_bem_pert_field", "spt_geometry_field",
     "spt_pair_forces", "spt_kill_mask", "spt_trace", "spt_bem_potential", "spt_mlp_nparams", "spt_mlp_grad",
     "spt_mlp_train", "spt_mlp_infer", "spt_surrogate_set", "spt_toroidal_q", "spt_ring_modes",
+    "spt_image_upload", "spt_image_force",
 )
 MLP_FOURIER, MLP_WIDTH, MLP_DEPTH = 32, 128, 4
 
@@ -134,6 +135,10 @@ def load():
             "spt_mlp_infer": (ctypes.c_int, [_p, P(MlpCfg_t), _p, _p, ctypes.c_int64, ctypes.c_int32, _p]),
             "spt_surrogate_set": (ctypes.c_int, [_p, P(MlpCfg_t), _p]),
             "spt_toroidal_q": (ctypes.c_int, [_p, _p, ctypes.c_int64, ctypes.c_int32, _p, _p]),
+            "spt_image_upload": (ctypes.c_int, [_p, _p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                                ctypes.c_double, ctypes.c_double, _p]),
+            "spt_image_force": (ctypes.c_int, [_p, P(Geometry_t), _p, _p, _p, _p, ctypes.c_int32, ctypes.c_double,
+                                               _p]),
             "spt_ring_modes": (ctypes.c_int, [_p, _p, _p, _p, _p, ctypes.c_int64, ctypes.c_int32, _p, _p, _p, _p]),
         }
         for name, (res, args) in sig.items():

@@ -15,7 +15,7 @@ import numpy as np
 
 from .. import _lib
 from .. import geometry as geometry_module
-from .mesh import cylindrical_well_profile, n_for_length, post_profile, sphere_profile
+from .mesh import cylindrical_well_image_profile, cylindrical_well_profile, n_for_length, post_profile, sphere_profile
 from .solver import AxisymmetricBEMSolution
 
 
@@ -146,17 +146,32 @@ class CylindricalWellBEMGeometry(_AxisymmetricBEMGeometry):
     the conductor, the same flat-cathode blend; particles are killed only
     when they re-enter the conductor. Default ``z_handoff`` = 20 max(R, H),
     ``field_plane_radius`` = z_handoff, ``field_max_length`` = min(R, H)/19
-    (the reference's defaults)."""
+    (the reference's defaults).
+
+    With ``z0`` set, the image force is the image-charge BEM on the
+    recessed image surface (``cylindrical_well_image_profile``, plane out to
+    ``image_plane_radius`` = 5 max(R, H), segments <= min(R, H)/14,
+    ``image_n_max`` = 16 azimuthal modes, mirror-charge blend between
+    ``image_d_lo`` R and ``image_d_hi`` R), blended with the plane image at
+    z0 over [z_handoff/2, z_handoff]; the corners then default to rounded
+    (bottom min(z0/2, (H - z0)/5), rim z0 plus that), as in the reference.
+    Traces with it run in lockstep mode."""
 
     def __init__(self, E_gun, R, H, field_plane_radius=None, field_max_length=None, n_subdiv=8,
-                 bottom_fillet_radius=None, rim_fillet_radius=None, z0=None, z_handoff=None, sigma=None, device=0):
-        _no_image(z0)
-        self.z0 = None
+                 bottom_fillet_radius=None, rim_fillet_radius=None, z0=None, image_plane_radius=None,
+                 image_max_length=None, image_n_max=16, image_d_lo=0.1, image_d_hi=0.5, z_handoff=None,
+                 sigma=None, device=0):
+        self.z0 = None if z0 is None else float(z0)
         self.H = float(H)
         R = float(R)
         zh = 20.0 * max(R, self.H) if z_handoff is None else float(z_handoff)
         self.field_plane_radius = zh if field_plane_radius is None else float(field_plane_radius)
         self.field_max_length = field_max_length
+        if self.z0 is not None:
+            avail = self.H - self.z0
+            r = min(0.5 * self.z0, 0.2 * avail)
+            bottom_fillet_radius = r if bottom_fillet_radius is None else float(bottom_fillet_radius)
+            rim_fillet_radius = self.z0 + r if rim_fillet_radius is None else float(rim_fillet_radius)
         self.bottom_fillet_radius = bottom_fillet_radius
         self.rim_fillet_radius = rim_fillet_radius
         max_length = min(R, self.H) / 19.0 if field_max_length is None else float(field_max_length)
@@ -164,23 +179,85 @@ class CylindricalWellBEMGeometry(_AxisymmetricBEMGeometry):
                                            bottom_fillet_radius=bottom_fillet_radius,
                                            rim_fillet_radius=rim_fillet_radius)
         self._setup(E_gun, R, profile, n_subdiv, sigma, zh, None, device)
+        self.image_plane_radius = 5.0 * max(R, self.H) if image_plane_radius is None else float(image_plane_radius)
+        self.image_max_length = min(R, self.H) / 14.0 if image_max_length is None else float(image_max_length)
+        self.image_n_max = int(image_n_max)
+        self.image_d_lo = float(image_d_lo)
+        self.image_d_hi = float(image_d_hi)
+        self.image_profile = None
+        if self.z0 is not None:
+            self.image_profile = cylindrical_well_image_profile(
+                R, self.H, self.z0, self.image_plane_radius, self.image_max_length,
+                bottom_fillet_radius=bottom_fillet_radius, rim_fillet_radius=rim_fillet_radius)
+            self._flat_fallback = geometry_module.FlatCathode(E_gun, z0=self.z0, kill_z_below=None, device=device)
+
+    def _ctx(self):
+        ctx = super()._ctx()
+        if self.image_profile is not None:
+            key = ("image", id(self), self.image_profile.shape[0], self.image_n_max)
+            if getattr(ctx, "image_key", None) != key:
+                prof, pp = _lib.host_ptr(np.asarray(self.image_profile, dtype=np.float64))
+                ctx.check(ctx.lib.spt_image_upload(ctx.h, pp, prof.shape[0], self.image_n_max, 4,
+                                                   self.image_d_lo * self.R, self.image_d_hi * self.R, None),
+                          "spt_image_upload")
+                ctx.image_key = key
+        return ctx
+
+    def image_operators(self):
+        """The per-mode operators (n_max + 1, P, P) as assembled on the device."""
+        ctx = super()._ctx()
+        prof, pp = _lib.host_ptr(np.asarray(self.image_profile, dtype=np.float64))
+        P = prof.shape[0]
+        A = np.zeros((self.image_n_max + 1, P, P))
+        _, ap = _lib.host_ptr(A)
+        ctx.check(ctx.lib.spt_image_upload(ctx.h, pp, P, self.image_n_max, 4, self.image_d_lo * self.R,
+                                           self.image_d_hi * self.R, ap), "spt_image_upload")
+        ctx.image_key = ("image", id(self), P, self.image_n_max)
+        return A
 
     def spt_geometry(self, z_max=None):
         g = super().spt_geometry(z_max)
         g.kind = _lib.GEOM_BEM_WELL
         g.bottom_fillet = float(self.bottom_fillet_radius or 0.0)
         g.rim_fillet = float(self.rim_fillet_radius or 0.0)
+        if self.z0 is not None:
+            g.image = _lib.IMAGE_BEM
+            g.z0 = self.z0
         return g
+
+    def image_force(self, position, charge, active, plummer_radius):
+        if self.z0 is None:
+            return super().image_force(position, charge, active, plummer_radius)
+        import torch
+
+        ctx = self._ctx()
+        g = self.spt_geometry()
+        is_np = not hasattr(position, "detach")
+        P = torch.as_tensor(np.asarray(position) if is_np else position, dtype=torch.float64, device=ctx.torch_device)
+        G, E, _ = P.shape
+        q = torch.as_tensor(np.asarray(charge) if is_np else charge, dtype=torch.float64, device=ctx.torch_device)
+        a = torch.as_tensor(np.asarray(active) if is_np else active, device=ctx.torch_device).to(torch.uint8)
+        P, q, a = P.reshape(-1, 3).contiguous(), q.reshape(-1).contiguous(), a.reshape(-1).contiguous()
+        off = torch.arange(0, G * E + 1, E, dtype=torch.int64, device=ctx.torch_device)
+        out = torch.zeros_like(P)
+        ctx.bind_stream()
+        ctx.check(ctx.lib.spt_image_force(ctx.h, g, _lib.ptr(P), _lib.ptr(q), _lib.ptr(a), _lib.ptr(off), G,
+                                          float(plummer_radius), _lib.ptr(out)), "spt_image_force")
+        ctx.sync()
+        out = out.reshape(G, E, 3)
+        return out.cpu().numpy() if is_np else out
 
     def worker_args(self):
         return ("cylindrical_well_bem", self.E_gun, self.R, self.H, self.field_plane_radius, self.field_max_length,
-                self.n_subdiv, self.bottom_fillet_radius, self.rim_fillet_radius, None, self.z_handoff)
+                self.n_subdiv, self.bottom_fillet_radius, self.rim_fillet_radius, self.z0, self.image_plane_radius,
+                self.image_max_length, self.image_n_max, self.image_d_lo, self.image_d_hi, self.z_handoff)
 
     @classmethod
     def _from_worker_args(cls, args):
-        E_gun, R, H, fpr, fml, n_subdiv, bf, rf, z0, z_handoff = args
+        (E_gun, R, H, fpr, fml, n_subdiv, bf, rf, z0, ipr, iml, inm, idl, idh, z_handoff) = args
         return cls(E_gun, R, H, field_plane_radius=fpr, field_max_length=fml, n_subdiv=n_subdiv,
-                   bottom_fillet_radius=bf, rim_fillet_radius=rf, z0=z0, z_handoff=z_handoff)
+                   bottom_fillet_radius=bf, rim_fillet_radius=rf, z0=z0, image_plane_radius=ipr,
+                   image_max_length=iml, image_n_max=inm, image_d_lo=idl, image_d_hi=idh, z_handoff=z_handoff)
 
 
 geometry_module._WORKER_REGISTRY["cylindrical_well_bem"] = CylindricalWellBEMGeometry

@@ -136,3 +136,52 @@ def cylindrical_well_profile(R, H, plane_radius, max_length, bottom_fillet_radiu
     parts.append(graded_annulus(R + rr, plane_radius, 0.0, region_max_length(max_length, plane_max_length),
                                 growth=plane_growth)[1:])
     return np.concatenate(parts, axis=0)
+
+
+def cylindrical_well_image_profile(R, H, z0, plane_radius, max_length, bottom_fillet_radius=None,
+                                   rim_fillet_radius=None, bottom_max_length=None, bottom_fillet_max_length=None,
+                                   wall_max_length=None, rim_fillet_max_length=None, plane_max_length=None,
+                                   plane_growth=1.15):
+    """The recessed image surface of the well: the real surface moved z0
+    into the conductor (reference ``bem/mesh.py:412-529``). The bottom disk
+    sits at z = -H - z0, the wall at rho = R + z0, the plane at z = -z0;
+    the bottom (reflex) corner is always rounded with radius
+    bottom_fillet + z0 about the real fillet's centre, the rim (convex)
+    corner keeps a rounding of radius rim_fillet - z0 only when that is
+    positive (else the sharp mitre at (R + z0, -z0))."""
+    if not (0.0 < z0 < min(R, H)):
+        raise ValueError(f"z0 must satisfy 0 < z0 < min(R, H) (got z0={z0!r}, R={R!r}, H={H!r})")
+    rb = 0.0 if bottom_fillet_radius is None else float(bottom_fillet_radius)
+    rr = 0.0 if rim_fillet_radius is None else float(rim_fillet_radius)
+    if rb < 0.0:
+        raise ValueError(f"bottom_fillet_radius must be >= 0 (got {bottom_fillet_radius!r})")
+    if rr < 0.0:
+        raise ValueError(f"rim_fillet_radius must be >= 0 (got {rim_fillet_radius!r})")
+    top = max(rr, z0)
+    if rb + top >= H:
+        raise ValueError("bottom_fillet_radius + max(rim_fillet_radius, z0) must be < H "
+                         f"(got {rb!r} + {top!r} >= {H!r})")
+    img_bottom = rb + z0
+    img_rim = rr - z0
+    parts = []
+    n = n_for_length(R - rb, region_max_length(max_length, bottom_max_length))
+    parts.append(flat_annulus(0.0, R - rb, -H - z0, n))
+    n = n_for_length(0.5 * np.pi * img_bottom, region_max_length(max_length, bottom_fillet_max_length),
+                     min_points=MIN_FILLET_SEGMENTS + 1)
+    a = arc(R - rb, img_bottom, -0.5 * np.pi, 0.0, n)[1:]
+    a[:, 1] += -H + rb
+    parts.append(a)
+    n = n_for_length(-top - (-H + rb), region_max_length(max_length, wall_max_length))
+    parts.append(wall(R + z0, -H + rb, -top, n)[1:])
+    if img_rim > 0.0:
+        n = n_for_length(0.5 * np.pi * img_rim, region_max_length(max_length, rim_fillet_max_length),
+                         min_points=MIN_FILLET_SEGMENTS + 1)
+        a = arc(R + rr, img_rim, np.pi, 0.5 * np.pi, n)[1:]
+        a[:, 1] += -rr
+        parts.append(a)
+        r0 = R + rr
+    else:
+        r0 = R + z0
+    parts.append(graded_annulus(r0, plane_radius, -z0, region_max_length(max_length, plane_max_length),
+                                growth=plane_growth)[1:])
+    return np.concatenate(parts, axis=0)

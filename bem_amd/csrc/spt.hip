@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
+#include <vector>
 
 #include <math.h>
 #include <stdio.h>
@@ -1010,6 +1011,7 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void geometry_field_kernel(
 
 #include "spt_mlp.inc"
 #include "spt_toroidal.inc"
+#include "spt_image.inc"
 
 // ============================================================ lockstep mode
 // The reference's schedule (batched_rk45.py:219-319): every running group
@@ -1036,6 +1038,7 @@ struct LockParams {
   uint8_t* sur;            // per particle: surrogate flag of this stage
   MlpView V;               // the surrogate (when sur != nullptr): split BF16
   Mlp64View V64;           // and FP64 forms
+  const double* imgF;      // image-charge BEM force per particle (n x 3), or nullptr
 };
 
 __global__ void ls_pgroup_kernel(const int64_t* offsets, int n_groups, int32_t* pgroup) {
@@ -1273,6 +1276,8 @@ __global__ __launch_bounds__(TG) void ls_accel_kernel(LockParams L, int st) {
   const double hr = L.g_hr[g];
   double Fc[3], Fi[3];
   pair_forces_p<PR>(L.T.G, tid, n, sx, sy, sz, sq, act, L.T.a2, Fc, Fi);
+  if (L.imgF)
+    for (int c = 0; c < 3; ++c) Fi[c] = L.imgF[3 * gi + c];
   const double mass = L.T.mass[gi];
   for (int c = 0; c < 3; ++c) {
     const double ac = n == 1 ? 0.0 : Fc[c] / mass;
@@ -1599,6 +1604,19 @@ struct spt_ctx {
   float* d_train = nullptr;
   size_t train_floats = 0;
   int train_batch = 0;
+  // image-charge BEM table and per-call scratch
+  int img_P = 0, img_modes = 0, img_quad = 0;
+  double img_floor = 0.0, img_dlo = 0.0, img_dhi = 0.0;
+  double* d_img_prof = nullptr;
+  double* d_img_segs = nullptr;
+  double* d_img_quadv = nullptr;
+  double* d_img_Ainv = nullptr;
+  double* d_img_rhs = nullptr;
+  double* d_img_sigma = nullptr;
+  size_t img_groups_cap = 0;
+  double* d_img_mirror = nullptr;
+  double* d_img_F = nullptr;
+  size_t img_part_cap = 0;
 };
 
 static int fail(spt_ctx* c, int code, const std::string& msg) {
@@ -1679,6 +1697,9 @@ void spt_destroy(spt_ctx* c) {
   if (c->d_mlp_frag) (void)hipFree(c->d_mlp_frag);
   if (c->d_mlp_split) (void)hipFree(c->d_mlp_split);
   if (c->d_train) (void)hipFree(c->d_train);
+  for (double* p : {c->d_img_prof, c->d_img_segs, c->d_img_quadv, c->d_img_Ainv, c->d_img_rhs, c->d_img_sigma,
+                    c->d_img_mirror, c->d_img_F})
+    if (p) (void)hipFree(p);
   if (c->ev_in) (void)hipEventDestroy(c->ev_in);
   if (c->ev_out) (void)hipEventDestroy(c->ev_out);
   if (c->own) (void)hipStreamDestroy(c->own);
@@ -1870,7 +1891,11 @@ int spt_bem_pert_field(spt_ctx* c, const double* pos, int64_t n, double* E, int3
 static int check_geom(spt_ctx* c, const spt_geometry* g) {
   if (!g) return fail(c, SPT_EINVAL, "null geometry");
   if (g->kind < SPT_GEOM_FLAT || g->kind > SPT_GEOM_BEM_WELL) return fail(c, SPT_EINVAL, "unknown geometry kind");
-  if (g->image < SPT_IMAGE_NONE || g->image > SPT_IMAGE_TIP3) return fail(c, SPT_EINVAL, "unknown image model");
+  if (g->image < SPT_IMAGE_NONE || g->image > SPT_IMAGE_BEM) return fail(c, SPT_EINVAL, "unknown image model");
+  if (g->image == SPT_IMAGE_BEM) {
+    if (g->kind != SPT_GEOM_BEM_WELL) return fail(c, SPT_EINVAL, "the image-charge BEM is built for the well");
+    if (!c->img_P) return fail(c, SPT_ENOBEM, "image-charge BEM without an uploaded image table");
+  }
   const bool bem = g->kind == SPT_GEOM_BEM_POST || g->kind == SPT_GEOM_BEM_WELL;
   if (bem) {
     if (!c->bem_nodes) return fail(c, SPT_ENOBEM, "BEM geometry without an uploaded table");
@@ -1880,7 +1905,8 @@ static int check_geom(spt_ctx* c, const spt_geometry* g) {
     if (!(g->H > 0.0)) return fail(c, SPT_EINVAL, "well depth H must be positive");
     if (!(g->bottom_fillet >= 0.0 && g->rim_fillet >= 0.0 && g->bottom_fillet + g->rim_fillet < g->H))
       return fail(c, SPT_EINVAL, "well fillet radii must be >= 0 with sum < H");
-    if (g->image != SPT_IMAGE_NONE) return fail(c, SPT_EINVAL, "image charges on the well are not supported");
+    if (g->image != SPT_IMAGE_NONE && g->image != SPT_IMAGE_BEM)
+      return fail(c, SPT_EINVAL, "the well takes no image charges or the image-charge BEM");
   }
   if (g->kind != SPT_GEOM_FLAT && !(g->R > 0.0)) return fail(c, SPT_EINVAL, "R must be positive");
   return SPT_OK;
@@ -1967,6 +1993,67 @@ static int launch_trace(spt_ctx* c, const TraceParams& P, int n_groups) {
 }
 
 
+static ImageView image_view(const spt_ctx* c) {
+  ImageView B;
+  B.P = c->img_P;
+  B.n_modes = c->img_modes;
+  B.n_quad = c->img_quad;
+  B.prof = c->d_img_prof;
+  B.segs = c->d_img_segs;
+  B.quad = c->d_img_quadv;
+  B.Ainv = c->d_img_Ainv;
+  B.rho_floor = c->img_floor;
+  B.d_lo = c->img_dlo;
+  B.d_hi = c->img_dhi;
+  B.rhs = c->d_img_rhs;
+  B.sigma = c->d_img_sigma;
+  B.mirror = c->d_img_mirror;
+  return B;
+}
+
+static int ensure_image_scratch(spt_ctx* c, int n_groups, int64_t n) {
+  if ((size_t)n_groups > c->img_groups_cap) {
+    if (c->d_img_rhs) (void)hipFree(c->d_img_rhs);
+    if (c->d_img_sigma) (void)hipFree(c->d_img_sigma);
+    c->d_img_rhs = c->d_img_sigma = nullptr;
+    c->img_groups_cap = 0;
+    const size_t per = (size_t)2 * IMG_MAX_MODES * (size_t)c->img_P;
+    HIPCHK(c, hipMalloc(&c->d_img_rhs, sizeof(double) * per * n_groups));
+    HIPCHK(c, hipMalloc(&c->d_img_sigma, sizeof(double) * per * n_groups));
+    c->img_groups_cap = n_groups;
+  }
+  if ((size_t)n > c->img_part_cap) {
+    if (c->d_img_mirror) (void)hipFree(c->d_img_mirror);
+    if (c->d_img_F) (void)hipFree(c->d_img_F);
+    c->d_img_mirror = c->d_img_F = nullptr;
+    c->img_part_cap = 0;
+    HIPCHK(c, hipMalloc(&c->d_img_mirror, sizeof(double) * 4 * n));
+    HIPCHK(c, hipMalloc(&c->d_img_F, sizeof(double) * 3 * n));
+    c->img_part_cap = n;
+  }
+  return SPT_OK;
+}
+
+static PosView stage_pos_view(const LockParams& L) {
+  PosView X;
+  X.x = L.T.kbuf + (size_t)48 * L.T.n_total;
+  X.y = L.T.kbuf + (size_t)49 * L.T.n_total;
+  X.z = L.T.kbuf + (size_t)50 * L.T.n_total;
+  X.stride = 1;
+  return X;
+}
+
+// The three image-charge kernels for every (running) group; the scratch
+// must have been sized by ensure_image_scratch.
+static void launch_image_force(spt_ctx* c, const spt_geometry& G, PosView X, const double* charge,
+                               const uint8_t* active, const int64_t* offsets, const int32_t* gflags, int running,
+                               int n_groups, double a2, double* F, hipStream_t s) {
+  const ImageView B = image_view(c);
+  img_rhs_kernel<<<n_groups, 256, 0, s>>>(B, X, charge, active, offsets, gflags, running);
+  img_solve_kernel<<<dim3(n_groups, 2 * B.n_modes), 128, 0, s>>>(B, gflags, running);
+  img_force_kernel<<<n_groups, 256, 0, s>>>(B, G, X, charge, active, offsets, gflags, running, a2, F);
+}
+
 constexpr int kLockstepUnroll = 4;  // attempts per captured graph replay
 
 // PR: SPT_PREC_FP64, SPT_PREC_MIXED (FP32 far-field sums and pairs) or
@@ -1998,6 +2085,8 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
         ls_field_kernel<192, 64, FPR><<<nb_field, 192, 0, s>>>(L, PB);
       else
         ls_field_kernel<256, 64, FPR><<<nb_field, 256, 0, s>>>(L, PB);
+      if (L.imgF) launch_image_force(c, L.T.G, stage_pos_view(L), L.T.charge, L.active, L.T.offsets, L.g_flags,
+                                     LS_RUNNING, n_groups, L.T.a2, (double*)L.imgF, s);
       ls_accel_kernel<TG, APR><<<n_groups, TG, 0, s>>>(L, st);
     }
     ls_end_kernel<TG><<<n_groups, TG, 0, s>>>(L);
@@ -2102,6 +2191,8 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
     return fail(c, SPT_EINVAL, "unknown precision");
   if (a->precision == SPT_PREC_PAIR32 && a->mode != SPT_MODE_LOCKSTEP)
     return fail(c, SPT_EINVAL, "precision PAIR32 (FP64 field, FP32 pairs) runs in lockstep mode only");
+  if (a->geom.image == SPT_IMAGE_BEM && a->mode != SPT_MODE_LOCKSTEP)
+    return fail(c, SPT_EINVAL, "the image-charge BEM runs in lockstep mode only");
   if (a->use_surrogate) {
     if (a->mode != SPT_MODE_LOCKSTEP) return fail(c, SPT_EINVAL, "the field surrogate runs in lockstep mode only");
     if (!c->have_sur) return fail(c, SPT_EINVAL, "use_surrogate set but no surrogate (spt_surrogate_set)");
@@ -2171,6 +2262,12 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
     L.sur = a->use_surrogate ? c->d_sur : nullptr;
     memset(&L.V, 0, sizeof(L.V));
     memset(&L.V64, 0, sizeof(L.V64));
+    L.imgF = nullptr;
+    if (a->geom.image == SPT_IMAGE_BEM) {
+      int rc4 = ensure_image_scratch(c, a->n_groups, a->n_particles);
+      if (rc4) return rc4;
+      L.imgF = c->d_img_F;
+    }
     if (a->use_surrogate) {
       L.V.cfg = c->sur_cfg;
       L.V.p = c->d_sur_p;
@@ -2283,6 +2380,9 @@ static int ensure_mlp_train(spt_ctx* c, int B) {
   const size_t need = mlp_train_floats(B) + 64 * 32;
   if (need > c->train_floats) {
     if (c->d_train) (void)hipFree(c->d_train);
+  for (double* p : {c->d_img_prof, c->d_img_segs, c->d_img_quadv, c->d_img_Ainv, c->d_img_rhs, c->d_img_sigma,
+                    c->d_img_mirror, c->d_img_F})
+    if (p) (void)hipFree(p);
     c->d_train = nullptr;
     c->train_floats = 0;
     HIPCHK(c, hipMalloc(&c->d_train, sizeof(float) * need));
@@ -2501,6 +2601,113 @@ int spt_ring_modes(spt_ctx* c, const double* rho, const double* z, const double*
   HIPCHK(c, hipSetDevice(c->device));
   ring_modes_kernel<<<(unsigned)((n + 127) / 128), 128, 0, c->stream>>>(rho, z, a, za, n, n_max, phi, dphi_drho,
                                                                        dphi_dz, g);
+  HIPCHK(c, hipGetLastError());
+  return SPT_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int spt_image_upload(spt_ctx* c, const double* prof, int32_t n, int32_t n_max, int32_t n_sub, double d_lo,
+                     double d_hi, double* A_out) {
+  if (!c) return SPT_EINVAL;
+  if (!prof || n < 3 || n_max < 0 || n_max >= IMG_MAX_MODES || n_sub < 1 || n_sub > 64)
+    return fail(c, SPT_EINVAL, "spt_image_upload: need n_nodes >= 3, 0 <= n_max <= 32, 1 <= n_subdiv <= 64");
+  HIPCHK(c, hipSetDevice(c->device));
+  for (double** p : {&c->d_img_prof, &c->d_img_segs, &c->d_img_quadv, &c->d_img_Ainv, &c->d_img_rhs,
+                     &c->d_img_sigma}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  c->img_P = 0;
+  c->img_groups_cap = 0;
+  const int M = n_max + 1, ns = n - 1, nq = ns * n_sub * 4;
+  double rmax = 0.0;
+  for (int i = 0; i < n; ++i) rmax = prof[2 * i] > rmax ? prof[2 * i] : rmax;
+  const double floor_ = 1e-6 * (rmax > 1e-300 ? rmax : 1e-300);
+  HIPCHK(c, hipMalloc(&c->d_img_prof, sizeof(double) * 2 * n));
+  HIPCHK(c, hipMalloc(&c->d_img_segs, sizeof(double) * IMG_SEG * ns));
+  HIPCHK(c, hipMalloc(&c->d_img_quadv, sizeof(double) * 5 * nq));
+  HIPCHK(c, hipMalloc(&c->d_img_Ainv, sizeof(double) * M * n * n));
+  HIPCHK(c, hipMemcpyAsync(c->d_img_prof, prof, sizeof(double) * 2 * n, hipMemcpyHostToDevice, c->stream));
+  img_seg_table_kernel<<<(ns + 255) / 256, 256, 0, c->stream>>>(c->d_img_prof, ns, c->d_img_segs);
+  img_quad_kernel<<<(nq + 255) / 256, 256, 0, c->stream>>>(c->d_img_prof, ns, n_sub, c->d_img_quadv);
+  double *lo = nullptr, *hi = nullptr;
+  HIPCHK(c, hipMalloc(&lo, sizeof(double) * M * n * ns));
+  HIPCHK(c, hipMalloc(&hi, sizeof(double) * M * n * ns));
+  const int64_t pairs = (int64_t)n * ns;
+  img_assemble_kernel<<<(unsigned)((pairs + 63) / 64), 64, 0, c->stream>>>(c->d_img_prof, n, M, floor_, lo, hi);
+  img_combine_kernel<<<(unsigned)(((int64_t)M * n * n + 255) / 256), 256, 0, c->stream>>>(lo, hi, n, M,
+                                                                                         c->d_img_Ainv);
+  HIPCHK(c, hipGetLastError());
+  if (A_out) {  // operators before inversion, (m, i, j) row-major
+    std::vector<double> h((size_t)M * n * n);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->d_img_Ainv, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int m = 0; m < M; ++m)
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) A_out[((size_t)m * n + i) * n + j] = h[((size_t)m * n + j) * n + i];
+  }
+  (void)hipFree(lo);
+  (void)hipFree(hi);
+  if (!c->blas) {
+    if (rocblas_create_handle(&c->blas) != rocblas_status_success) return fail(c, SPT_EHIP, "rocblas_create_handle");
+    rocblas_set_stream(c->blas, c->stream);
+  }
+  rocblas_int *ipiv = nullptr, *info = nullptr;
+  HIPCHK(c, hipMalloc(&ipiv, sizeof(rocblas_int) * n));
+  HIPCHK(c, hipMalloc(&info, sizeof(rocblas_int)));
+  int bad = 0;
+  for (int m = 0; m < M && !bad; ++m) {
+    double* A = c->d_img_Ainv + (size_t)m * n * n;
+    rocblas_status s1 = rocsolver_dgetrf(c->blas, n, n, A, n, ipiv, info);
+    rocblas_int h_info = 0;
+    HIPCHK(c, hipMemcpyAsync(&h_info, info, sizeof(rocblas_int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    rocblas_status s2 = rocsolver_dgetri(c->blas, n, A, n, ipiv, info);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (s1 != rocblas_status_success || s2 != rocblas_status_success || h_info != 0) bad = 1;
+  }
+  (void)hipFree(ipiv);
+  (void)hipFree(info);
+  if (bad) return fail(c, SPT_EHIP, "image operator inversion failed (singular mode operator?)");
+  c->img_P = n;
+  c->img_modes = M;
+  c->img_quad = nq;
+  c->img_floor = floor_;
+  c->img_dlo = d_lo;
+  c->img_dhi = d_hi;
+  return SPT_OK;
+}
+
+int spt_image_force(spt_ctx* c, const spt_geometry* g, const double* pos, const double* charge,
+                    const uint8_t* active, const int64_t* offsets, int32_t n_groups, double plummer_radius,
+                    double* F) {
+  if (!c) return SPT_EINVAL;
+  int rc = check_geom(c, g);
+  if (rc) return rc;
+  if (g->image != SPT_IMAGE_BEM) return fail(c, SPT_EINVAL, "spt_image_force needs image = SPT_IMAGE_BEM");
+  if (n_groups < 0 || (n_groups > 0 && (!pos || !charge || !active || !offsets || !F)))
+    return fail(c, SPT_EINVAL, "spt_image_force: bad arguments");
+  if (n_groups == 0) return SPT_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  int64_t last = 0;
+  std::vector<int64_t> h(n_groups + 1);
+  HIPCHK(c, hipMemcpy(h.data(), offsets, sizeof(int64_t) * (n_groups + 1), hipMemcpyDeviceToHost));
+  for (int i = 0; i < n_groups; ++i)
+    if (h[i + 1] - h[i] < 1 || h[i + 1] - h[i] > 256)
+      return fail(c, SPT_EINVAL, "groups must hold 1 to 256 particles");
+  last = h[n_groups];
+  rc = ensure_image_scratch(c, n_groups, last);
+  if (rc) return rc;
+  PosView X;
+  X.x = pos;
+  X.y = pos + 1;
+  X.z = pos + 2;
+  X.stride = 3;
+  launch_image_force(c, *g, X, charge, active, offsets, nullptr, 0, n_groups, plummer_radius * plummer_radius, F,
+                     c->stream);
   HIPCHK(c, hipGetLastError());
   return SPT_OK;
 }

@@ -153,7 +153,10 @@ class SpecificParticleTracer:
         return np.argsort(-sizes, kind="stable").astype(np.int32)
 
     def _mode_code(self, ctx):
-        if self.surrogate is not None or self.precision == "pairs32":
+        if getattr(self.geometry, "image_profile", None) is not None and self.mode == "persistent":
+            raise ValueError("the image-charge BEM runs in lockstep mode only")
+        if (self.surrogate is not None or self.precision == "pairs32"
+                or getattr(self.geometry, "image_profile", None) is not None):
             return _lib.MODE_LOCKSTEP
         if self.mode == "persistent":
             return _lib.MODE_PERSISTENT

@@ -56,6 +56,9 @@ extern "C" {
 #define SPT_IMAGE_NONE 0
 #define SPT_IMAGE_PLANE 1  /* image at z' = -2 z0 - z, charge -q            */
 #define SPT_IMAGE_TIP3 2   /* sphere R - z0 + plane z0: three images        */
+#define SPT_IMAGE_BEM 3    /* well: image-charge BEM on the uploaded image  */
+                           /* surface near, plane image z0 far, blended     */
+                           /* (spt_image_upload; LOCKSTEP traces only)      */
 
 typedef struct spt_ctx spt_ctx;
 
@@ -237,6 +240,25 @@ int spt_pair_forces(spt_ctx* ctx, const spt_geometry* geom, const double* pos, c
 /* Geometry.kill_mask (+ z_max, reference geometry.py:304-322): one byte per
  * device point. */
 int spt_kill_mask(spt_ctx* ctx, const spt_geometry* geom, const double* pos, int64_t n, uint8_t* out);
+
+/* ImageChargeBEMSolution.solve (reference bem/image_charge.py:469-493,
+ * 538-566): upload the image surface profile (host, n_nodes x 2), assemble
+ * the per-mode single-layer operators m = 0..n_max (n_max <= 32) on the
+ * device and invert them (rocSOLVER getrf/getri); mirror-charge blend
+ * distances d_lo < d_hi [m] and the field-quadrature n_subdiv are kept
+ * for spt_image_force / spt_trace. A_out (host, optional): the operators
+ * before inversion, (n_max + 1) x n x n row-major. */
+int spt_image_upload(spt_ctx* ctx, const double* profile_rz, int32_t n_nodes, int32_t n_max, int32_t n_subdiv,
+                     double d_lo, double d_hi, double* A_out);
+
+/* CylindricalWellBEMGeometry.image_force with z0 (reference
+ * bem/geometry.py:786-801, bem/image_charge.py:816-1056): per group (CSR
+ * offsets, 1..256 members) the joint image-charge BEM force, blended with
+ * the plane image at z0 over [z_handoff/2, z_handoff]; device arrays, F
+ * (n x 3) [N]. geom->image must be SPT_IMAGE_BEM. */
+int spt_image_force(spt_ctx* ctx, const spt_geometry* geom, const double* pos, const double* charge,
+                    const uint8_t* active, const int64_t* group_offsets, int32_t n_groups, double plummer_radius,
+                    double* F);
 
 /* toroidal_Q (reference bem/toroidal.py:188-219): Q_{m-1/2}(chi) and
  * dQ/dchi for m = 0..n_max (n_max <= 64) at n device values chi >= 1 into

@@ -51,6 +51,15 @@ def field(geom, pos):
 def image_force(geom, pos, q, active, a):
     k = geom["kind"]
     z0 = geom.get("z0")
+    if k == "well_bem" and z0 is not None:
+        # reference bem/geometry.py:786-801: image BEM near, plane image far, blended
+        from . import image
+
+        near = image.image_force(geom["image_profile"], geom["image_A_inv"], geom["n_max"], pos, q, active,
+                                 geom["d_lo"], geom["d_hi"])
+        far = forces.flat_image(pos, q, active, z0, a)
+        from .fields import blend
+        return blend(near, far, pos[..., 2], 0.5 * geom["z_handoff"], geom["z_handoff"])
     if z0 is None or k in ("tip_bem", "post_bem", "well_bem"):
         return np.zeros_like(pos)
     if k == "flat":

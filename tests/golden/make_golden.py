@@ -16,6 +16,8 @@ captured here:
   solved by the reference's AxisymmetricBEMSolution.solve;
 * bem_post400.npz -- sigma for the 400-segment post profile of the
   benchmark workload;
+* bem_well_image.npz -- reference CylindricalWellBEMGeometry with z0: image
+  profile, per-mode operators, image force of a particle batch;
 * bem_well.npz    -- reference CylindricalWellBEMGeometry (sharp and
   rounded corners): profile, sigma, perturbation and blended field at
   sample points, kill mask; the profile generator with per-region caps;
@@ -156,6 +158,45 @@ def capture_well():
     np.savez(os.path.join(HERE, "bem_well.npz"), Ez=Ez, R=R, H=H, **out)
 
 
+def well_image_batch(R, H, rng):
+    """(3, 4) particles near the well bottom, wall, rim and above the plane;
+    one inactive."""
+    pos = np.zeros((3, 4, 3))
+    ph = 2 * np.pi * rng.random((3, 4))
+    rz = np.array([[[0.3 * R, -H + 2e-9], [0.5 * R, -H + 6e-9], [0.8 * R, -0.5 * H], [0.2 * R, -H + 12e-9]],
+                   [[0.95 * R, -0.4 * H], [0.9 * R, -3e-9], [1.1 * R, 4e-9], [0.6 * R, 2e-9]],
+                   [[1.5 * R, 5e-9], [2.0 * R, 20e-9], [0.1 * R, 10e-9], [0.0, -0.5 * H]]])
+    pos[..., 0] = rz[..., 0] * np.cos(ph)
+    pos[..., 1] = rz[..., 0] * np.sin(ph)
+    pos[..., 2] = rz[..., 1]
+    charge = np.full((3, 4), -1.602176634e-19)
+    active = np.ones((3, 4), dtype=bool)
+    active[1, 2] = False
+    return pos, charge, active
+
+
+def capture_well_image():
+    """Reference CylindricalWellBEMGeometry with z0 = 3 nm: image profile,
+    per-mode operators, the image force of a particle batch (bare BEM and
+    blended geometry force)."""
+    Ez, R, H, z0 = -1e8, 50e-9, 30e-9, 3e-9
+    t0 = time.time()
+    geo = ref_bem_geometry.CylindricalWellBEMGeometry(Ez, R, H, z0=z0, field_max_length=3e-9,
+                                                      image_max_length=4e-9, image_n_max=8)
+    sol = geo.image_solution
+    pos, charge, active = well_image_batch(R, H, np.random.default_rng(21))
+    near = sol.image_force(pos, charge, active, d_lo=geo.image_d_lo * R, d_hi=geo.image_d_hi * R, n_max=8)
+    force = geo.image_force(pos, charge, active, 1e-12)
+    real = geo._field.solution
+    np.savez(os.path.join(HERE, "bem_well_image.npz"), Ez=Ez, R=R, H=H, z0=z0, n_max=8,
+             d_lo=geo.image_d_lo * R, d_hi=geo.image_d_hi * R, image_profile=sol.profile, A=sol.A,
+             real_profile=real.profile, sigma=real.sigma, bottom_fillet=geo.bottom_fillet_radius,
+             rim_fillet=geo.rim_fillet_radius, z_handoff=geo.z_handoff, image_plane_radius=geo.image_plane_radius,
+             image_max_length=geo.image_max_length, pos=pos, charge=charge, active=active, near=near, force=force)
+    print(f"bem_well_image: {len(sol.profile)} image nodes, {len(real.profile)} real nodes, "
+          f"{time.time() - t0:.1f} s", flush=True)
+
+
 def capture_post(n_seg, with_field):
     Ez, R, H = -1e8, 1e-6, 10e-6
     prof = post_profile(R, H, n_seg)
@@ -270,6 +311,14 @@ def main(which):
         capture_post(400, False)
     if "well" in which:
         capture_well()
+    if "wellimage" in which:
+        capture_well_image()
+    if "wellimgrun" in which:
+        R, H = 50e-9, 30e-9
+        src = emission.well_bottom_source([3, 3], R, H, t_offset_max=5e-15, radius_fraction=0.5, seed=7)
+        geo = ref_bem_geometry.CylindricalWellBEMGeometry(-1e8, R, H, z0=3e-9, field_max_length=3e-9,
+                                                          image_max_length=4e-9, image_n_max=8)
+        traced_run("wellimg", src, 3, geo, [-25e-9, -15e-9], rtol=1e-8, atol=1e-12, t_max=8e-14)
     if "wellrun" in which:
         R, H = 50e-9, 30e-9
         src = emission.well_bottom_source([4, 4, 4], R, H, t_offset_max=10e-15, seed=6)
@@ -298,4 +347,5 @@ def main(which):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or ["sphere", "post80", "c1", "c1small", "tipbem", "post", "post400", "well", "wellrun"])
+    main(sys.argv[1:] or ["sphere", "post80", "c1", "c1small", "tipbem", "post", "post400", "well", "wellrun", "wellimage",
+                          "wellimgrun"])

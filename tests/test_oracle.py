@@ -198,6 +198,13 @@ def _geom_for(name):
         w = golden("bem_well.npz")
         return dict(kind="well_bem", E_gun=float(w["Ez"]), R=float(w["R"]), H=float(w["H"]),
                     profile=w["sharp_profile"], sigma=w["sharp_sigma"], z_handoff=float(w["sharp_z_handoff"]))
+    if name == "wellimg":
+        w = golden("bem_well_image.npz")
+        return dict(kind="well_bem", E_gun=float(w["Ez"]), R=float(w["R"]), H=float(w["H"]),
+                    profile=w["real_profile"], sigma=w["sigma"], z_handoff=float(w["z_handoff"]),
+                    bottom_fillet=float(w["bottom_fillet"]), rim_fillet=float(w["rim_fillet"]), z0=float(w["z0"]),
+                    image_profile=w["image_profile"], image_A_inv=np.linalg.inv(w["A"]), n_max=int(w["n_max"]),
+                    d_lo=float(w["d_lo"]), d_hi=float(w["d_hi"]))
     p = golden("bem_post80.npz")
     R, H = float(p["R"]), float(p["H"])
     return dict(kind="post_bem", E_gun=-1e8, R=R, H=H, profile=p["profile"], sigma=p["sigma"],
@@ -213,7 +220,7 @@ def run_oracle(name):
     return d, st, out
 
 
-@pytest.mark.parametrize("name", ["c1small", "tipbem", "post80", "well"])
+@pytest.mark.parametrize("name", ["c1small", "tipbem", "post80", "well", "wellimg"])
 def test_oracle_integrator_matches_reference_run(name):
     d, st, out = run_oracle(name)
     np.testing.assert_array_equal(out["attempts"], d["attempts"])

@@ -1,0 +1,55 @@
+"""CPU checks of the image-charge BEM oracle (reference
+bem/image_charge.py) and the well's image profile generator: against the
+reference's own golden values, and by the subtraction's mathematics (the
+converged force does not depend on the mirror-charge blend, up to the
+surface discretisation)."""
+import numpy as np
+
+from conftest import golden
+from oracle import image as oi
+from oracle import tracker as otr
+
+
+def test_image_and_real_profiles_match_reference():
+    from bem_amd.bem.mesh import cylindrical_well_image_profile, cylindrical_well_profile
+
+    d = golden("bem_well_image.npz")
+    R, H, z0 = float(d["R"]), float(d["H"]), float(d["z0"])
+    bf, rf = float(d["bottom_fillet"]), float(d["rim_fillet"])
+    p = cylindrical_well_image_profile(R, H, z0, float(d["image_plane_radius"]), float(d["image_max_length"]),
+                                       bottom_fillet_radius=bf, rim_fillet_radius=rf)
+    np.testing.assert_array_equal(p, d["image_profile"])
+    q = cylindrical_well_profile(R, H, float(d["z_handoff"]), 3e-9, bottom_fillet_radius=bf, rim_fillet_radius=rf)
+    np.testing.assert_array_equal(q, d["real_profile"])
+
+
+def test_operators_and_force_match_reference():
+    d = golden("bem_well_image.npz")
+    A = oi.assemble_mode_operators(d["image_profile"], int(d["n_max"]))
+    assert np.max(np.abs(A - d["A"]) / np.abs(d["A"]).max(axis=(1, 2), keepdims=True)) < 1e-13
+    Ai = np.linalg.inv(d["A"])
+    F = oi.image_force(d["image_profile"], Ai, int(d["n_max"]), d["pos"], d["charge"], d["active"],
+                       float(d["d_lo"]), float(d["d_hi"]))
+    assert np.abs(F - d["near"]).max() / np.abs(d["near"]).max() < 1e-13
+    geom = dict(kind="well_bem", z0=float(d["z0"]), image_profile=d["image_profile"], image_A_inv=Ai,
+                n_max=int(d["n_max"]), d_lo=float(d["d_lo"]), d_hi=float(d["d_hi"]), z_handoff=float(d["z_handoff"]))
+    Fb = otr.image_force(geom, d["pos"], d["charge"], d["active"], 1e-12)
+    assert np.abs(Fb - d["force"]).max() / np.abs(d["force"]).max() < 1e-13
+
+
+def test_converged_force_does_not_depend_on_the_mirror_blend():
+    d = golden("bem_well_image.npz")
+    prof = d["image_profile"]
+    R, H = float(d["R"]), float(d["H"])
+    Ai = np.linalg.inv(oi.assemble_mode_operators(prof, 24))
+    pos = np.array([[[0.3 * R, 0.1 * R, -H + 8e-9], [0.6 * R * np.cos(1.0), 0.6 * R * np.sin(1.0), -0.5 * H]]])
+    q = np.full((1, 2), -1.602176634e-19)
+    act = np.ones((1, 2), dtype=bool)
+    f16 = oi.image_force(prof, Ai, 16, pos, q, act, 0.1 * R, 0.5 * R)
+    f24 = oi.image_force(prof, Ai, 24, pos, q, act, 0.1 * R, 0.5 * R)
+    bare = oi.image_force(prof, Ai, 24, pos, q, act, 0.0, 0.0)
+    assert np.abs(f24 - f16).max() < 1e-6 * np.abs(f24).max()     # modes converged
+    assert np.abs(f24 - bare).max() < 1e-3 * np.abs(f24).max()    # blend-independent up to discretisation
+    # the force pulls the electron towards the nearest conductor (bottom for the first, wall for the second)
+    assert f24[0, 0, 2] < 0
+    assert f24[0, 1, 0] * pos[0, 1, 0] + f24[0, 1, 1] * pos[0, 1, 1] > 0
