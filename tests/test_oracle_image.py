@@ -42,7 +42,7 @@ def test_converged_force_does_not_depend_on_the_mirror_blend():
     prof = d["image_profile"]
     R, H = float(d["R"]), float(d["H"])
     Ai = np.linalg.inv(oi.assemble_mode_operators(prof, 24))
-    pos = np.array([[[0.3 * R, 0.1 * R, -H + 8e-9], [0.6 * R * np.cos(1.0), 0.6 * R * np.sin(1.0), -0.5 * H]]])
+    pos = np.array([[[0.3 * R, 0.1 * R, -H + 8e-9], [0.85 * R * np.cos(1.0), 0.85 * R * np.sin(1.0), -0.5 * H]]])
     q = np.full((1, 2), -1.602176634e-19)
     act = np.ones((1, 2), dtype=bool)
     f16 = oi.image_force(prof, Ai, 16, pos, q, act, 0.1 * R, 0.5 * R)
