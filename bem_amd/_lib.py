@@ -30,7 +30,7 @@ EXPORTS = (
     "spt_bem_upload", "spt_bem_assemble", "spt_bem_solve", "spt_bem_pert_field", "spt_geometry_field",
     "spt_pair_forces", "spt_kill_mask", "spt_trace", "spt_bem_potential", "spt_mlp_nparams", "spt_mlp_grad",
     "spt_mlp_train", "spt_mlp_infer", "spt_surrogate_set", "spt_toroidal_q", "spt_ring_modes",
-    "spt_image_upload", "spt_image_force",
+    "spt_image_upload", "spt_image_force", "spt_set_field_shape",
 )
 MLP_FOURIER, MLP_WIDTH, MLP_DEPTH = 32, 128, 4
 
@@ -135,6 +135,7 @@ def load():
             "spt_mlp_infer": (ctypes.c_int, [_p, P(MlpCfg_t), _p, _p, ctypes.c_int64, ctypes.c_int32, _p]),
             "spt_surrogate_set": (ctypes.c_int, [_p, P(MlpCfg_t), _p]),
             "spt_toroidal_q": (ctypes.c_int, [_p, _p, ctypes.c_int64, ctypes.c_int32, _p, _p]),
+            "spt_set_field_shape": (ctypes.c_int, [_p, ctypes.c_int32, ctypes.c_int32]),
             "spt_image_upload": (ctypes.c_int, [_p, _p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                 ctypes.c_double, ctypes.c_double, _p]),
             "spt_image_force": (ctypes.c_int, [_p, P(Geometry_t), _p, _p, _p, _p, ctypes.c_int32, ctypes.c_double,

@@ -1564,6 +1564,8 @@ __global__ void potential_kernel(const double* prof, const double* sigma, int n,
 }
 
 // ================================================================ host side
+constexpr int FIELD_NMAX_PUBLIC = 64;  // largest points-per-block of the field gather
+
 struct spt_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -1604,6 +1606,8 @@ struct spt_ctx {
   float* d_train = nullptr;
   size_t train_floats = 0;
   int train_batch = 0;
+  // field-gather launch shape chosen by a tuner (0 = automatic)
+  int tune_threads = 0, tune_pb = 0;
   // image-charge BEM table and per-call scratch
   int img_P = 0, img_modes = 0, img_quad = 0;
   double img_floor = 0.0, img_dlo = 0.0, img_dhi = 0.0;
@@ -1720,6 +1724,17 @@ int spt_sync(spt_ctx* c) {
   if (!c) return SPT_EINVAL;
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SPT_OK;
+}
+
+int spt_set_field_shape(spt_ctx* c, int32_t threads, int32_t points_per_block) {
+  if (!c) return SPT_EINVAL;
+  if (threads != 0 && threads != 192 && threads != 256) return fail(c, SPT_EINVAL, "threads must be 0, 192 or 256");
+  if (points_per_block < 0 || points_per_block > FIELD_NMAX_PUBLIC ||
+      (points_per_block & (points_per_block - 1)) != 0)
+    return fail(c, SPT_EINVAL, "points_per_block must be 0 or a power of two <= 64");
+  c->tune_threads = threads;
+  c->tune_pb = points_per_block;
   return SPT_OK;
 }
 
@@ -1847,7 +1862,8 @@ struct FieldShape {
 };
 static FieldShape field_shape(const spt_ctx* c, int64_t n, bool bem) {
   if (!bem) return {256, FIELD_NMAX};
-  const int PB = field_points_per_block(c, n);
+  const int PB = c->tune_pb ? c->tune_pb : field_points_per_block(c, n);
+  if (c->tune_threads) return {c->tune_threads, PB};
   const int64_t blocks = (n + PB - 1) / PB;
   double best = -1.0;
   FieldShape out{256, PB};

@@ -89,6 +89,7 @@ class SpecificParticleTracer:
             if surrogate.geometry is not geometry:
                 raise ValueError("the surrogate was trained for a different geometry object")
         self.surrogate = surrogate
+        self.field_shape = None  # (threads, points per block) of the field gather; None = automatic
         self.max_steps = int(max_steps)
         self._setup_state(group_sizes)
         self.t_max = t_max if t_max is not None else self._default_t_max()
@@ -229,10 +230,15 @@ class SpecificParticleTracer:
         a.event_count = _lib.ptr(ev_count)
 
         ctx.bind_stream()
+        shape = getattr(self, "field_shape", None) or (0, 0)
+        ctx.check(ctx.lib.spt_set_field_shape(ctx.h, int(shape[0]), int(shape[1])), "spt_set_field_shape")
         t0 = torch.cuda.Event(enable_timing=True)
         t1 = torch.cuda.Event(enable_timing=True)
         t0.record()
-        ctx.check(ctx.lib.spt_trace(ctx.h, a), "spt_trace")
+        try:
+            ctx.check(ctx.lib.spt_trace(ctx.h, a), "spt_trace")
+        finally:
+            ctx.lib.spt_set_field_shape(ctx.h, 0, 0)
         t1.record()
         ctx.sync()
         torch.cuda.synchronize(dev)

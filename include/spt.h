@@ -260,6 +260,12 @@ int spt_image_force(spt_ctx* ctx, const spt_geometry* geom, const double* pos, c
                     const uint8_t* active, const int64_t* group_offsets, int32_t n_groups, double plummer_radius,
                     double* F);
 
+/* Launch shape of the BEM field gather (no counterpart in the reference;
+ * set by the autotuner in bem_amd/autotune.py): threads per block 192 or
+ * 256 and points per block (a power of two <= 64); 0 keeps the automatic
+ * rule (about three resident blocks per compute unit, whole rounds). */
+int spt_set_field_shape(spt_ctx* ctx, int32_t threads, int32_t points_per_block);
+
 /* toroidal_Q (reference bem/toroidal.py:188-219): Q_{m-1/2}(chi) and
  * dQ/dchi for m = 0..n_max (n_max <= 64) at n device values chi >= 1 into
  * device q, dq, each (n_max + 1) x n, mode-major. */
