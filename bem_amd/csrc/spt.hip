@@ -1623,6 +1623,15 @@ struct spt_ctx {
   size_t img_part_cap = 0;
 };
 
+// Every entry point starts here: make the context's device current and
+// clear any error another library left behind on this thread, so the
+// launch checks below see only this call's errors.
+static hipError_t spt_begin(spt_ctx* c) {
+  const hipError_t e = hipSetDevice(c->device);
+  (void)hipGetLastError();
+  return e;
+}
+
 static int fail(spt_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   return code;
@@ -1722,7 +1731,7 @@ int spt_set_stream(spt_ctx* c, void* s) {
 
 int spt_sync(spt_ctx* c) {
   if (!c) return SPT_EINVAL;
-  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, spt_begin(c));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return SPT_OK;
 }
@@ -1748,7 +1757,7 @@ int spt_bem_upload(spt_ctx* c, const double* prof, int32_t n_nodes, const double
   if (!c) return SPT_EINVAL;
   if (!prof || !sigma || n_nodes < 2 || n_sub < 1 || n_sub > 64)
     return fail(c, SPT_EINVAL, "spt_bem_upload: need n_nodes >= 2 and 1 <= n_subdiv <= 64");
-  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, spt_begin(c));
   free_bem(c);
   const int n_seg = n_nodes - 1;
   const size_t nq = (size_t)n_seg * n_sub * 4;
@@ -1793,7 +1802,7 @@ static int assemble_device(spt_ctx* c, const double* prof, int n, double rr, int
 
 int spt_bem_assemble(spt_ctx* c, const double* prof, int32_t n, double rr, int32_t md, double* A_out) {
   if (!c || !prof || !A_out) return SPT_EINVAL;
-  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, spt_begin(c));
   double *A = nullptr, *b = nullptr;
   int rc = assemble_device(c, prof, n, rr, md, 0.0, &A, &b);
   if (rc) return rc;
@@ -1812,7 +1821,7 @@ int spt_bem_assemble(spt_ctx* c, const double* prof, int32_t n, double rr, int32
 
 int spt_bem_solve(spt_ctx* c, const double* prof, int32_t n, double E_gun, double rr, int32_t md, double* sigma_out) {
   if (!c || !prof || !sigma_out) return SPT_EINVAL;
-  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, spt_begin(c));
   double *A = nullptr, *b = nullptr;
   int rc = assemble_device(c, prof, n, rr, md, E_gun, &A, &b);
   if (rc) return rc;
@@ -1886,7 +1895,7 @@ int spt_bem_pert_field(spt_ctx* c, const double* pos, int64_t n, double* E, int3
   if (!c->bem_nodes) return fail(c, SPT_ENOBEM, "no BEM table uploaded");
   if (n < 0 || (n > 0 && (!pos || !E))) return fail(c, SPT_EINVAL, "spt_bem_pert_field: bad arguments");
   if (n == 0) return SPT_OK;
-  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, spt_begin(c));
   const FieldShape fs = field_shape(c, n, true);
   const unsigned nb = (unsigned)((n + fs.PB - 1) / fs.PB);
   if (fs.T == 192) {
@@ -1935,7 +1944,7 @@ int spt_geometry_field(spt_ctx* c, const spt_geometry* g, const double* pos, int
   if (rc) return rc;
   if (n < 0 || (n > 0 && (!pos || !E))) return fail(c, SPT_EINVAL, "spt_geometry_field: bad arguments");
   if (n == 0) return SPT_OK;
-  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, spt_begin(c));
   const int PB = (g->kind == SPT_GEOM_BEM_POST || g->kind == SPT_GEOM_BEM_WELL) ? field_points_per_block(c, n)
                                                                                 : FIELD_NMAX;
   const unsigned nb = (unsigned)((n + PB - 1) / PB);
@@ -1957,7 +1966,7 @@ int spt_pair_forces(spt_ctx* c, const spt_geometry* g, const double* pos, const 
   if (n_groups < 0 || (n_groups > 0 && (!pos || !charge || !active || !offsets)))
     return fail(c, SPT_EINVAL, "spt_pair_forces: bad arguments");
   if (n_groups == 0) return SPT_OK;
-  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, spt_begin(c));
   // group sizes are checked against the block size on the host side of
   // the caller's offsets: read them back (small) to validate.
   int64_t* h = new int64_t[n_groups + 1];
@@ -1988,7 +1997,7 @@ int spt_kill_mask(spt_ctx* c, const spt_geometry* g, const double* pos, int64_t 
   if (rc) return rc;
   if (n < 0 || (n > 0 && (!pos || !out))) return fail(c, SPT_EINVAL, "spt_kill_mask: bad arguments");
   if (n == 0) return SPT_OK;
-  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, spt_begin(c));
   kill_kernel<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(*g, pos, n, out);
   HIPCHK(c, hipGetLastError());
   return SPT_OK;
@@ -2179,7 +2188,7 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
       (a->event_capacity > 0 && !a->events))
     return fail(c, SPT_EINVAL, "spt_trace: null pointer");
   if (!(a->rtol >= 0.0) || !(a->atol >= 0.0) || !(a->max_steps > 0)) return fail(c, SPT_EINVAL, "bad tolerances");
-  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, spt_begin(c));
   // validate the CSR offsets and find the largest group
   int64_t* h = new int64_t[a->n_groups + 1];
   hipError_t e = hipMemcpy(h, a->group_offsets, sizeof(int64_t) * (a->n_groups + 1), hipMemcpyDeviceToHost);
@@ -2395,10 +2404,7 @@ static MlpTrainBufs mlp_train_bufs(float* base, int B) {
 static int ensure_mlp_train(spt_ctx* c, int B) {
   const size_t need = mlp_train_floats(B) + 64 * 32;
   if (need > c->train_floats) {
-    if (c->d_train) (void)hipFree(c->d_train);
-  for (double* p : {c->d_img_prof, c->d_img_segs, c->d_img_quadv, c->d_img_Ainv, c->d_img_rhs, c->d_img_sigma,
-                    c->d_img_mirror, c->d_img_F})
-    if (p) (void)hipFree(p);
+    if (c->d_train) HIPCHK(c, hipFree(c->d_train));
     c->d_train = nullptr;
     c->train_floats = 0;
     HIPCHK(c, hipMalloc(&c->d_train, sizeof(float) * need));
@@ -2463,7 +2469,7 @@ int spt_bem_potential(spt_ctx* c, const double* pos, int64_t n, double* phi) {
   if (!c->bem_nodes) return fail(c, SPT_ENOBEM, "no BEM table uploaded");
   if (n < 0 || (n > 0 && (!pos || !phi))) return fail(c, SPT_EINVAL, "spt_bem_potential: bad arguments");
   if (n == 0) return SPT_OK;
-  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, spt_begin(c));
   potential_kernel<<<(unsigned)((n + 127) / 128), 128, 0, c->stream>>>(c->d_prof, c->d_sigma, c->bem_nodes, 1.0, 20,
                                                                        pos, n, phi);
   HIPCHK(c, hipGetLastError());
@@ -2483,7 +2489,7 @@ int spt_mlp_grad(spt_ctx* c, const spt_mlp_cfg* cfg, const float* params, const 
   if (rc) return rc;
   if (!params || !x || !y || !loss_weights || !grad || !loss) return fail(c, SPT_EINVAL, "spt_mlp_grad: null pointer");
   if (batch < 32 || batch % 32) return fail(c, SPT_EINVAL, "batch must be a positive multiple of 32");
-  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, spt_begin(c));
   rc = ensure_mlp_train(c, batch);
   if (rc) return rc;
   const MlpTrainBufs T = mlp_train_bufs(c->d_train, batch);
@@ -2511,13 +2517,14 @@ int spt_mlp_train(spt_ctx* c, const spt_mlp_cfg* cfg, float* params, float* adam
   if (batch < 32 || batch % 32) return fail(c, SPT_EINVAL, "batch must be a positive multiple of 32");
   if (n_samples < 1 || steps < 0 || adam_t0 < 0) return fail(c, SPT_EINVAL, "spt_mlp_train: bad sizes");
   if (steps == 0) return SPT_OK;
-  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, spt_begin(c));
   rc = ensure_mlp_train(c, batch);
   if (rc) return rc;
   const MlpTrainBufs T = mlp_train_bufs(c->d_train, batch);
   hipStream_t s = c->stream;
   float* d_loss = nullptr;
   HIPCHK(c, hipMalloc(&d_loss, sizeof(float) * steps));
+  HIPCHK(c, hipGetLastError());
   const double b1 = 0.9, b2 = 0.999;
   for (int i = 0; i < steps; ++i) {
     const int64_t t = adam_t0 + i + 1;
@@ -2529,8 +2536,19 @@ int spt_mlp_train(spt_ctx* c, const spt_mlp_cfg* cfg, float* params, float* adam
     const float c1 = (float)(1.0 - pow(b1, (double)t)), c2 = (float)(1.0 - pow(b2, (double)t));
     mlp_adam_kernel<<<(unsigned)((MLP_NPARAM + 255) / 256), 256, 0, s>>>(params, T.grad, adam_m, adam_v, MLP_NPARAM,
                                                                          lr, (float)b1, (float)b2, 1e-8f, c1, c2);
+    if (i == 0) {
+      const hipError_t e0 = hipGetLastError();
+      if (e0 != hipSuccess) {
+        (void)hipFree(d_loss);
+        return fail(c, SPT_EHIP, std::string("spt_mlp_train: first step launch: ") + hipGetErrorString(e0));
+      }
+    }
   }
   hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    (void)hipFree(d_loss);
+    return fail(c, SPT_EHIP, std::string("spt_mlp_train: step launches: ") + hipGetErrorString(e));
+  }
   if (e == hipSuccess) e = hipMemcpyAsync(loss_out, d_loss, sizeof(float) * steps, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   (void)hipFree(d_loss);
@@ -2546,7 +2564,7 @@ int spt_mlp_infer(spt_ctx* c, const spt_mlp_cfg* cfg, const float* params, const
   if (!params || n < 0 || (n > 0 && (!pos || !out))) return fail(c, SPT_EINVAL, "spt_mlp_infer: bad arguments");
   if (precision != SPT_PREC_FP64 && precision != SPT_PREC_MIXED) return fail(c, SPT_EINVAL, "unknown precision");
   if (n == 0) return SPT_OK;
-  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, spt_begin(c));
   hipStream_t s = c->stream;
   if (precision == SPT_PREC_FP64) {
     if (!c->d_mlp_frag) HIPCHK(c, hipMalloc(&c->d_mlp_frag, sizeof(double) * MLP64_FRAG_N));
@@ -2578,7 +2596,7 @@ int spt_surrogate_set(spt_ctx* c, const spt_mlp_cfg* cfg, const float* params) {
   int rc = check_mlp_cfg(c, cfg);
   if (rc) return rc;
   if (!(cfg->band >= 0.0)) return fail(c, SPT_EINVAL, "surrogate band must be >= 0");
-  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, spt_begin(c));
   if (!c->d_sur_p) HIPCHK(c, hipMalloc(&c->d_sur_p, sizeof(float) * MLP_NPARAM));
   if (!c->d_sur_split) HIPCHK(c, hipMalloc(&c->d_sur_split, sizeof(__bf16) * 3 * MLP_NPARAM));
   if (!c->d_sur_frag) HIPCHK(c, hipMalloc(&c->d_sur_frag, sizeof(double) * MLP64_FRAG_N));
@@ -2602,7 +2620,7 @@ int spt_toroidal_q(spt_ctx* c, const double* chi, int64_t n, int32_t n_max, doub
   if (n < 0 || n_max < 0 || n_max > SPT_TOROIDAL_MAX_M || (n > 0 && (!chi || !q || !dq)))
     return fail(c, SPT_EINVAL, "spt_toroidal_q: need 0 <= n_max <= 64 and device arrays");
   if (n == 0) return SPT_OK;
-  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, spt_begin(c));
   toroidal_q_kernel<<<(unsigned)((n + 127) / 128), 128, 0, c->stream>>>(chi, n, n_max, q, dq);
   HIPCHK(c, hipGetLastError());
   return SPT_OK;
@@ -2614,7 +2632,7 @@ int spt_ring_modes(spt_ctx* c, const double* rho, const double* z, const double*
   if (n < 0 || n_max < 0 || n_max > SPT_TOROIDAL_MAX_M || (n > 0 && (!rho || !z || !a || !za)))
     return fail(c, SPT_EINVAL, "spt_ring_modes: need 0 <= n_max <= 64 and device arrays");
   if (n == 0) return SPT_OK;
-  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, spt_begin(c));
   ring_modes_kernel<<<(unsigned)((n + 127) / 128), 128, 0, c->stream>>>(rho, z, a, za, n, n_max, phi, dphi_drho,
                                                                        dphi_dz, g);
   HIPCHK(c, hipGetLastError());
@@ -2630,7 +2648,7 @@ int spt_image_upload(spt_ctx* c, const double* prof, int32_t n, int32_t n_max, i
   if (!c) return SPT_EINVAL;
   if (!prof || n < 3 || n_max < 0 || n_max >= IMG_MAX_MODES || n_sub < 1 || n_sub > 64)
     return fail(c, SPT_EINVAL, "spt_image_upload: need n_nodes >= 3, 0 <= n_max <= 32, 1 <= n_subdiv <= 64");
-  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, spt_begin(c));
   for (double** p : {&c->d_img_prof, &c->d_img_segs, &c->d_img_quadv, &c->d_img_Ainv, &c->d_img_rhs,
                      &c->d_img_sigma}) {
     if (*p) (void)hipFree(*p);
@@ -2707,7 +2725,7 @@ int spt_image_force(spt_ctx* c, const spt_geometry* g, const double* pos, const 
   if (n_groups < 0 || (n_groups > 0 && (!pos || !charge || !active || !offsets || !F)))
     return fail(c, SPT_EINVAL, "spt_image_force: bad arguments");
   if (n_groups == 0) return SPT_OK;
-  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, spt_begin(c));
   int64_t last = 0;
   std::vector<int64_t> h(n_groups + 1);
   HIPCHK(c, hipMemcpy(h.data(), offsets, sizeof(int64_t) * (n_groups + 1), hipMemcpyDeviceToHost));

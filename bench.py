@@ -86,7 +86,7 @@ def screen_time_diff(a, b):
     return dict(max=float(dt.max()), median=float(np.median(dt)))
 
 
-def field_gather_timing(wl, geo, n_pts=262144, reps=10):
+def field_gather_timing(wl, geo, n_pts=262144, reps=10, shape=None):
     """Average launch duration of the BEM field gather over n_pts points
     displaced 10 nm .. 10 um off the cap, timed with HIP events on the
     stream the kernel is launched on (the library is bound to torch's
@@ -107,6 +107,8 @@ def field_gather_timing(wl, geo, n_pts=262144, reps=10):
     pt = torch.as_tensor(pts, device=ctx.torch_device).contiguous()
     out = torch.empty_like(pt)
     ctx.bind_stream()
+    shape = shape or (0, 0)
+    ctx.check(ctx.lib.spt_set_field_shape(ctx.h, int(shape[0]), int(shape[1])), "spt_set_field_shape")
     for _ in range(2):
         ctx.check(ctx.lib.spt_bem_pert_field(ctx.h, _lib.ptr(pt), n_pts, _lib.ptr(out), _lib.PREC_FP64), "field")
     e0 = torch.cuda.Event(enable_timing=True)
@@ -116,6 +118,7 @@ def field_gather_timing(wl, geo, n_pts=262144, reps=10):
         ctx.check(ctx.lib.spt_bem_pert_field(ctx.h, _lib.ptr(pt), n_pts, _lib.ptr(out), _lib.PREC_FP64), "field")
     e1.record()
     torch.cuda.synchronize()
+    ctx.lib.spt_set_field_shape(ctx.h, 0, 0)
     return e0.elapsed_time(e1) / reps, n_pts
 
 
@@ -218,12 +221,18 @@ def main():
     sur = surrogate.train(geo, n_samples=1 << 20, steps=args.train_steps, batch=8192, tol=args.tol, chunk=10000)
     rep = sur.report
     say(f"bench: trained in {rep['t_train_s']:.1f} s; band at tol {args.tol:g}: {rep['band']:.3e} m")
+    from bem_amd import autotune
+
+    plain_tr = make_tracer(wl, geo)
+    tuning = autotune.tune(plain_tr, attempts=4)
+    say(f"bench: autotuned field shape {tuning['best']}")
     tr = make_tracer(wl, geo, surrogate=sur)
+    tr.field_shape = plain_tr.field_shape
     outs, wall = timed_traces(tr, args.steps, args.warmup)
     say(f"bench: {args.steps} surrogate traces in {wall:.2f} s")
     stages = sum(o["particle_stages"] for o in outs)
     act = sum(int(o["active_stages"].sum()) for o in outs)
-    plain_outs, plain_wall = timed_traces(make_tracer(wl, geo), 1, 0)
+    plain_outs, plain_wall = timed_traces(plain_tr, 1, 0)
     plain_value = plain_outs[0]["particle_stages"] / plain_wall
     say(f"bench: plain trace {plain_wall:.2f} s")
     sweep = []
@@ -233,12 +242,14 @@ def main():
             if not np.isfinite(s.cfg["band"]):
                 sweep.append(dict(tol=tol, band_m=None))
                 continue
-            o, w = timed_traces(make_tracer(wl, geo, surrogate=s), 1, 0)
+            t2 = make_tracer(wl, geo, surrogate=s)
+            t2.field_shape = plain_tr.field_shape
+            o, w = timed_traces(t2, 1, 0)
             sweep.append(dict(tol=tol, band_m=s.cfg["band"], value=o[0]["particle_stages"] / w,
                               wall_s=w, gain_wall=plain_wall / w, attempts_mean=float(o[0]["attempts"].mean()),
                               screen_t_rel_diff_vs_plain=screen_time_diff(plain_outs[0], o[0])))
             say(f"bench: tol {tol:g} trace {w:.2f} s")
-    field_ms, n_pts = field_gather_timing(wl, geo)
+    field_ms, n_pts = field_gather_timing(wl, geo, shape=plain_tr.field_shape)
     n_quad = wl["n_segments"] * 8 * 4
     achieved = n_pts * n_quad * C_RING / (field_ms / 1e3) / 1e12
     value = stages / wall
@@ -259,7 +270,7 @@ def main():
                                f"{wl['n_el'] // wl['group']} groups of 16, rtol 1e-8, screen 90 um, anode kill 100 um",
                    "form": f"surrogate + batched lockstep, band at field err <= {args.tol:g}; FP64 field and step "
                            "control, FP32 + Kahan pair forces",
-                   "n_quad": n_quad},
+                   "n_quad": n_quad, "field_shape": list(plain_tr.field_shape or (0, 0))},
         "plain_value": plain_value,
         "gain_vs_plain": value / plain_value,
         "surrogate": {"tol": args.tol, "band_m": rep["band"], "band_panels": rep["band_panels"],
@@ -269,6 +280,7 @@ def main():
                       "batch": rep["batch"], "n_samples": rep["n_samples"],
                       "screen_t_rel_diff_vs_plain": screen_time_diff(plain_outs[0], outs[-1])},
         "tolerance_sweep": sweep,
+        "autotune": {"best": tuning["best"], "table": tuning["table"], "sample_attempts": tuning["attempts"]},
         "attempts_per_group_mean": float(np.mean(outs[-1]["attempts"])),
         "plain_attempts_per_group_mean": float(np.mean(plain_outs[0]["attempts"])),
         "active_particle_stages_per_s": act / wall,
