@@ -91,6 +91,8 @@ class TraceArgs_t(ctypes.Structure):
         ("screens", ctypes.c_double * SPT_MAX_SCREENS),
         ("attempts", _p), ("accepted", _p), ("active_stages", _p), ("margin", _p), ("status", _p),
         ("events", _p), ("event_capacity", ctypes.c_int64), ("event_count", _p),
+        ("t_out", _p), ("n_out", ctypes.c_int32), ("pad1", ctypes.c_int32),
+        ("snapshots", _p), ("snapshot_capacity", ctypes.c_int64), ("snapshot_count", _p),
     ]
 
 

@@ -691,7 +691,50 @@ struct TraceParams {
   int64_t ev_cap;
   unsigned long long* ev_count;
   int* queue;
+  const double* t_out;
+  int n_out;
+  spt_event* snaps;
+  int64_t snap_cap;
+  unsigned long long* snap_count;
 };
+
+// Output times at or before t (the ascending list's count below t's
+// position), and one trajectory record (reference batched_rk45.py:237-242,
+// trajectories.py:44-74).
+__device__ __forceinline__ int outputs_reached(const TraceParams& P, double t) {
+  int k = 0;
+  while (k < P.n_out && P.t_out[k] <= t) ++k;
+  return k;
+}
+
+__device__ __forceinline__ void record_snapshot(const TraceParams& P, int k, int g, int p, int64_t batch, double t,
+                                                const double* x, const double* v) {
+  const unsigned long long i = atomicAdd(P.snap_count, 1ull);
+  if (i >= (unsigned long long)P.snap_cap) return;
+  spt_event& e = P.snaps[i];
+  e.group = g;
+  e.particle = p;
+  e.screen = k;
+  e.interval = 0;
+  e.batch = batch;
+  e.t = t;
+  for (int c = 0; c < 3; ++c) {
+    e.pos[c] = x[c];
+    e.vel[c] = v[c];
+  }
+}
+
+// A group's start time coinciding exactly with an output time records its
+// particles born by then (reference batched_rk45.py:189-209).
+__device__ __forceinline__ void snapshot_at_start(const TraceParams& P, int g, int p, int64_t gi, double t, double tb) {
+  if (P.n_out == 0) return;
+  const int last = outputs_reached(P, t) - 1;
+  if (last >= 0 && P.t_out[last] == t && tb <= t) {
+    const double x[3] = {P.pos[3 * gi], P.pos[3 * gi + 1], P.pos[3 * gi + 2]};
+    const double v[3] = {P.vel[3 * gi], P.vel[3 * gi + 1], P.vel[3 * gi + 2]};
+    record_snapshot(P, last, g, p, -1, t, x, v);
+  }
+}
 
 template <int T, int NMAX>
 struct TraceShared {
@@ -771,6 +814,7 @@ __device__ void run_group(const TraceParams& P, int g, TraceShared<T, NMAX>& sh)
   }
   double t = block_min<T>(mine ? tb : INFINITY, sh.red);
   double h = (P.t_max - t) * P.init_frac;
+  if (mine) snapshot_at_start(P, g, tid, gi, t, tb);
   bool running = true;
   int64_t attempts = 0, nacc = 0, act_st = 0;
   double margin = INFINITY;
@@ -787,7 +831,10 @@ __device__ void run_group(const TraceParams& P, int g, TraceShared<T, NMAX>& sh)
     const double nxt_cp = block_min<T>((mine && tb > t) ? tb : INFINITY, sh.red);
     const bool exhausted = n_reached >= (double)n;
     const double next = exhausted ? P.t_max : nxt_cp;
-    const double target = fmin(next, P.t_max);
+    const int optr = outputs_reached(P, t);
+    const bool oexh = optr >= P.n_out;
+    const double nout = oexh ? P.t_max : P.t_out[optr];
+    const double target = fmin(fmin(next, P.t_max), nout);
     const double hr = fmin(h, target - t);
     const bool active = mine && (t >= tb) && (t < td);
     const double n_act = block_sum<T>(active ? 1.0 : 0.0, sh.red);
@@ -890,6 +937,7 @@ __device__ void run_group(const TraceParams& P, int g, TraceShared<T, NMAX>& sh)
         }
         if (active && kill_point(P.G, x1[0], x1[1], x1[2])) td = t;
         screens_particle(P.SP, P.events, P.ev_cap, P.ev_count, g, tid, batch, t_old, t, x0, v0, x1, v1);
+        if (!oexh && t >= nout - tol && tb <= t) record_snapshot(P, optr, g, tid, batch, t, x1, v1);
       }
       running = t < P.t_max * (1 - 1e-14);
     }
@@ -1069,7 +1117,9 @@ __device__ void ls_begin(const LockParams& L, int g, int n, double t, double h, 
   const double nxt_cp = block_min<TG>((mine && tb > t) ? tb : INFINITY, red);
   const bool exhausted = n_reached >= (double)n;
   const double next = exhausted ? L.T.t_max : nxt_cp;
-  const double target = fmin(next, L.T.t_max);
+  const int optr = outputs_reached(L.T, t);
+  const double nout = optr >= L.T.n_out ? L.T.t_max : L.T.t_out[optr];
+  const double target = fmin(fmin(next, L.T.t_max), nout);
   const double hr = fmin(h, target - t);
   const bool active = mine && (t >= tb) && (t < L.T.t_death[gi]);
   const double n_act = block_sum<TG>(active ? 1.0 : 0.0, red);
@@ -1101,6 +1151,7 @@ __global__ __launch_bounds__(TG) void ls_init_kernel(LockParams L) {
   if (mine) L.T.t_death[gi] = INFINITY;
   const double t = block_min<TG>(mine ? L.T.t_birth[gi] : INFINITY, red);
   const double h = (L.T.t_max - t) * L.T.init_frac;
+  if (mine) snapshot_at_start(L.T, g, tid, gi, t, L.T.t_birth[gi]);
   if (tid == 0) {
     L.g_t[g] = t;
     L.g_h[g] = h;
@@ -1354,6 +1405,9 @@ __global__ __launch_bounds__(TG) void ls_end_kernel(LockParams L) {
   double t = L.g_t[g];
   double h = hr * fac;
   bool running = true;
+  const int optr = outputs_reached(L.T, t);
+  const bool oexh = optr >= L.T.n_out;
+  const double nout = oexh ? L.T.t_max : L.T.t_out[optr];
   if (acc) {
     const double t_old = t;
     t = t + hr;
@@ -1371,6 +1425,7 @@ __global__ __launch_bounds__(TG) void ls_end_kernel(LockParams L) {
       }
       if (L.active[gi] && kill_point(L.T.G, x1[0], x1[1], x1[2])) L.T.t_death[gi] = t;
       screens_particle(L.T.SP, L.T.events, L.T.ev_cap, L.T.ev_count, g, tid, batch, t_old, t, x0, v0, x1, v1);
+      if (!oexh && t >= nout - tol && L.T.t_birth[gi] <= t) record_snapshot(L.T, optr, g, tid, batch, t, x1, v1);
     }
     running = t < L.T.t_max * (1 - 1e-14);
   }
@@ -2093,6 +2148,7 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
   ls_pgroup_kernel<<<(n_groups + 255) / 256, 256, 0, s>>>(L.T.offsets, n_groups, (int32_t*)L.pgroup);
   reset_u32_kernel<<<1, 1, 0, s>>>(L.done);
   reset_u64_kernel<<<1, 1, 0, s>>>(L.T.ev_count);
+  if (L.T.n_out > 0) reset_u64_kernel<<<1, 1, 0, s>>>(L.T.snap_count);
   ls_init_kernel<TG><<<n_groups, TG, 0, s>>>(L);
   HIPCHK(c, hipGetLastError());
   hipGraph_t graph = nullptr;
@@ -2188,6 +2244,8 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
       (a->event_capacity > 0 && !a->events))
     return fail(c, SPT_EINVAL, "spt_trace: null pointer");
   if (!(a->rtol >= 0.0) || !(a->atol >= 0.0) || !(a->max_steps > 0)) return fail(c, SPT_EINVAL, "bad tolerances");
+  if (a->n_out < 0 || (a->n_out > 0 && (!a->t_out || !a->snapshot_count || (a->snapshot_capacity > 0 && !a->snapshots))))
+    return fail(c, SPT_EINVAL, "spt_trace: output times need t_out, snapshots and snapshot_count");
   HIPCHK(c, spt_begin(c));
   // validate the CSR offsets and find the largest group
   int64_t* h = new int64_t[a->n_groups + 1];
@@ -2270,6 +2328,11 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
   P.ev_cap = a->event_capacity;
   P.ev_count = (unsigned long long*)a->event_count;
   P.queue = c->d_queue;
+  P.t_out = a->t_out;
+  P.n_out = a->n_out;
+  P.snaps = a->snapshots;
+  P.snap_cap = a->snapshot_capacity;
+  P.snap_count = (unsigned long long*)a->snapshot_count;
   if (a->mode == SPT_MODE_LOCKSTEP) {
     int rc2 = ensure_lockstep_scratch(c, a->n_particles, a->n_groups);
     if (rc2) return rc2;
@@ -2324,6 +2387,7 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
   }
   reset_int_kernel<<<1, 1, 0, c->stream>>>(c->d_queue, 0);
   reset_u64_kernel<<<1, 1, 0, c->stream>>>(P.ev_count);
+  if (P.n_out > 0) reset_u64_kernel<<<1, 1, 0, c->stream>>>(P.snap_count);
   switch (T) {
     case 64: return launch_trace<64, 64, SPT_PREC_FP64>(c, P, a->n_groups);
     case 128: return launch_trace<128, 128, SPT_PREC_FP64>(c, P, a->n_groups);

@@ -49,10 +49,9 @@ class SpecificParticleTracer:
         if n_workers != 1:
             raise ValueError(f"n_workers is only supported for backend='cpu' (got backend={backend!r}, "
                              f"n_workers={n_workers!r})")
-        if t_out is not None:
-            raise NotImplementedError("t_out trajectory snapshots are not implemented on the mi355x backend yet")
         self.initial_particles = initial_particles
         self.n_emit = n_emit
+        self.output_times_host = None if t_out is None else np.sort(np.asarray(t_out, dtype=float))
         self.screen_positions = list(screens)
         if len(self.screen_positions) > _lib.SPT_MAX_SCREENS:
             raise ValueError(f"at most {_lib.SPT_MAX_SCREENS} screens are supported")
@@ -145,7 +144,10 @@ class SpecificParticleTracer:
         zs = np.array(self.screen_positions, dtype=float)
         zp = zs[zs > 0]
         t_transit = np.sqrt(2 * zp.max() / a0) if len(zp) > 0 and a0 > 0 else 1e-12
-        return float(self.t_birth.max()) + 3.0 * t_transit
+        t_max = float(self.t_birth.max()) + 3.0 * t_transit
+        if self.output_times_host is not None and len(self.output_times_host):
+            t_max = max(t_max, float(self.output_times_host.max()))
+        return t_max
 
     def _queue_order(self):
         """Largest groups first (longest-processing-time order); ties keep
@@ -198,6 +200,12 @@ class SpecificParticleTracer:
         cap = event_capacity if event_capacity is not None else max(1024, 4 * n * max(ns, 1))
         events = torch.empty((cap, _lib.EVENT_DTYPE.itemsize), dtype=torch.uint8, device=dev)
         ev_count = torch.zeros(1, dtype=torch.int64, device=dev)
+        outs = self.output_times_host
+        n_out = 0 if outs is None else len(outs)
+        t_out = up(outs) if n_out else None
+        snap_cap = n * n_out  # every particle at most once per output time
+        snaps = torch.empty((max(snap_cap, 1), _lib.EVENT_DTYPE.itemsize), dtype=torch.uint8, device=dev)
+        snap_count = torch.zeros(1, dtype=torch.int64, device=dev)
 
         a = _lib.TraceArgs_t()
         a.geom = self.geometry.spt_geometry(self.z_max)
@@ -228,6 +236,11 @@ class SpecificParticleTracer:
         a.events = _lib.ptr(events)
         a.event_capacity = cap
         a.event_count = _lib.ptr(ev_count)
+        a.t_out = _lib.ptr(t_out)
+        a.n_out = n_out
+        a.snapshots = _lib.ptr(snaps)
+        a.snapshot_capacity = snap_cap
+        a.snapshot_count = _lib.ptr(snap_count)
 
         ctx.bind_stream()
         shape = getattr(self, "field_shape", None) or (0, 0)
@@ -246,10 +259,12 @@ class SpecificParticleTracer:
         if n_ev > cap:
             return self.trace(event_capacity=n_ev)
         ev = np.frombuffer(events[:n_ev].cpu().numpy().tobytes(), dtype=_lib.EVENT_DTYPE)
+        n_sn = min(int(snap_count.item()), snap_cap)
+        snapshots = np.frombuffer(snaps[:n_sn].cpu().numpy().tobytes(), dtype=_lib.EVENT_DTYPE)
         out = dict(pos=pos.cpu().numpy(), vel=vel.cpu().numpy(), t_death=td.cpu().numpy(),
                    attempts=attempts.cpu().numpy(), accepted=accepted.cpu().numpy(),
                    active_stages=act.cpu().numpy(), margin=margin.cpu().numpy(), status=status.cpu().numpy(),
-                   events=ev, device_ms=t0.elapsed_time(t1))
+                   events=ev, snapshots=snapshots, device_ms=t0.elapsed_time(t1))
         sizes = np.diff(self.offsets)
         out["particle_stages"] = int(N_STAGES * np.sum(sizes * out["attempts"]))
         return out
@@ -271,7 +286,14 @@ class SpecificParticleTracer:
             e = ev[ev["screen"] == si]
             e = e[np.lexsort((e["particle"], e["group"], e["interval"], e["batch"]))]
             screens.append(self._build_particle_group(e))
-        return screens, []
+        trajectories = []
+        if self.output_times_host is not None:
+            sn = out["snapshots"]
+            for k in range(len(self.output_times_host)):
+                e = sn[sn["screen"] == k]
+                e = e[np.lexsort((e["particle"], e["group"], e["batch"]))]
+                trajectories.append(self._build_particle_group(e))
+        return screens, trajectories
 
     def _build_particle_group(self, e):
         """Reference tracker.py:388-417."""

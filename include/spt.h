@@ -184,6 +184,16 @@ typedef struct spt_trace_args {
   spt_event* events;            /* capacity entries                         */
   int64_t event_capacity;
   uint64_t* event_count;        /* total found (may exceed capacity)        */
+  /* trajectory snapshots at output times (reference tracker.py t_out,
+   * batched_rk45.py:181-209,295-312): steps land on each time; every
+   * particle born by then is recorded (screen field = output index,
+   * batch = attempt index, -1 at a group's start) */
+  const double* t_out;          /* ascending, device; NULL when n_out = 0   */
+  int32_t n_out;
+  int32_t pad1;
+  spt_event* snapshots;         /* device, capacity records                 */
+  int64_t snapshot_capacity;
+  uint64_t* snapshot_count;     /* device, total recorded                   */
 } spt_trace_args;
 
 /* Context. */

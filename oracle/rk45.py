@@ -1,7 +1,7 @@
 """Oracle per-group adaptive Dormand-Prince 5(4) integrator (reference
 ``batched_rk45.py:30-401``), with per-group attempt/accept counters and the
-error-norm margin added for parity tests. ``t_out`` output times are not
-part of this oracle (no parity case uses them)."""
+error-norm margin added for parity tests; output times (``t_out``) are
+restated too (reference ``batched_rk45.py:181-209,243-249,295-312``)."""
 import numpy as np
 
 from .constants import SPEED_OF_LIGHT
@@ -50,8 +50,12 @@ def attempt(accel, pos, vel, t, h, t_birth, t_death, gidx, rtol, atol, safety, f
 
 
 def integrate(accel, pos0, vel0, t_birth, t_max, rtol, atol, on_step=None, kill=None,
-              safety=0.9, fmin=0.2, fmax=10.0, init_frac=1e-6, max_steps=10_000_000):
-    """Reference ``batched_rk45.py:55-323`` (without output times).
+              safety=0.9, fmin=0.2, fmax=10.0, init_frac=1e-6, max_steps=10_000_000, output_times=None,
+              on_output=None):
+    """Reference ``batched_rk45.py:55-323``; output times (shared by all
+    groups) clip each step's target like birth checkpoints and fire
+    ``on_output(k, groups, t, pos, vel)`` when a group lands on the k-th,
+    also at a group's start time (reference 181-209, 243-249, 295-312).
 
     Returns dict(pos, vel, t_death, attempts, accepted, margin) where
     attempts/accepted are per group and margin is the smallest |err - 1|
@@ -64,6 +68,18 @@ def integrate(accel, pos0, vel0, t_birth, t_max, rtol, atol, on_step=None, kill=
     cps = np.sort(t_birth, axis=1)
     t = cps[:, 0].copy()
     h = (t_max - t) * init_frac
+    have_out = output_times is not None
+    if have_out:
+        outs = np.sort(np.asarray(output_times, dtype=float))
+        n_out = outs.shape[0]
+        if on_output is not None:
+            last = np.sum(outs[None, :] <= t[:, None], axis=1) - 1
+            at_start = (last >= 0) & (outs[np.maximum(last, 0)] == t)
+            if np.any(at_start):
+                ai0 = np.nonzero(at_start)[0]
+                for p in np.unique(last[ai0]):
+                    sel = ai0[last[ai0] == p]
+                    on_output(int(p), sel, t[sel], pos[sel], vel[sel])
     running = np.ones(G, dtype=bool)
     attempts = np.zeros(G, dtype=np.int64)
     accepted_n = np.zeros(G, dtype=np.int64)
@@ -77,6 +93,11 @@ def integrate(accel, pos0, vel0, t_birth, t_max, rtol, atol, on_step=None, kill=
         exhausted = n_reached >= n
         nxt = np.where(exhausted, t_max, cps[idx, cp_idx])
         target = np.minimum(nxt, t_max)
+        if have_out:
+            optr = np.sum(outs[None, :] <= t[idx][:, None], axis=1)
+            oexh = optr >= n_out
+            nout = np.where(oexh, t_max, outs[np.minimum(optr, n_out - 1)])
+            target = np.minimum(target, nout)
         h_r = np.minimum(h[idx], target - t[idx])
         pn, vn, tn, acc, hn, err = attempt(accel, pos[idx], vel[idx], t[idx], h_r, t_birth[idx],
                                           t_death[idx], idx, rtol, atol, safety, fmin, fmax)
@@ -99,6 +120,14 @@ def integrate(accel, pos0, vel0, t_birth, t_max, rtol, atol, on_step=None, kill=
                 dead = was_active & kill(pos[ai])
                 if np.any(dead):
                     t_death[ai] = np.where(dead, t[ai][:, None], t_death[ai])
+            if have_out and on_output is not None:
+                hit = (~oexh[acc]) & (t[ai] >= nout[acc] - tol)
+                if np.any(hit):
+                    hl = np.nonzero(hit)[0]
+                    pk = optr[acc][hl]
+                    for p in np.unique(pk):
+                        sel = ai[hl[pk == p]]
+                        on_output(int(p), sel, t[sel], pos[sel], vel[sel])
             if on_step is not None:
                 on_step(it, ai, t_old, p_old, v_old, t[ai], pos[ai], vel[ai])
             running[ai] = t[ai] < t_max * (1 - 1e-14)

@@ -123,17 +123,20 @@ def setup(x, y, z, px, py, pz, t, ids, n_emit, weight=None):
     return st
 
 
-def default_t_max(st, E_gun, screen_z):
-    """Reference ``tracker.py:290-307`` (no output times)."""
+def default_t_max(st, E_gun, screen_z, t_out=None):
+    """Reference ``tracker.py:290-307``: covers the output times too."""
     a0 = abs(E_gun) * abs(st["q_over_m"])
     zs = np.array(screen_z, dtype=float)
     zp = zs[zs > 0]
     tt = np.sqrt(2 * zp.max() / a0) if len(zp) and a0 > 0 else 1e-12
-    return float(st["t_birth"].max()) + 3.0 * tt
+    t_max = float(st["t_birth"].max()) + 3.0 * tt
+    if t_out is not None and len(t_out):
+        t_max = max(t_max, float(np.max(t_out)))
+    return t_max
 
 
 def run(st, geom, screen_z, t_max, rtol=1e-7, atol=1e-12, plummer_radius=1e-12, z_max=None,
-        max_steps=10_000_000, count_stages=True):
+        max_steps=10_000_000, count_stages=True, t_out=None):
     """In-process tracker run (reference ``tracker.py:340-386``).
 
     Returns dict with final pos/vel/t_death, per-group attempts/accepted,
@@ -158,8 +161,21 @@ def run(st, geom, screen_z, t_max, rtol=1e-7, atol=1e-12, plummer_radius=1e-12, 
             e["group"] = int(gi[e.pop("row")])
             events.append(e)
 
+    outs = None if t_out is None else np.sort(np.asarray(t_out, dtype=float))
+    traj = None if outs is None else [[] for _ in outs]
+
+    def on_output(k, gi, t, pos, vel):
+        # reference trajectories.py:44-74: the group's particles born by then
+        for r, g_ in enumerate(gi):
+            for p in range(pos.shape[1]):
+                if st["t_birth"][g_, p] <= t[r]:
+                    traj[k].append(dict(group=int(g_), particle=p, t=float(t[r]), pos=pos[r, p].copy(),
+                                        vel=vel[r, p].copy()))
+
     out = rk45.integrate(counted if count_stages else accel, st["pos"], st["vel"], st["t_birth"], t_max,
-                         rtol, atol, on_step=on_step, kill=lambda p: kill_mask(g, p), max_steps=max_steps)
+                         rtol, atol, on_step=on_step, kill=lambda p: kill_mask(g, p), max_steps=max_steps,
+                         output_times=outs, on_output=on_output if outs is not None else None)
+    out["trajectories"] = traj
     # screens are recorded per screen; within one screen the reference's
     # order is (batch, interval, group, particle)
     out["events"] = events

@@ -282,7 +282,7 @@ def traced_run(name, src, n_emit, geometry, screens, **kw):
                                  species="electron"))
         t0 = time.time()
         tr = SpecificParticleTracer(pg, n_emit, geometry, screens, **kw)
-        res, _ = tr.run()
+        res, traj = tr.run()
         dt = time.time() - t0
     finally:
         batched_rk45._attempt_step = orig_attempt
@@ -297,6 +297,11 @@ def traced_run(name, src, n_emit, geometry, screens, **kw):
     for si, s in enumerate(res):
         for f in ("x", "y", "z", "px", "py", "pz", "t", "id"):
             out[f"s{si}_{f}"] = np.asarray(getattr(s, f))
+    if kw.get("t_out") is not None:
+        out["t_out"] = np.asarray(kw["t_out"], dtype=float)
+        for k, s in enumerate(traj):
+            for f in ("x", "y", "z", "px", "py", "pz", "t", "id"):
+                out[f"o{k}_{f}"] = np.asarray(getattr(s, f)) if s is not None and len(s) else np.array([])
     np.savez(os.path.join(HERE, f"run_{name}.npz"), **out)
     print(f"run_{name}: groups {G}, attempts {out['attempts'].tolist()[:8]}..., "
           f"events {[len(s) for s in res]}, {dt:.1f} s", flush=True)
@@ -333,6 +338,12 @@ def main(which):
         src["t"] = np.repeat([0.0, 2e-14, 0.0, 5e-14], 8) + np.tile(np.linspace(0, 3e-14, 8), 4)
         traced_run("c1small", src, 8, FlatCathode(-1e8, z0=3e-9), [0.2e-6, 1e-6], rtol=1e-8, atol=1e-12,
                    t_max=1.5e-12)
+    if "c1out" in which:
+        # c1small with trajectory snapshots: one at the first group's start, one before some births
+        src = emission.flat_gaussian_source(32, seed=3)
+        src["t"] = np.repeat([0.0, 2e-14, 0.0, 5e-14], 8) + np.tile(np.linspace(0, 3e-14, 8), 4)
+        traced_run("c1out", src, 8, FlatCathode(-1e8, z0=3e-9), [0.2e-6, 1e-6], rtol=1e-8, atol=1e-12,
+                   t_max=1.5e-12, t_out=[0.0, 3e-14, 1e-13, 4.5e-13, 1.5e-12])
     if "tipbem" in which:
         R = 1e-6
         src = emission.cap_source(8, R, 0.0, seed=4)
@@ -348,4 +359,4 @@ def main(which):
 
 if __name__ == "__main__":
     main(sys.argv[1:] or ["sphere", "post80", "c1", "c1small", "tipbem", "post", "post400", "well", "wellrun", "wellimage",
-                          "wellimgrun"])
+                          "wellimgrun", "c1out"])

@@ -186,6 +186,35 @@ def test_trace_matches_reference_runs(torch_cuda, name, mode):
             np.testing.assert_allclose(getattr(s, f), ref, rtol=1e-9, atol=1e-9 * max(np.abs(ref).max(), 1e-300))
 
 
+@pytest.mark.parametrize("mode", ["persistent", "lockstep"])
+def test_trace_output_times_match_reference_run(torch_cuda, mode):
+    """t_out snapshots (reference tracker.py t_out, trajectories.py): the
+    steps land on every output time, also one equal to a group's start
+    time; records in the reference's order, identical step counts."""
+    from bem_amd import FlatCathode, SpecificParticleTracer
+    from bem_amd.particles import ParticleGroup
+
+    _say("trace output times", mode)
+    d = golden("run_c1out.npz")
+    src = {k[3:]: d[k] for k in d.files if k.startswith("in_")}
+    n = len(src["x"])
+    pg = ParticleGroup(dict(src, weight=np.full(n, 1.602176634e-19), status=np.ones(n), species="electron"))
+    tr = SpecificParticleTracer(pg, int(d["n_emit"]), FlatCathode(-1e8, z0=3e-9), list(d["screens"]),
+                                t_out=list(d["t_out"]), rtol=float(d["rtol"]), atol=float(d["atol"]),
+                                t_max=float(d["t_max"]), mode=mode)
+    screens, traj = tr.run()
+    np.testing.assert_array_equal(tr.last_run["attempts"], d["attempts"])
+    np.testing.assert_array_equal(tr.last_run["accepted"], d["accepted"])
+    assert len(traj) == len(d["t_out"])
+    for k, s in enumerate(traj):
+        np.testing.assert_array_equal(s.id, d[f"o{k}_id"])
+        if len(s.id):
+            np.testing.assert_allclose(s.t, d[f"o{k}_t"], rtol=1e-15)
+            for f in ("x", "y", "z", "px", "py", "pz"):
+                ref = d[f"o{k}_{f}"]
+                np.testing.assert_allclose(getattr(s, f), ref, rtol=1e-9, atol=1e-9 * max(np.abs(ref).max(), 1e-300))
+
+
 def test_trace_block_sizes_agree(torch_cuda):
     """Workgroup size changes only the field partial-sum order: results stay
     within 1e-12 and step counts are unchanged."""

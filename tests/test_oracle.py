@@ -220,6 +220,22 @@ def run_oracle(name):
     return d, st, out
 
 
+def test_oracle_output_times_match_reference_run():
+    d = golden("run_c1out.npz")
+    st = otr.setup(d["in_x"], d["in_y"], d["in_z"], d["in_px"], d["in_py"], d["in_pz"], d["in_t"], d["in_id"],
+                   int(d["n_emit"]))
+    out = otr.run(st, _geom_for("c1small"), list(d["screens"]), float(d["t_max"]), rtol=float(d["rtol"]),
+                  atol=float(d["atol"]), t_out=d["t_out"])
+    np.testing.assert_array_equal(out["attempts"], d["attempts"])
+    for k in range(len(d["t_out"])):
+        rec = out["trajectories"][k]
+        ids = np.array([st["ids"][r["group"], r["particle"]] for r in rec], dtype=int)
+        np.testing.assert_array_equal(ids, d[f"o{k}_id"])
+        if len(rec):
+            np.testing.assert_allclose([r["t"] for r in rec], d[f"o{k}_t"], rtol=1e-15)
+            np.testing.assert_allclose([r["pos"][2] for r in rec], d[f"o{k}_z"], rtol=1e-9, atol=1e-15)
+
+
 @pytest.mark.parametrize("name", ["c1small", "tipbem", "post80", "well", "wellimg"])
 def test_oracle_integrator_matches_reference_run(name):
     d, st, out = run_oracle(name)
