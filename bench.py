@@ -41,7 +41,7 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-RK-stages/sec, 1 GPU, surrogate+batched vs plain at field err ≤1e-6"
 PEAK_FP64_TFLOPS = 78.6
 C_RING = 120          # FP64 flops per ring-kernel node (K, E, log, kernel algebra, accumulation)
-TRAFFIC_BYTES = 43.5e6  # HBM bytes per 262,144-point field-gather launch, from the rocprofv3 counter runs
+TRAFFIC_BYTES = 54.7e6  # HBM bytes per 262,144-point field-gather launch at (192, 32), from the rocprofv3 counter runs
 
 
 def workload(n_el=262144, group=16, n_segments=400, seed=0, rtol=1e-8):
@@ -191,6 +191,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--field-only", action="store_true",
                     help="only the field-gather timing and a 256 MiB device copy (for hardware-counter runs)")
+    ap.add_argument("--field-shape", default="0,0",
+                    help="threads,points-per-block of the field gather for --field-only (0,0 = automatic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the tracing path is benchmarked per GPU (groups shard across GPUs "
@@ -209,12 +211,13 @@ def main():
     if args.field_only:
         import torch
 
-        field_ms, n_pts = field_gather_timing(wl, geo, reps=3)
+        shape = tuple(int(v) for v in args.field_shape.split(","))
+        field_ms, n_pts = field_gather_timing(wl, geo, reps=3, shape=shape)
         src = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-        print(json.dumps(dict(field_ms=field_ms, points=n_pts, copy_bytes_read=src.numel() * 4,
+        print(json.dumps(dict(field_ms=field_ms, points=n_pts, field_shape=shape, copy_bytes_read=src.numel() * 4,
                               copy_bytes_written=dst.numel() * 4)))
         return
     say(f"bench: BEM solve {t_solve:.2f} s; training the surrogate ({args.train_steps} steps)")
