@@ -9,6 +9,7 @@
 #include <stdint.h>
 
 #include "ellip_coeffs.h"
+#include "log_coeffs.h"
 
 namespace spt {
 
@@ -26,13 +27,46 @@ __constant__ double c_QK[11] = SPT_ELLIP_QK;
 
 // Horner evaluation of a degree-10 polynomial and its derivative.
 __device__ __forceinline__ void horner11_d(const double* c, double x, double& s, double& d) {
-  s = c[10];
-  d = 0.0;
+  d = c[10];
+  s = fma(d, x, c[9]);
 #pragma unroll
-  for (int i = 9; i >= 0; --i) {
+  for (int i = 8; i >= 0; --i) {
     d = fma(d, x, s);
     s = fma(s, x, c[i]);
   }
+}
+
+// Natural logarithm of a finite x >= 0 (x = 0 gives -inf), within 0.6 x
+// 2^-52 relative: x = 2^e m, m in [sqrt(1/2), sqrt(2)), f = m - 1,
+// s = f / (2 + f), ln m = 2 atanh(s) = f - (f^2/2 - s (f^2/2 + R(s^2))),
+// R(z) = z T(z) with T fitted by gen_log_coeffs.py (the atanh reduction
+// of Cody and Waite's logarithm). Only the small terms carry the rounding
+// of s, so one Newton correction of the quotient suffices. About 30
+// instructions against the ~75 of the library log, whose double-double
+// steps this kernel does not need (K and E tolerate ~1 ulp in ln p).
+__device__ __forceinline__ double log_pos(double x) {
+  constexpr double T[7] = SPT_LOG_T;
+  double m = __builtin_amdgcn_frexp_mant(x);
+  int e = __builtin_amdgcn_frexp_exp(x);
+  if (m < 0.70710678118654752440) {
+    m = m + m;
+    e = e - 1;
+  }
+  const double f = m - 1.0;
+  const double d = 2.0 + f;
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  double s = f * r;
+  s = fma(r, fma(-d, s, f), s);
+  const double z = s * s;
+  double t = T[6];
+#pragma unroll
+  for (int i = 5; i >= 0; --i) t = fma(t, z, T[i]);
+  const double R = z * t;
+  const double hf = 0.5 * (f * f);
+  const double de = (double)e;
+  const double v = fma(de, SPT_LOG_LN2_HI, -((hf - fma(s, hf + R, de * SPT_LOG_LN2_LO)) - f));
+  return x == 0.0 ? -INFINITY : v;
 }
 
 // K(1-p), E(1-p) for p in [0, 1].
@@ -42,7 +76,7 @@ __device__ __forceinline__ void horner11_d(const double* c, double x, double& s,
 //   (DLMF 19.4.1) with m = 1 - p:  E = p K - 2 (1-p) p dK/dp,
 //   p dK/dp = p PK'(p) - QK(p) - p ln(p) QK'(p)  (~1.6 ulp).
 __device__ __forceinline__ void ellip_ke_c(double p, double& K, double& E) {
-  const double lp = log(p);
+  const double lp = log_pos(p);
   double P, dP, Q, dQ;
   horner11_d(c_PK, p, P, dP);
   horner11_d(c_QK, p, Q, dQ);
@@ -52,7 +86,7 @@ __device__ __forceinline__ void ellip_ke_c(double p, double& K, double& E) {
 }
 
 __device__ __forceinline__ double ellip_k_c(double p) {
-  const double lp = log(p);
+  const double lp = log_pos(p);
   double s = c_PK[10], q = c_QK[10];
 #pragma unroll
   for (int i = 9; i >= 0; --i) {
@@ -62,23 +96,21 @@ __device__ __forceinline__ double ellip_k_c(double p) {
   return fma(-lp, q, s);
 }
 
-// 1/x to ~1 ulp: hardware reciprocal estimate plus two Newton steps.
+// 1/x to ~1 ulp: hardware reciprocal estimate r (relative error ~2^-23),
+// e = 1 - x r, then one third-order step 1/x = r (1 + e + e^2) + O(e^3).
 __device__ __forceinline__ double rcp64(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-x, r, 1.0);
-  return fma(r, e, r);
+  const double r = __builtin_amdgcn_rcp(x);
+  const double e = fma(-x, r, 1.0);
+  return fma(r, fma(e, e, e), r);
 }
 
-// 1/sqrt(x) to ~1 ulp: hardware estimate plus two Newton steps.
+// 1/sqrt(x) to ~1 ulp: hardware estimate y (relative error ~2^-23),
+// e = 1 - x y^2, then one third-order step
+// 1/sqrt(x) = y (1 + e/2 + 3 e^2/8) + O(e^3).
 __device__ __forceinline__ double rsqrt64(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  double h = 0.5 * x;
-  double e = fma(-h * y, y, 0.5);
-  y = fma(y, e, y);
-  e = fma(-h * y, y, 0.5);
-  return fma(y, e, y);
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-(x * y), y, 1.0);
+  return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 
 // Near-axis Taylor terms of the ring field (reference

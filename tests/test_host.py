@@ -81,6 +81,59 @@ def _coeffs(name):
     return [float(x) for x in re.search(r"SPT_ELLIP_%s \{ (.*) \}" % name, src).group(1).split(",")]
 
 
+def _fma(a, b, c):
+    """fma in 64-bit-mantissa long double, rounded once more to double
+    (within 2^-11 ulp of the exactly rounded fma)."""
+    L = np.longdouble
+    return (L(a) * L(b) + L(c)).astype(np.float64)
+
+
+def _device_log(x, drop_lo=False, n_terms=7):
+    """The device logarithm (spt_device.h log_pos) restated in NumPy, the
+    hardware reciprocal estimate modelled by a float32-rounded 1/d.
+    drop_lo / n_terms plant mistakes for the test's own check."""
+    src = open(os.path.join(ROOT, "bem_amd", "csrc", "log_coeffs.h")).read()
+    T = [float(v) for v in re.search(r"SPT_LOG_T \{ (.*) \}", src).group(1).split(",")][:n_terms]
+    hi = float(re.search(r"SPT_LOG_LN2_HI (\S+)", src).group(1))
+    lo = 0.0 if drop_lo else float(re.search(r"SPT_LOG_LN2_LO (\S+)", src).group(1))
+    x = np.asarray(x, dtype=np.float64)
+    m, e = np.frexp(x)
+    low = m < 0.70710678118654752440
+    m = np.where(low, m + m, m)
+    e = np.where(low, e - 1, e).astype(np.float64)
+    f = m - 1.0
+    d = 2.0 + f
+    with np.errstate(divide="ignore"):
+        r = (1.0 / d).astype(np.float32).astype(np.float64)
+    r = _fma(r, _fma(-d, r, 1.0), r)
+    s = f * r
+    s = _fma(r, _fma(-d, s, f), s)
+    z = s * s
+    t = np.full_like(z, T[-1])
+    for c in reversed(T[:-1]):
+        t = _fma(t, z, c)
+    R = z * t
+    hf = 0.5 * (f * f)
+    v = _fma(e, hi, -((hf - _fma(s, hf + R, e * lo)) - f))
+    return np.where(x == 0.0, -np.inf, v)
+
+
+def test_device_log_against_numpy():
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.random(20000), 10 ** rng.uniform(-300, 0, 20000), 1 - 10 ** rng.uniform(-16, 0, 20000),
+                        [5e-324, 2.0**-1022, 0.5, 0.70710678118654752440, np.nextafter(1.0, 0.0)]])
+    got, ref = _device_log(x), np.log(x)
+    nz = ref != 0
+    assert np.max(np.abs(got[nz] / ref[nz] - 1)) < 2 * 2.0**-52
+    assert _device_log(np.array([1.0]))[0] == 0.0
+    assert _device_log(np.array([0.0]))[0] == -np.inf
+    # planted slips fail the same bound: ln 2's low part dropped, the
+    # polynomial one term short
+    for kw in (dict(drop_lo=True), dict(n_terms=6)):
+        bad = _device_log(x, **kw)
+        assert np.max(np.abs(bad[nz] / ref[nz] - 1)) > 2 * 2.0**-52, kw
+
+
 def _device_ke(p):
     """The device formula (spt_device.h ellip_ke_c) restated in NumPy."""
     PK, QK = _coeffs("PK"), _coeffs("QK")
@@ -93,7 +146,7 @@ def _device_ke(p):
             s = s * x + a
         return s, d
 
-    lp = np.log(p)
+    lp = _device_log(p)
     P, dP = hd(PK, p)
     Q, dQ = hd(QK, p)
     K = P - lp * Q
