@@ -120,7 +120,7 @@ __global__ void node_table_kernel(const double* prof, const double* sigma, int n
 // Minimum waves per SIMD requested for the field-gather kernels (caps their
 // VGPR budget at 512 / this).
 #ifndef SPT_FIELD_MIN_WAVES
-#define SPT_FIELD_MIN_WAVES 1
+#define SPT_FIELD_MIN_WAVES 3
 #endif
 // Near segment (closer than its own length): s = s0 + d sinh(v), Gauss-8 in
 // intervals of width <= 0.5 over [asinh(-s0/d), asinh((L-s0)/d)], density
@@ -254,11 +254,21 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
             ring_accum32(re, re2, ze - v.y, v.x, (float)v.z, acc32, acc);
           }
         } else {
+          // Segments that cannot meet the near-axis branch take the
+          // branch-free main term, two nodes per trip (nq is a multiple of 4).
           const double4* nd = B.nodes + (size_t)j * B.nq;
+          if (ring_axis_possible(re2, ze, g[S_P0R], g[S_P0Z], g[S_ABR], g[S_ABZ])) {
 #pragma unroll 1
-          for (int q = 0; q < B.nq; ++q) {
-            const double4 v = nd[q];
-            ring_accum(re, re2, ze - v.y, v.x, v.z, acc);
+            for (int q = 0; q < B.nq; ++q) {
+              const double4 v = nd[q];
+              ring_accum(re, re2, ze - v.y, v.x, v.z, acc);
+            }
+          } else {
+#pragma unroll 2
+            for (int q = 0; q < B.nq; ++q) {
+              const double4 v = nd[q];
+              ring_accum_main(re, re2, ze - v.y, v.x, v.z, acc);
+            }
           }
         }
       }
@@ -290,10 +300,18 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
           }
         } else {
           const double4* nd = B.nodes + (size_t)u * 4;
+          if (ring_axis_possible(re2, ze, g[S_P0R], g[S_P0Z], g[S_ABR], g[S_ABZ])) {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const double4 v = nd[q];
-            ring_accum(re, re2, ze - v.y, v.x, v.z, acc);
+            for (int q = 0; q < 4; ++q) {
+              const double4 v = nd[q];
+              ring_accum(re, re2, ze - v.y, v.x, v.z, acc);
+            }
+          } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const double4 v = nd[q];
+              ring_accum_main(re, re2, ze - v.y, v.x, v.z, acc);
+            }
           }
         }
       }

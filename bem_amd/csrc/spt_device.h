@@ -133,16 +133,10 @@ struct RingAcc {
   __device__ void zero() { er_main = ez_main = er_ax = ez_ax = 0.0; }
 };
 
-__device__ __forceinline__ void ring_accum(double rho, double rho2, double zz, double a, double w, RingAcc& acc) {
+// Main-branch term of ring_accum for a node known not to be near-axis.
+__device__ __forceinline__ void ring_accum_main(double rho, double rho2, double zz, double a, double w, RingAcc& acc) {
   const double zz2 = zz * zz;
   const double a2 = a * a;
-  if (rho2 < 1e-10 * (a2 + zz2)) {
-    double er, ez;
-    ring_field_axis(rho, zz, a, er, ez);
-    acc.er_ax = fma(w, er, acc.er_ax);
-    acc.ez_ax = fma(w, ez, acc.ez_ax);
-    return;
-  }
   const double sp = rho + a, sm = rho - a;
   const double D = fma(sp, sp, zz2);
   const double d2 = fma(sm, sm, zz2);
@@ -155,6 +149,29 @@ __device__ __forceinline__ void ring_accum(double rho, double rho2, double zz, d
   const double wr = w * rD;
   acc.er_main = fma(wr, fma(-c * E, inv_d2, K), acc.er_main);
   acc.ez_main = fma(wr * zz, E * inv_d2, acc.ez_main);
+}
+
+__device__ __forceinline__ void ring_accum(double rho, double rho2, double zz, double a, double w, RingAcc& acc) {
+  if (rho2 < 1e-10 * (a * a + zz * zz)) {
+    double er, ez;
+    ring_field_axis(rho, zz, a, er, ez);
+    acc.er_ax = fma(w, er, acc.er_ax);
+    acc.ez_ax = fma(w, ez, acc.ez_ax);
+    return;
+  }
+  ring_accum_main(rho, rho2, zz, a, w, acc);
+}
+
+// Can any node of the segment (p0r, p0z) + t (abr, abz), t in [0, 1],
+// take ring_accum's near-axis branch for a field point at (rho, z)? Its
+// nodes have a <= max(p0r, p1r) and |z - z_node| <= max(|z - p0z|,
+// |z - p1z|), so no node does when rho^2 >= 1e-10 (a_max^2 + zz_max^2);
+// the 1 % margin covers the rounding of the node table and of the test.
+__device__ __forceinline__ bool ring_axis_possible(double rho2, double z, double p0r, double p0z, double abr,
+                                                   double abz) {
+  const double amax = fmax(fabs(p0r), fabs(p0r + abr));
+  const double zmax = fmax(fabs(z - p0z), fabs(z - (p0z + abz)));
+  return rho2 < 1.01e-10 * (amax * amax + zmax * zmax);
 }
 
 __device__ __forceinline__ void ring_finish(const RingAcc& acc, double rho, double& er, double& ez) {
