@@ -25,7 +25,8 @@ trace): algorithmic FP64 flops = points x 12,800 nodes x 120 flops per
 node, over its average launch time from HIP events on the launch stream,
 on 262,144 points 10 nm .. 10 um off the cap. cpu_baseline: the oracle
 (NumPy FP64 restatement of the reference) on a time-bounded sample of the
-same workload, one process.
+same workload, one forked process per host core (up to 16), run before the
+GPU is touched.
 """
 import argparse
 import json
@@ -122,22 +123,23 @@ def field_gather_timing(wl, geo, n_pts=262144, reps=10, shape=None):
     return e0.elapsed_time(e1) / reps, n_pts
 
 
-def cpu_baseline(wl, seconds=15.0):
-    """Oracle on the first group of the workload until `seconds` of work."""
-    from oracle import bem as ob
+_CPU = {}  # oracle state shared with the forked baseline workers
+
+
+def _cpu_group_rate(gi):
+    """Oracle trace of group gi of the workload for _CPU["seconds"] of work:
+    (particle-stages, elapsed s)."""
+    from threadpoolctl import threadpool_limits
+
     from oracle import rk45
     from oracle import tracker as otr
-    from bem_amd.bem.mesh import post_profile
 
-    src = wl["src"]
-    g = wl["group"]
-    R, H = wl["R"], wl["H"]
-    prof = post_profile(R, H, wl["n_segments"])
-    sigma = ob.solve(prof, lambda r, z: wl["E_gun"] * z)
-    st = otr.setup(*(src[k][:g] for k in ("x", "y", "z", "px", "py", "pz", "t", "id")), g)
-    geom = dict(kind="post_bem", E_gun=wl["E_gun"], R=R, H=H, profile=prof, sigma=sigma, z_handoff=20 * (R + H),
-                kill_z_below=0.0, z_max=wl["z_max"])
-    accel = otr.make_accel(geom, st["charge"], st["mass_p"], st["q_over_m"], 1e-12)
+    threadpool_limits(1)  # one core per process: no BLAS threads on top
+    wl, geom_base, seconds = _CPU["wl"], _CPU["geom"], _CPU["seconds"]
+    src, g = wl["src"], wl["group"]
+    sl = slice(gi * g, (gi + 1) * g)
+    st = otr.setup(*(src[k][sl] for k in ("x", "y", "z", "px", "py", "pz", "t", "id")), g)
+    accel = otr.make_accel(geom_base, st["charge"], st["mass_p"], st["q_over_m"], 1e-12)
     count = [0]
     t0 = [None]
 
@@ -155,13 +157,37 @@ def cpu_baseline(wl, seconds=15.0):
     t_max = otr.default_t_max(st, wl["E_gun"], wl["screens"])
     try:
         rk45.integrate(timed, st["pos"], st["vel"], st["t_birth"], t_max, wl["rtol"], wl["atol"],
-                       kill=lambda p: otr.kill_mask(geom, p))
+                       kill=lambda p: otr.kill_mask(geom_base, p))
     except Done:
         pass
-    el = time.perf_counter() - t0[0]
-    return dict(value=count[0] / el, unit="particle-RK-stages/s", cores=1, kind="port",
-                sample=f"oracle (NumPy FP64) on group 0 ({g} electrons) of the workload for {el:.1f} s "
-                       f"({count[0]} particle-stages), 1 process")
+    return count[0], time.perf_counter() - t0[0]
+
+
+def cpu_baseline(wl, seconds=15.0, workers=None):
+    """The oracle on the first `workers` groups of the workload, one forked
+    process per group and host core (the reference's own process-level
+    group partitioning), each for `seconds` of work. Must run before
+    anything touches the GPU (the workers are forked)."""
+    import multiprocessing as mp
+
+    from oracle import bem as ob
+    from bem_amd.bem.mesh import post_profile
+
+    R, H = wl["R"], wl["H"]
+    prof = post_profile(R, H, wl["n_segments"])
+    sigma = ob.solve(prof, lambda r, z: wl["E_gun"] * z)
+    _CPU.update(wl=wl, seconds=seconds,
+                geom=dict(kind="post_bem", E_gun=wl["E_gun"], R=R, H=H, profile=prof, sigma=sigma,
+                          z_handoff=20 * (R + H), kill_z_below=0.0, z_max=wl["z_max"]))
+    if workers is None:
+        workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    with mp.get_context("fork").Pool(workers) as pool:
+        parts = pool.map(_cpu_group_rate, range(workers))
+    count = sum(c for c, _ in parts)
+    el = max(e for _, e in parts)
+    return dict(value=count / el, unit="particle-RK-stages/s", cores=workers, kind="port",
+                sample=f"oracle (NumPy FP64) on groups 0..{workers - 1} ({wl['group']} electrons each) of the "
+                       f"workload, one process per group and core, {el:.1f} s each ({count} particle-stages)")
 
 
 def c2_plain_rate(steps=3):
@@ -205,6 +231,11 @@ def main():
         print(*a, file=sys.stderr, flush=True)
 
     wl = workload(n_el=args.electrons)
+    cpu = None
+    if not args.no_cpu_baseline and not args.field_only:
+        say(f"bench: CPU baseline (oracle, {args.cpu_seconds:.0f} s per process)")
+        cpu = cpu_baseline(wl, args.cpu_seconds)  # before the GPU is touched: the workers are forked
+        say(f"bench: CPU baseline {cpu['value']:.0f} particle-stages/s on {cpu['cores']} cores")
     t0 = time.perf_counter()
     geo = PostTipBEMGeometry(wl["E_gun"], wl["R"], wl["H"], n_segments=wl["n_segments"])
     t_solve = time.perf_counter() - t0
@@ -298,9 +329,8 @@ def main():
                      "note": "FP64 VALU-bound; MI355X FP64 vector peak = FP64 MFMA peak = 78.6 TFLOP/s"},
         "c2_plain_fp64": c2_plain_rate(),
     }
-    if not args.no_cpu_baseline:
-        wl_cpu = dict(wl)
-        res["cpu_baseline"] = cpu_baseline(wl_cpu, args.cpu_seconds)
+    if cpu is not None:
+        res["cpu_baseline"] = cpu
     print(json.dumps(res))
 
 
