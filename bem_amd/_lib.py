@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(_HERE, "libspt_hip.so")
 SPT_OK, SPT_EINVAL, SPT_ENOMEM, SPT_EHIP, SPT_ENOBEM, SPT_ESTEPCAP = 0, -1, -2, -3, -4, -5
 SPT_MAX_SCREENS = 16
 GEOM_FLAT, GEOM_TIP, GEOM_BEM_POST, GEOM_BEM_WELL = 0, 1, 2, 3
-IMAGE_NONE, IMAGE_PLANE, IMAGE_TIP3, IMAGE_BEM = 0, 1, 2, 3
+IMAGE_NONE, IMAGE_PLANE, IMAGE_TIP3, IMAGE_BEM, IMAGE_BEM_MIRROR = 0, 1, 2, 3, 4
 STATUS_DONE, STATUS_STEPCAP = 0, 1
 MODE_PERSISTENT, MODE_LOCKSTEP = 0, 1
 PREC_FP64, PREC_MIXED, PREC_PAIR32 = 0, 1, 2

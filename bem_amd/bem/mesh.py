@@ -185,3 +185,47 @@ def cylindrical_well_image_profile(R, H, z0, plane_radius, max_length, bottom_fi
     parts.append(graded_annulus(r0, plane_radius, -z0, region_max_length(max_length, plane_max_length),
                                 growth=plane_growth)[1:])
     return np.concatenate(parts, axis=0)
+
+
+def sphere_cap(radius, n_theta):
+    """Sphere cap from the pole (polar angle 0) to the equator, uniform in
+    polar angle."""
+    th = np.linspace(0.0, 0.5 * np.pi, n_theta)
+    return np.column_stack([radius * np.sin(th), radius * np.cos(th)])
+
+
+def _recessed_tip(R, z0, max_length, cap_max_length, fillet_max_length):
+    """Cap of radius R - z0 (pole first) and the quarter-torus fillet of tube
+    radius z0 about rho = R that rounds the recessed ridge down to the
+    point (R, -z0), where its tangent is horizontal."""
+    if not (0.0 < z0 < R):
+        raise ValueError(f"z0 must satisfy 0 < z0 < R (got z0={z0!r}, R={R!r})")
+    cap = sphere_cap(R - z0, n_for_length(0.5 * np.pi * (R - z0), region_max_length(max_length, cap_max_length)))
+    n = n_for_length(0.5 * np.pi * z0, region_max_length(max_length, fillet_max_length),
+                     min_points=MIN_FILLET_SEGMENTS + 1)
+    return cap, arc(R, z0, np.pi, 1.5 * np.pi, n)[1:]
+
+
+def hemisphere_tip_image_profile(R, z0, plane_radius, max_length, cap_max_length=None, fillet_max_length=None,
+                                 plane_max_length=None, plane_growth=1.15):
+    """The recessed image surface of a hemisphere of radius R on a plane,
+    truncated at plane_radius (reference ``bem/mesh.py:200-240``): the cap
+    of radius R - z0, the ridge fillet of radius z0 (tangent to the cap and
+    to the plane), then the graded plane at z = -z0 out to plane_radius."""
+    cap, fillet = _recessed_tip(R, z0, max_length, cap_max_length, fillet_max_length)
+    plane = graded_annulus(R, plane_radius, -z0, region_max_length(max_length, plane_max_length),
+                           growth=plane_growth)[1:]
+    return np.concatenate([cap, fillet, plane], axis=0)
+
+
+def hemisphere_tip_image_doubled_profile(R, z0, max_length, cap_max_length=None, fillet_max_length=None):
+    """The same recessed cap and fillet, closed by their reflection through
+    the image plane z = -z0 instead of a truncated plane (reference
+    ``bem/mesh.py:243-314``): pole to mirrored pole, the corner (R, -z0)
+    shared by both halves. Used with a phantom source -q at the reflection
+    of every real charge."""
+    cap, fillet = _recessed_tip(R, z0, max_length, cap_max_length, fillet_max_length)
+    upper = np.concatenate([cap, fillet], axis=0)
+    lower = upper[-2::-1].copy()
+    lower[:, 1] = -2.0 * z0 - lower[:, 1]
+    return np.concatenate([upper, lower], axis=0)

@@ -364,6 +364,10 @@ __device__ __forceinline__ double post_r2(double x, double y, double z, double H
   return x * x + y * y + dz * dz;
 }
 
+__host__ __device__ inline bool is_image_bem(int image) {
+  return image == SPT_IMAGE_BEM || image == SPT_IMAGE_BEM_MIRROR;
+}
+
 __device__ __forceinline__ bool is_bem(const spt_geometry& G) {
   return G.kind == SPT_GEOM_BEM_POST || G.kind == SPT_GEOM_BEM_WELL;
 }
@@ -506,7 +510,7 @@ __device__ void pair_forces_i32(const spt_geometry& G, int i, int n, const doubl
   Fc[0] = cx.value();
   Fc[1] = cy.value();
   Fc[2] = cz.value();
-  if (G.image == SPT_IMAGE_NONE) return;
+  if (G.image != SPT_IMAGE_PLANE && G.image != SPT_IMAGE_TIP3) return;  // the BEM image models have their own kernels
   Kahan32 ix, iy, iz;
   ix.zero();
   iy.zero();
@@ -1989,9 +1993,13 @@ int spt_bem_pert_field(spt_ctx* c, const double* pos, int64_t n, double* E, int3
 static int check_geom(spt_ctx* c, const spt_geometry* g) {
   if (!g) return fail(c, SPT_EINVAL, "null geometry");
   if (g->kind < SPT_GEOM_FLAT || g->kind > SPT_GEOM_BEM_WELL) return fail(c, SPT_EINVAL, "unknown geometry kind");
-  if (g->image < SPT_IMAGE_NONE || g->image > SPT_IMAGE_BEM) return fail(c, SPT_EINVAL, "unknown image model");
-  if (g->image == SPT_IMAGE_BEM) {
-    if (g->kind != SPT_GEOM_BEM_WELL) return fail(c, SPT_EINVAL, "the image-charge BEM is built for the well");
+  if (g->image < SPT_IMAGE_NONE || g->image > SPT_IMAGE_BEM_MIRROR) return fail(c, SPT_EINVAL, "unknown image model");
+  if (is_image_bem(g->image)) {
+    if (g->kind != SPT_GEOM_BEM_WELL && g->kind != SPT_GEOM_BEM_POST)
+      return fail(c, SPT_EINVAL, "the image-charge BEM needs a BEM geometry");
+    if (g->image == SPT_IMAGE_BEM_MIRROR && g->kind != SPT_GEOM_BEM_POST)
+      return fail(c, SPT_EINVAL, "the mirror-doubled image surface is built for the tip");
+    if (!(g->z0 > 0.0)) return fail(c, SPT_EINVAL, "the image-charge BEM needs z0 > 0");
     if (!c->img_P) return fail(c, SPT_ENOBEM, "image-charge BEM without an uploaded image table");
   }
   const bool bem = g->kind == SPT_GEOM_BEM_POST || g->kind == SPT_GEOM_BEM_WELL;
@@ -2106,6 +2114,8 @@ static ImageView image_view(const spt_ctx* c) {
   B.rhs = c->d_img_rhs;
   B.sigma = c->d_img_sigma;
   B.mirror = c->d_img_mirror;
+  B.mirror_mode = 0;
+  B.plane_z = 0.0;
   return B;
 }
 
@@ -2125,7 +2135,7 @@ static int ensure_image_scratch(spt_ctx* c, int n_groups, int64_t n) {
     if (c->d_img_F) (void)hipFree(c->d_img_F);
     c->d_img_mirror = c->d_img_F = nullptr;
     c->img_part_cap = 0;
-    HIPCHK(c, hipMalloc(&c->d_img_mirror, sizeof(double) * 4 * n));
+    HIPCHK(c, hipMalloc(&c->d_img_mirror, sizeof(double) * 8 * n));
     HIPCHK(c, hipMalloc(&c->d_img_F, sizeof(double) * 3 * n));
     c->img_part_cap = n;
   }
@@ -2146,7 +2156,11 @@ static PosView stage_pos_view(const LockParams& L) {
 static void launch_image_force(spt_ctx* c, const spt_geometry& G, PosView X, const double* charge,
                                const uint8_t* active, const int64_t* offsets, const int32_t* gflags, int running,
                                int n_groups, double a2, double* F, hipStream_t s) {
-  const ImageView B = image_view(c);
+  ImageView B = image_view(c);
+  if (G.image == SPT_IMAGE_BEM_MIRROR) {
+    B.mirror_mode = 1;
+    B.plane_z = -G.z0;
+  }
   img_rhs_kernel<<<n_groups, 256, 0, s>>>(B, X, charge, active, offsets, gflags, running);
   img_solve_kernel<<<dim3(n_groups, 2 * B.n_modes), 128, 0, s>>>(B, gflags, running);
   img_force_kernel<<<n_groups, 256, 0, s>>>(B, G, X, charge, active, offsets, gflags, running, a2, F);
@@ -2292,7 +2306,7 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
     return fail(c, SPT_EINVAL, "unknown precision");
   if (a->precision == SPT_PREC_PAIR32 && a->mode != SPT_MODE_LOCKSTEP)
     return fail(c, SPT_EINVAL, "precision PAIR32 (FP64 field, FP32 pairs) runs in lockstep mode only");
-  if (a->geom.image == SPT_IMAGE_BEM && a->mode != SPT_MODE_LOCKSTEP)
+  if (is_image_bem(a->geom.image) && a->mode != SPT_MODE_LOCKSTEP)
     return fail(c, SPT_EINVAL, "the image-charge BEM runs in lockstep mode only");
   if (a->use_surrogate) {
     if (a->mode != SPT_MODE_LOCKSTEP) return fail(c, SPT_EINVAL, "the field surrogate runs in lockstep mode only");
@@ -2369,7 +2383,7 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
     memset(&L.V, 0, sizeof(L.V));
     memset(&L.V64, 0, sizeof(L.V64));
     L.imgF = nullptr;
-    if (a->geom.image == SPT_IMAGE_BEM) {
+    if (is_image_bem(a->geom.image)) {
       int rc4 = ensure_image_scratch(c, a->n_groups, a->n_particles);
       if (rc4) return rc4;
       L.imgF = c->d_img_F;
@@ -2803,7 +2817,8 @@ int spt_image_force(spt_ctx* c, const spt_geometry* g, const double* pos, const 
   if (!c) return SPT_EINVAL;
   int rc = check_geom(c, g);
   if (rc) return rc;
-  if (g->image != SPT_IMAGE_BEM) return fail(c, SPT_EINVAL, "spt_image_force needs image = SPT_IMAGE_BEM");
+  if (!is_image_bem(g->image))
+    return fail(c, SPT_EINVAL, "spt_image_force needs image = SPT_IMAGE_BEM or SPT_IMAGE_BEM_MIRROR");
   if (n_groups < 0 || (n_groups > 0 && (!pos || !charge || !active || !offsets || !F)))
     return fail(c, SPT_EINVAL, "spt_image_force: bad arguments");
   if (n_groups == 0) return SPT_OK;

@@ -56,9 +56,15 @@ extern "C" {
 #define SPT_IMAGE_NONE 0
 #define SPT_IMAGE_PLANE 1  /* image at z' = -2 z0 - z, charge -q            */
 #define SPT_IMAGE_TIP3 2   /* sphere R - z0 + plane z0: three images        */
-#define SPT_IMAGE_BEM 3    /* well: image-charge BEM on the uploaded image  */
-                           /* surface near, plane image z0 far, blended     */
-                           /* (spt_image_upload; LOCKSTEP traces only)      */
+#define SPT_IMAGE_BEM 3    /* well or tip: image-charge BEM on the uploaded */
+                           /* truncated-plane image surface near, plane     */
+                           /* image z0 far, blended (spt_image_upload;      */
+                           /* LOCKSTEP traces only)                         */
+#define SPT_IMAGE_BEM_MIRROR 4 /* tip: as SPT_IMAGE_BEM on the uploaded     */
+                           /* mirror-doubled image surface, with a phantom  */
+                           /* -q at z' = -2 z0 - z per particle and its     */
+                           /* plane image (reference bem/geometry.py:       */
+                           /* 346-427)                                      */
 
 typedef struct spt_ctx spt_ctx;
 
@@ -261,11 +267,12 @@ int spt_kill_mask(spt_ctx* ctx, const spt_geometry* geom, const double* pos, int
 int spt_image_upload(spt_ctx* ctx, const double* profile_rz, int32_t n_nodes, int32_t n_max, int32_t n_subdiv,
                      double d_lo, double d_hi, double* A_out);
 
-/* CylindricalWellBEMGeometry.image_force with z0 (reference
- * bem/geometry.py:786-801, bem/image_charge.py:816-1056): per group (CSR
+/* CylindricalWellBEMGeometry / HemisphericalTipBEMGeometry.image_force with
+ * z0 (reference bem/geometry.py:326-427,786-801, bem/image_charge.py:
+ * 816-1056): per group (CSR
  * offsets, 1..256 members) the joint image-charge BEM force, blended with
  * the plane image at z0 over [z_handoff/2, z_handoff]; device arrays, F
- * (n x 3) [N]. geom->image must be SPT_IMAGE_BEM. */
+ * (n x 3) [N]. geom->image must be SPT_IMAGE_BEM or SPT_IMAGE_BEM_MIRROR. */
 int spt_image_force(spt_ctx* ctx, const spt_geometry* geom, const double* pos, const double* charge,
                     const uint8_t* active, const int64_t* group_offsets, int32_t n_groups, double plummer_radius,
                     double* F);

@@ -120,9 +120,24 @@ def quadrature_geometry(profile, n_subdiv):
     return np.array(rq), np.array(zq), np.array(wq), np.array(sq), np.array(tq)
 
 
-def image_force(profile, A_inv, n_max, position, charge, active, d_lo, d_hi, n_subdiv=4):
+def image_force(profile, A_inv, n_max, position, charge, active, d_lo, d_hi, n_subdiv=4, mirror_plane_z=None):
     """Induced (image) force on every active particle, per group (G, E, 3),
-    from one joint solve per group (reference ``image_charge.py:816-1056``)."""
+    from one joint solve per group (reference ``image_charge.py:816-1056``).
+
+    mirror_plane_z: the profile is the image surface mirror-doubled through
+    that plane (reference ``bem/geometry.py:346-427``): a phantom source -q
+    at the reflection of every particle joins the group's solve, local
+    mirror charges are kept only where |z - mirror_plane_z| exceeds the
+    distance to the surface (``image_charge.py:620-681``), and the forces
+    are those on the real particles (the phantoms' direct pull, the plane
+    image, is the caller's to add)."""
+    E_eval = position.shape[1]
+    if mirror_plane_z is not None:
+        ph = position.copy()
+        ph[..., 2] = 2.0 * mirror_plane_z - position[..., 2]
+        position = np.concatenate([position, ph], axis=1)
+        charge = np.concatenate([charge, -charge], axis=1)
+        active = np.concatenate([active, active], axis=1)
     G, E, _ = position.shape
     fl = rho_floor(profile)
     q = np.where(active, charge, 0.0)
@@ -135,6 +150,8 @@ def image_force(profile, A_inv, n_max, position, charge, active, d_lo, d_hi, n_s
     tu = tan[bseg] / np.linalg.norm(tan[bseg], axis=-1, keepdims=True)
     dist = np.sqrt(bd2)
     w = image_weight(dist, d_lo, d_hi) if d_hi > d_lo else np.zeros_like(dist)
+    if mirror_plane_z is not None:
+        w = np.where(np.abs(z.reshape(-1) - mirror_plane_z) > dist, w, 0.0)
     ri, zi, side = mirror_points(rho.reshape(-1), z.reshape(-1), closest, tu, bn)
     phi_i = np.where(side > 0, phi.reshape(-1), phi.reshape(-1) + np.pi)
     q_i = -q.reshape(-1) * w
@@ -159,7 +176,8 @@ def image_force(profile, A_inv, n_max, position, charge, active, d_lo, d_hi, n_s
     sig = np.matmul(A_inv[: n_max + 1], rhs)
     sig = np.moveaxis(sig.reshape(n_max + 1, P, G, 2), 1, 2)
     sC, sS = sig[..., 0], sig[..., 1]
-    # field of the residual density at the particles
+    # field of the residual density at the particles evaluated (the real ones)
+    rho, z, phi, q = rho[:, :E_eval], z[:, :E_eval], phi[:, :E_eval], q[:, :E_eval]
     rq, zq, wq, sq, tq = quadrature_geometry(profile, n_subdiv)
     dc = (sC[..., sq] + tq * (sC[..., sq + 1] - sC[..., sq])) * wq  # (M, G, Q)
     ds = (sS[..., sq] + tq * (sS[..., sq + 1] - sS[..., sq])) * wq
@@ -181,4 +199,4 @@ def image_force(profile, A_inv, n_max, position, charge, active, d_lo, d_hi, n_s
     coef = np.where(r > 0.0, q_i[:, None, :] / (4.0 * np.pi * rs**3), 0.0)
     Emir = np.einsum("gnm,gnmc->gnc", coef, diff)
     F = q[..., None] * (Eres + Emir) / EPS0
-    return np.where(active[..., None], F, 0.0)
+    return np.where(active[:, :E_eval, None], F, 0.0)

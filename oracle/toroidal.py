@@ -46,26 +46,32 @@ def toroidal_Q(chi, n_max, pad_min=50, pad_const=40.0):
     eta = np.arccosh(c)
     den = (c - 1.0) * (c + 1.0)
     near = M * eta <= 1.0
-    for i in range(c.size):
-        ci = c[i]
-        out[0, i] = q0[i]
-        if near[i]:
-            out[1, i] = q1[i]
-            for n in range(1, M):
-                out[n + 1, i] = (2.0 * n * ci * out[n, i] - (n - 0.5) * out[n - 1, i]) / (n + 0.5)
-        else:
-            pad = max(pad_min, int(np.ceil(pad_const / eta[i])))
-            r_next = 0.0
-            r = np.empty(M + 1)
-            for n in range(M + pad, 0, -1):
-                rn = (n - 0.5) / (2.0 * n * ci - (n + 0.5) * r_next)
-                if n <= M:
-                    r[n] = rn
-                r_next = rn
-            v = q0[i]
-            for n in range(1, M + 1):
-                v = v * r[n]
-                out[n, i] = v
+    # upward recurrence, all near elements at once
+    out[0] = q0
+    iu = np.nonzero(near)[0]
+    if iu.size:
+        cu = c[iu]
+        out[1, iu] = q1[iu]
+        for n in range(1, M):
+            out[n + 1, iu] = (2.0 * n * cu * out[n, iu] - (n - 0.5) * out[n - 1, iu]) / (n + 0.5)
+    # downward ratio recurrence, each element from its own start M + pad (masked)
+    idn = np.nonzero(~near)[0]
+    if idn.size:
+        cd = c[idn]
+        pad = np.maximum(pad_min, np.ceil(pad_const / eta[idn]).astype(np.int64))
+        start = M + pad
+        r_next = np.zeros(idn.size)
+        r = np.empty((M + 1, idn.size))
+        for n in range(int(start.max()), 0, -1):
+            on = n <= start
+            rn = (n - 0.5) / (2.0 * n * cd - (n + 0.5) * r_next)
+            r_next = np.where(on, rn, r_next)
+            if n <= M:
+                r[n] = rn
+        v = q0[idn]
+        for n in range(1, M + 1):
+            v = v * r[n]
+            out[n, idn] = v
     dq[0] = dq0
     for n in range(1, M + 1):
         dq[n] = (n - 0.5) * (c * out[n] - out[n - 1]) / den

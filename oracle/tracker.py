@@ -9,7 +9,9 @@ Geometries are plain dicts:
   "bottom_fillet", "rim_fillet"}`` (cylindrical well, reference bem/geometry.py:510-812)
 * ``{"kind": "tip_bem" | "post_bem", "E_gun", "R", "H" (post only), "profile",
   "sigma", "n_subdiv", "z_handoff", "kill_z_below"}`` (image force off, the
-  reference BEM default ``z0=None``)
+  reference BEM default ``z0=None``); the tip with ``"z0"``, ``"image_profile"``,
+  ``"image_A_inv"``, ``"n_max"``, ``"d_lo"``, ``"d_hi"`` and
+  ``"image_mirror_symmetric"`` takes the image-charge BEM force
 """
 import numpy as np
 
@@ -51,13 +53,17 @@ def field(geom, pos):
 def image_force(geom, pos, q, active, a):
     k = geom["kind"]
     z0 = geom.get("z0")
-    if k == "well_bem" and z0 is not None:
-        # reference bem/geometry.py:786-801: image BEM near, plane image far, blended
+    if k in ("well_bem", "tip_bem") and z0 is not None:
+        # reference bem/geometry.py:326-344,786-801: image BEM near, plane image far, blended; the tip's
+        # mirror-doubled image surface adds the phantom charges' direct pull, the plane image (346-427)
         from . import image
 
+        mpz = -z0 if geom.get("image_mirror_symmetric") else None
         near = image.image_force(geom["image_profile"], geom["image_A_inv"], geom["n_max"], pos, q, active,
-                                 geom["d_lo"], geom["d_hi"])
+                                 geom["d_lo"], geom["d_hi"], mirror_plane_z=mpz)
         far = forces.flat_image(pos, q, active, z0, a)
+        if mpz is not None:
+            near = near + far
         from .fields import blend
         return blend(near, far, pos[..., 2], 0.5 * geom["z_handoff"], geom["z_handoff"])
     if z0 is None or k in ("tip_bem", "post_bem", "well_bem"):

@@ -197,6 +197,53 @@ def capture_well_image():
           f"{time.time() - t0:.1f} s", flush=True)
 
 
+def tip_image_batch(R, rng):
+    """(3, 4) particles around a hemisphere tip of radius R on its plane:
+    over the cap, at the belt, over the plane outside it, above the axis;
+    one inactive."""
+    th = np.array([[0.1, 0.6, 1.0, 1.35], [1.5, 1.55, 0.3, 0.0], [0.8, 1.2, 0.05, 0.45]])
+    d = np.array([[2e-9, 5e-9, 1.5e-9, 3e-9], [1e-9, 4e-9, 20e-9, 3e-9], [10e-9, 2.5e-9, 40e-9, 8e-9]])
+    r = R + d
+    pos = np.zeros((3, 4, 3))
+    ph = 2 * np.pi * rng.random((3, 4))
+    pos[..., 0] = r * np.sin(th) * np.cos(ph)
+    pos[..., 1] = r * np.sin(th) * np.sin(ph)
+    pos[..., 2] = r * np.cos(th)
+    pos[1, 0] = [1.3 * R, 0.0, 2e-9]   # over the plane outside the tip
+    pos[1, 1] = [0.0, 1.02 * R, 0.5e-9]  # at the belt
+    charge = np.full((3, 4), -1.602176634e-19)
+    active = np.ones((3, 4), dtype=bool)
+    active[2, 1] = False
+    return pos, charge, active
+
+
+def capture_tip_image():
+    """Reference HemisphericalTipBEMGeometry with z0 = 3 nm, truncated-plane
+    and mirror-doubled image surfaces: image profile, per-mode operators,
+    the bare image-BEM force (plus the phantoms' plane image in the doubled
+    form) and the blended geometry force of a particle batch."""
+    Ez, R, z0, n_max = -1e8, 50e-9, 3e-9, 8
+    pos, charge, active = tip_image_batch(R, np.random.default_rng(23))
+    out = dict(Ez=Ez, R=R, z0=z0, n_max=n_max, pos=pos, charge=charge, active=active)
+    for tag, mirror in (("plane", False), ("doubled", True)):
+        t0 = time.time()
+        geo = ref_bem_geometry.HemisphericalTipBEMGeometry(Ez, R, z0=z0, image_n_max=n_max,
+                                                           image_mirror_symmetric=mirror)
+        sol = geo.image_solution
+        if mirror:
+            near = geo._mirror_symmetric_image_force(pos, charge, active, 1e-12)
+        else:
+            near = sol.image_force(pos, charge, active, d_lo=geo.image_d_lo * R, d_hi=geo.image_d_hi * R,
+                                   n_max=n_max)
+        out.update({f"{tag}_image_profile": sol.profile, f"{tag}_A": sol.A, f"{tag}_near": near,
+                    f"{tag}_force": geo.image_force(pos, charge, active, 1e-12),
+                    f"{tag}_image_max_length": geo.image_max_length})
+        out.update(d_lo=geo.image_d_lo * R, d_hi=geo.image_d_hi * R, z_handoff=geo.z_handoff,
+                   real_profile=geo._field.solution.profile, sigma=geo._field.solution.sigma)
+        print(f"bem_tip_image {tag}: {len(sol.profile)} image nodes, {time.time() - t0:.1f} s", flush=True)
+    np.savez(os.path.join(HERE, "bem_tip_image.npz"), **out)
+
+
 def capture_post(n_seg, with_field):
     Ez, R, H = -1e8, 1e-6, 10e-6
     prof = post_profile(R, H, n_seg)
@@ -324,6 +371,14 @@ def main(which):
         geo = ref_bem_geometry.CylindricalWellBEMGeometry(-1e8, R, H, z0=3e-9, field_max_length=3e-9,
                                                           image_max_length=4e-9, image_n_max=8)
         traced_run("wellimg", src, 3, geo, [-25e-9, -15e-9], rtol=1e-8, atol=1e-12, t_max=8e-14)
+    if "tipimage" in which:
+        capture_tip_image()
+    if "tipimgrun" in which:
+        R = 50e-9
+        src = emission.cap_source(6, R, 0.0, seed=8)
+        geo = ref_bem_geometry.HemisphericalTipBEMGeometry(-1e8, R, z0=3e-9, image_n_max=8,
+                                                           image_mirror_symmetric=True)
+        traced_run("tipimg", src, 3, geo, [R + 20e-9, R + 60e-9], rtol=1e-8, atol=1e-12, t_max=8e-14)
     if "wellrun" in which:
         R, H = 50e-9, 30e-9
         src = emission.well_bottom_source([4, 4, 4], R, H, t_offset_max=10e-15, seed=6)
@@ -359,4 +414,4 @@ def main(which):
 
 if __name__ == "__main__":
     main(sys.argv[1:] or ["sphere", "post80", "c1", "c1small", "tipbem", "post", "post400", "well", "wellrun", "wellimage",
-                          "wellimgrun", "c1out"])
+                          "wellimgrun", "c1out", "tipimage", "tipimgrun"])

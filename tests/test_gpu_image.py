@@ -1,5 +1,7 @@
-"""Image-charge BEM on the well (reference bem/image_charge.py,
-bem/geometry.py:786-801) on the GPU: per-mode operators vs the
+"""Image-charge BEM on the well and on the hemispherical tip (reference
+bem/image_charge.py, bem/geometry.py:326-427,786-801; the tip both on its
+truncated-plane image surface and on the mirror-doubled one with phantom
+charges) on the GPU: per-mode operators vs the
 reference's (1e-12 of each mode's largest entry; the adaptive rule and
 leaf order are the reference's, K/E and Q differ by ulps), the blended
 image force of a particle batch vs the reference (1e-10 of the largest
@@ -83,6 +85,60 @@ def test_image_trace_matches_reference_run(gpu):
     out = tr.last_run
     G = tr.n_groups
     _say(f"image trace: attempts {out['attempts'].tolist()} vs {r['attempts'].tolist()}")
+    np.testing.assert_array_equal(out["attempts"], r["attempts"])
+    np.testing.assert_array_equal(out["accepted"], r["accepted"])
+    pos = out["pos"].reshape(G, -1, 3)
+    vel = out["vel"].reshape(G, -1, 3)
+    np.testing.assert_allclose(pos, r["pos"], rtol=1e-9, atol=1e-9 * np.abs(r["pos"]).max())
+    np.testing.assert_allclose(vel, r["vel"], rtol=1e-9, atol=1e-9 * np.abs(r["vel"]).max())
+    for si, s in enumerate(screens):
+        np.testing.assert_array_equal(s.id, r[f"s{si}_id"])
+        np.testing.assert_allclose(s.t, r[f"s{si}_t"], rtol=1e-11)
+
+
+def _tip_geo(tag):
+    from bem_amd.bem.geometry import HemisphericalTipBEMGeometry
+
+    d = golden("bem_tip_image.npz")
+    geo = HemisphericalTipBEMGeometry(float(d["Ez"]), float(d["R"]), z0=float(d["z0"]), image_n_max=int(d["n_max"]),
+                                      image_mirror_symmetric=tag == "doubled", sigma=d["sigma"])
+    return d, geo
+
+
+@pytest.mark.parametrize("tag", ["plane", "doubled"])
+def test_tip_image_operators_and_force_match_reference(gpu, tag):
+    _say(f"tip image [{tag}]")
+    d, geo = _tip_geo(tag)
+    np.testing.assert_array_equal(geo.image_profile, d[f"{tag}_image_profile"])
+    np.testing.assert_array_equal(geo.solution.profile, d["real_profile"])
+    A = geo.image_operators()
+    ref = d[f"{tag}_A"]
+    err = np.max(np.abs(A - ref) / np.abs(ref).max(axis=(1, 2), keepdims=True))
+    F = geo.image_force(d["pos"], d["charge"], d["active"], 1e-12)
+    fref = d[f"{tag}_force"]
+    ferr = np.abs(F - fref).max() / np.abs(fref).max()
+    _say(f"tip image [{tag}]: operators max rel {err:.2e}, force max rel {ferr:.2e}")
+    assert err < 1e-12
+    assert ferr < 1e-10
+    assert np.all(F[~d["active"]] == 0.0)
+
+
+def test_tip_mirror_image_trace_matches_reference_run(gpu):
+    from bem_amd import SpecificParticleTracer
+    from bem_amd.particles import ParticleGroup
+
+    _say("tip mirror-image trace")
+    d, geo = _tip_geo("doubled")
+    r = golden("run_tipimg.npz")
+    src = {k[3:]: r[k] for k in r.files if k.startswith("in_")}
+    n = len(src["x"])
+    pg = ParticleGroup(dict(src, weight=np.full(n, 1.602176634e-19), status=np.ones(n), species="electron"))
+    tr = SpecificParticleTracer(pg, int(r["n_emit"]), geo, list(r["screens"]), rtol=float(r["rtol"]),
+                                atol=float(r["atol"]), t_max=float(r["t_max"]))
+    screens, _ = tr.run()
+    out = tr.last_run
+    G = tr.n_groups
+    _say(f"tip mirror-image trace: attempts {out['attempts'].tolist()} vs {r['attempts'].tolist()}")
     np.testing.assert_array_equal(out["attempts"], r["attempts"])
     np.testing.assert_array_equal(out["accepted"], r["accepted"])
     pos = out["pos"].reshape(G, -1, 3)

@@ -198,6 +198,12 @@ def _geom_for(name):
         w = golden("bem_well.npz")
         return dict(kind="well_bem", E_gun=float(w["Ez"]), R=float(w["R"]), H=float(w["H"]),
                     profile=w["sharp_profile"], sigma=w["sharp_sigma"], z_handoff=float(w["sharp_z_handoff"]))
+    if name == "tipimg":
+        t = golden("bem_tip_image.npz")
+        return dict(kind="tip_bem", E_gun=float(t["Ez"]), R=float(t["R"]), profile=t["real_profile"],
+                    sigma=t["sigma"], kill_z_below=0.0, z_handoff=float(t["z_handoff"]), z0=float(t["z0"]),
+                    image_profile=t["doubled_image_profile"], image_A_inv=np.linalg.inv(t["doubled_A"]),
+                    n_max=int(t["n_max"]), d_lo=float(t["d_lo"]), d_hi=float(t["d_hi"]), image_mirror_symmetric=True)
     if name == "wellimg":
         w = golden("bem_well_image.npz")
         return dict(kind="well_bem", E_gun=float(w["Ez"]), R=float(w["R"]), H=float(w["H"]),
@@ -236,7 +242,7 @@ def test_oracle_output_times_match_reference_run():
             np.testing.assert_allclose([r["pos"][2] for r in rec], d[f"o{k}_z"], rtol=1e-9, atol=1e-15)
 
 
-@pytest.mark.parametrize("name", ["c1small", "tipbem", "post80", "well", "wellimg"])
+@pytest.mark.parametrize("name", ["c1small", "tipbem", "post80", "well", "wellimg", "tipimg"])
 def test_oracle_integrator_matches_reference_run(name):
     d, st, out = run_oracle(name)
     np.testing.assert_array_equal(out["attempts"], d["attempts"])
