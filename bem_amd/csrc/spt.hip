@@ -2151,19 +2151,33 @@ static PosView stage_pos_view(const LockParams& L) {
   return X;
 }
 
-// The three image-charge kernels for every (running) group; the scratch
-// must have been sized by ensure_image_scratch.
-static void launch_image_force(spt_ctx* c, const spt_geometry& G, PosView X, const double* charge,
-                               const uint8_t* active, const int64_t* offsets, const int32_t* gflags, int running,
-                               int n_groups, double a2, double* F, hipStream_t s) {
+// The image-charge force for every (running) group: the right-hand sides,
+// the per-mode joint solves sigma_m = A_m^-1 rhs_m of all groups as one
+// strided-batched FP64 GEMM over the modes (A_m^-1 is read once per call
+// instead of once per group; rows = profile nodes, columns = (group,
+// cos/sin)), then the force. The scratch must have been sized by
+// ensure_image_scratch. rocBLAS runs eagerly on s (never inside a captured
+// graph). Columns of groups that are not running hold stale right-hand
+// sides; their sigma is never read.
+static int launch_image_force(spt_ctx* c, const spt_geometry& G, PosView X, const double* charge,
+                              const uint8_t* active, const int64_t* offsets, const int32_t* gflags, int running,
+                              int n_groups, double a2, double* F, hipStream_t s) {
   ImageView B = image_view(c);
   if (G.image == SPT_IMAGE_BEM_MIRROR) {
     B.mirror_mode = 1;
     B.plane_z = -G.z0;
   }
   img_rhs_kernel<<<n_groups, 256, 0, s>>>(B, X, charge, active, offsets, gflags, running);
-  img_solve_kernel<<<dim3(n_groups, 2 * B.n_modes), 128, 0, s>>>(B, gflags, running);
+  if (rocblas_set_stream(c->blas, s) != rocblas_status_success) return fail(c, SPT_EHIP, "rocblas_set_stream");
+  const double one = 1.0, zero = 0.0;
+  const rocblas_int P = B.P, M = B.n_modes;
+  const rocblas_status st = rocblas_dgemm_strided_batched(
+      c->blas, rocblas_operation_none, rocblas_operation_none, P, 2 * n_groups, P, &one, B.Ainv, P,
+      (rocblas_stride)P * P, B.rhs, M * P, P, &zero, B.sigma, M * P, P, M);
+  if (st != rocblas_status_success) return fail(c, SPT_EHIP, "image solve (rocblas_dgemm_strided_batched)");
   img_force_kernel<<<n_groups, 256, 0, s>>>(B, G, X, charge, active, offsets, gflags, running, a2, F);
+  HIPCHK(c, hipGetLastError());
+  return SPT_OK;
 }
 
 constexpr int kLockstepUnroll = 4;  // attempts per captured graph replay
@@ -2183,33 +2197,54 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
   if (L.T.n_out > 0) reset_u64_kernel<<<1, 1, 0, s>>>(L.T.snap_count);
   ls_init_kernel<TG><<<n_groups, TG, 0, s>>>(L);
   HIPCHK(c, hipGetLastError());
+  // kLockstepUnroll attempts of every running group. Captured once in a HIP
+  // graph and replayed; with the image-charge BEM (whose solve is a rocBLAS
+  // GEMM, kept out of graphs) the same sequence is issued eagerly.
+  auto enqueue = [&]() -> int {
+    for (int u = 0; u < kLockstepUnroll; ++u) {
+      for (int st = 0; st < 7; ++st) {
+        if (L.sur) {
+          if (FPR == SPT_PREC_FP64)
+            ls_surrogate_kernel<true><<<(unsigned)((n + MLP64_PTS - 1) / MLP64_PTS), 64, 0, s>>>(L);
+          else
+            ls_surrogate_kernel<false><<<(unsigned)((n + MLP_WAVE_PTS - 1) / MLP_WAVE_PTS), 64, 0, s>>>(L);
+        }
+        if (fs.T == 192)
+          ls_field_kernel<192, 64, FPR><<<nb_field, 192, 0, s>>>(L, PB);
+        else
+          ls_field_kernel<256, 64, FPR><<<nb_field, 256, 0, s>>>(L, PB);
+        if (L.imgF) {
+          const int r = launch_image_force(c, L.T.G, stage_pos_view(L), L.T.charge, L.active, L.T.offsets,
+                                           L.g_flags, LS_RUNNING, n_groups, L.T.a2, (double*)L.imgF, s);
+          if (r) return r;
+        }
+        ls_accel_kernel<TG, APR><<<n_groups, TG, 0, s>>>(L, st);
+      }
+      ls_end_kernel<TG><<<n_groups, TG, 0, s>>>(L);
+    }
+    return SPT_OK;
+  };
+  const bool eager = L.imgF != nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  for (int u = 0; u < kLockstepUnroll; ++u) {
-    for (int st = 0; st < 7; ++st) {
-      if (L.sur) {
-        if (FPR == SPT_PREC_FP64)
-          ls_surrogate_kernel<true><<<(unsigned)((n + MLP64_PTS - 1) / MLP64_PTS), 64, 0, s>>>(L);
-        else
-          ls_surrogate_kernel<false><<<(unsigned)((n + MLP_WAVE_PTS - 1) / MLP_WAVE_PTS), 64, 0, s>>>(L);
-      }
-      if (fs.T == 192)
-        ls_field_kernel<192, 64, FPR><<<nb_field, 192, 0, s>>>(L, PB);
-      else
-        ls_field_kernel<256, 64, FPR><<<nb_field, 256, 0, s>>>(L, PB);
-      if (L.imgF) launch_image_force(c, L.T.G, stage_pos_view(L), L.T.charge, L.active, L.T.offsets, L.g_flags,
-                                     LS_RUNNING, n_groups, L.T.a2, (double*)L.imgF, s);
-      ls_accel_kernel<TG, APR><<<n_groups, TG, 0, s>>>(L, st);
-    }
-    ls_end_kernel<TG><<<n_groups, TG, 0, s>>>(L);
+  if (!eager) {
+    HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rq = enqueue();
+    HIPCHK(c, hipStreamEndCapture(s, &graph));
+    if (rq) return rq;
+    HIPCHK(c, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   }
-  HIPCHK(c, hipStreamEndCapture(s, &graph));
-  HIPCHK(c, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   int64_t iters = 0;
   int rc = SPT_OK;
   while (true) {
-    hipError_t e = hipGraphLaunch(exec, s);
+    hipError_t e = hipSuccess;
+    if (eager) {
+      rc = enqueue();
+      if (rc) break;
+      e = hipGetLastError();
+    } else {
+      e = hipGraphLaunch(exec, s);
+    }
     if (e == hipSuccess) e = hipMemcpyAsync(c->h_done, L.done, sizeof(unsigned int), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
@@ -2223,8 +2258,8 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
       break;
     }
   }
-  (void)hipGraphExecDestroy(exec);
-  (void)hipGraphDestroy(graph);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
   return rc;
 }
 
@@ -2837,10 +2872,8 @@ int spt_image_force(spt_ctx* c, const spt_geometry* g, const double* pos, const 
   X.y = pos + 1;
   X.z = pos + 2;
   X.stride = 3;
-  launch_image_force(c, *g, X, charge, active, offsets, nullptr, 0, n_groups, plummer_radius * plummer_radius, F,
-                     c->stream);
-  HIPCHK(c, hipGetLastError());
-  return SPT_OK;
+  return launch_image_force(c, *g, X, charge, active, offsets, nullptr, 0, n_groups, plummer_radius * plummer_radius,
+                            F, c->stream);
 }
 
 }  // extern "C"
