@@ -2167,7 +2167,13 @@ static int launch_image_force(spt_ctx* c, const spt_geometry& G, PosView X, cons
     B.mirror_mode = 1;
     B.plane_z = -G.z0;
   }
-  img_rhs_kernel<<<n_groups, 256, 0, s>>>(B, X, charge, active, offsets, gflags, running);
+  const int nm = B.n_modes;
+  if (nm <= 9)
+    img_rhs_kernel<9><<<n_groups, 256, 0, s>>>(B, X, charge, active, offsets, gflags, running);
+  else if (nm <= 17)
+    img_rhs_kernel<17><<<n_groups, 256, 0, s>>>(B, X, charge, active, offsets, gflags, running);
+  else
+    img_rhs_kernel<33><<<n_groups, 256, 0, s>>>(B, X, charge, active, offsets, gflags, running);
   if (rocblas_set_stream(c->blas, s) != rocblas_status_success) return fail(c, SPT_EHIP, "rocblas_set_stream");
   const double one = 1.0, zero = 0.0;
   const rocblas_int P = B.P, M = B.n_modes;
@@ -2175,7 +2181,12 @@ static int launch_image_force(spt_ctx* c, const spt_geometry& G, PosView X, cons
       c->blas, rocblas_operation_none, rocblas_operation_none, P, 2 * n_groups, P, &one, B.Ainv, P,
       (rocblas_stride)P * P, B.rhs, M * P, P, &zero, B.sigma, M * P, P, M);
   if (st != rocblas_status_success) return fail(c, SPT_EHIP, "image solve (rocblas_dgemm_strided_batched)");
-  img_force_kernel<<<n_groups, 256, 0, s>>>(B, G, X, charge, active, offsets, gflags, running, a2, F);
+  if (nm <= 9)
+    img_force_kernel<9><<<n_groups, 256, 0, s>>>(B, G, X, charge, active, offsets, gflags, running, a2, F);
+  else if (nm <= 17)
+    img_force_kernel<17><<<n_groups, 256, 0, s>>>(B, G, X, charge, active, offsets, gflags, running, a2, F);
+  else
+    img_force_kernel<33><<<n_groups, 256, 0, s>>>(B, G, X, charge, active, offsets, gflags, running, a2, F);
   HIPCHK(c, hipGetLastError());
   return SPT_OK;
 }
