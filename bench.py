@@ -204,6 +204,39 @@ def c2_plain_rate(steps=3):
                 attempts_mean=float(outs[-1]["attempts"].mean()), ms_per_trace=wall / steps * 1e3)
 
 
+def c3_sample_rate(attempts=8):
+    """C3's shape on a capped number of attempts: cylindrical well R 50 nm,
+    H 30 nm on a 2000-panel real profile, image-charge BEM (z0 3 nm, 16
+    modes), 4096 groups of U{1..256} electrons (530,287) born at per-group
+    offsets U[0, 10 ps], rtol 1e-8, lockstep, FP64; every group makes
+    `attempts` attempts (a full run is ~100x longer)."""
+    import torch
+
+    from bem_amd import SpecificParticleTracer
+    from bem_amd.bem.geometry import CylindricalWellBEMGeometry
+    from bem_amd.emission import well_bottom_source
+    from bem_amd.particles import ParticleGroup
+
+    R, H = 50e-9, 30e-9
+    geo = CylindricalWellBEMGeometry(-1e8, R, H, z0=3e-9, field_max_length=0.04223e-9)
+    sizes = np.random.default_rng(0).integers(1, 257, 4096)
+    src = well_bottom_source(sizes, R, H, t_offset_max=10e-12, seed=0)
+    n = len(src["x"])
+    pg = ParticleGroup(dict(src, weight=np.full(n, 1.602176634e-19), status=np.ones(n), species="electron"))
+    tr = SpecificParticleTracer(pg, 1, geo, [20e-9, 100e-9], rtol=1e-8, atol=1e-12, group_sizes=sizes, z_max=1e-6,
+                                max_steps=attempts)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = tr.trace()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return dict(value=out["particle_stages"] / wall, unit="particle-RK-stages/s",
+                workload=f"well R 50 nm H 30 nm, {len(geo.solution.profile) - 1} panels, image BEM z0 3 nm on "
+                         f"{len(geo.image_profile)} nodes x 17 modes, 4096 groups U{{1..256}} ({n} e-), rtol 1e-8, "
+                         f"first {attempts} attempts of every group",
+                wall_s=wall)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,6 +247,7 @@ def main():
     ap.add_argument("--train-steps", type=int, default=60000)
     ap.add_argument("--no-sweep", action="store_true", help="skip the looser-tolerance runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="skip the C3-shaped sample (well with image charges)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--field-only", action="store_true",
                     help="only the field-gather timing and a 256 MiB device copy (for hardware-counter runs)")
@@ -329,6 +363,9 @@ def main():
                      "note": "FP64 VALU-bound; MI355X FP64 vector peak = FP64 MFMA peak = 78.6 TFLOP/s"},
         "c2_plain_fp64": c2_plain_rate(),
     }
+    if not args.no_c3:
+        say("bench: C3-shaped sample (well, image-charge BEM, 4096 uneven groups, 8 attempts)")
+        res["c3_sample"] = c3_sample_rate()
     if cpu is not None:
         res["cpu_baseline"] = cpu
     print(json.dumps(res))
