@@ -1090,8 +1090,18 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void geometry_field_kernel(
 // kernel that also prepares the next attempt. Group scalars and particle
 // stage data live in global scratch between kernels.
 //
-// Extra per-particle slots: 48..50 stage position, 51..53 field there.
-constexpr int kSlotsLockstep = 54;
+// Extra per-particle slots: 48..50 stage position, 51..53 field there,
+// 54..56 the external field at the particle's current state, 57..59 the
+// field at this attempt's end point (stage 7).
+//
+// Stage-1 field reuse: the first stage of an attempt sits at the state,
+// where the field is already known (after an accepted step the state is
+// the previous attempt's stage-7 point; after a rejected one, the previous
+// attempt's stage-1 point) and the external field depends on the position
+// only, so it is copied instead of gathered again. Pair and image forces,
+// which depend on the frozen active set, are recomputed at every stage.
+// The work count stays 7 stages per attempt.
+constexpr int kSlotsLockstep = 60;
 constexpr int LS_RUNNING = 1, LS_EXHAUSTED = 2;
 
 struct LockParams {
@@ -1106,6 +1116,7 @@ struct LockParams {
   int32_t* g_flags;
   unsigned int* done;      // finished groups
   uint8_t* sur;            // per particle: surrogate flag of this stage
+  uint8_t* fvalid;         // per particle: slots 54..56 hold the field at the current state
   MlpView V;               // the surrogate (when sur != nullptr): split BF16
   Mlp64View V64;           // and FP64 forms
   const double* imgF;      // image-charge BEM force per particle (n x 3), or nullptr
@@ -1170,7 +1181,10 @@ __global__ __launch_bounds__(TG) void ls_init_kernel(LockParams L) {
   const int n = (int)(L.T.offsets[g + 1] - off);
   const bool mine = tid < n;
   const int64_t gi = off + tid;
-  if (mine) L.T.t_death[gi] = INFINITY;
+  if (mine) {
+    L.T.t_death[gi] = INFINITY;
+    L.fvalid[gi] = 0;
+  }
   const double t = block_min<TG>(mine ? L.T.t_birth[gi] : INFINITY, red);
   const double h = (L.T.t_max - t) * L.T.init_frac;
   if (mine) snapshot_at_start(L.T, g, tid, gi, t, L.T.t_birth[gi]);
@@ -1187,10 +1201,21 @@ __global__ __launch_bounds__(TG) void ls_init_kernel(LockParams L) {
   ls_begin<TG>(L, g, n, t, h, 0, red);
 }
 
+// Keep a freshly evaluated stage field (slots 51..53): stage 1's as the
+// field at the state, stage 7's as the field at the attempt's end point.
+__device__ __forceinline__ void ls_keep_field(const LockParams& L, int64_t gi, int st) {
+  if (st == 0) {
+    for (int c = 0; c < 3; ++c) LKB(54 + c) = LKB(51 + c);
+    L.fvalid[gi] = 1;
+  } else if (st == 6) {
+    for (int c = 0; c < 3; ++c) LKB(57 + c) = LKB(51 + c);
+  }
+}
+
 // External field at the stage positions of every active particle of every
 // running group; PB particles per block, T/PB threads per particle.
 template <int T, int NMAX, int PR>
-__global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_field_kernel(LockParams L, int PB) {
+__global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_field_kernel(LockParams L, int PB, int st) {
   __shared__ FieldShared<T, NMAX> sh;
   __shared__ double sx[NMAX], sy[NMAX], sz[NMAX];
   __shared__ int act[NMAX];
@@ -1205,7 +1230,13 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_field_kernel(LockPa
     sx[tid] = LKB(48);
     sy[tid] = LKB(49);
     sz[tid] = LKB(50);
-    if (L.sur && L.sur[gi] == 1) act[tid] = 0;  // written by ls_surrogate_kernel
+    if (act[tid] && st == 0 && L.fvalid[gi]) {  // the field at the state is known
+      for (int c = 0; c < 3; ++c) LKB(51 + c) = LKB(54 + c);
+      act[tid] = 0;
+    } else if (act[tid] && L.sur && L.sur[gi] == 1) {  // written by ls_surrogate_kernel
+      ls_keep_field(L, gi, st);
+      act[tid] = 0;
+    }
     sh.need[tid] = act[tid] && is_bem(G) && bem_needed(G, sx[tid], sy[tid], sz[tid]);
   }
   __syncthreads();
@@ -1240,6 +1271,7 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_field_kernel(LockPa
     LKB(51) = ex;
     LKB(52) = ey;
     LKB(53) = ez;
+    ls_keep_field(L, gi, st);
   }
   if (tid < m_all && act[tid] && !sh.need[tid]) {
     const int64_t gi = base + tid;
@@ -1248,6 +1280,7 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_field_kernel(LockPa
     LKB(51) = ex;
     LKB(52) = ey;
     LKB(53) = ez;
+    ls_keep_field(L, gi, st);
   }
 }
 
@@ -1259,7 +1292,7 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_field_kernel(LockPa
 // perturbation is parked in the field slots and the BEM one is to be
 // blended in, (1 - w) BEM + w network; 0 = BEM only.
 template <bool F64>
-__global__ __launch_bounds__(64) void ls_surrogate_kernel(LockParams L) {
+__global__ __launch_bounds__(64) void ls_surrogate_kernel(LockParams L, int st) {
   constexpr int PTS = F64 ? MLP64_PTS : MLP_WAVE_PTS;
   __shared__ union {
     Mlp64WaveShared d;
@@ -1283,7 +1316,8 @@ __global__ __launch_bounds__(64) void ls_surrogate_kernel(LockParams L) {
     x = LKB(48);
     y = LKB(49);
     z = LKB(50);
-    if ((L.g_flags[g] & LS_RUNNING) && L.active[gi] && bem_needed(G, x, y, z))
+    const bool reuse = st == 0 && L.fvalid[gi];  // ls_field_kernel copies the state's field
+    if (!reuse && (L.g_flags[g] & LS_RUNNING) && L.active[gi] && bem_needed(G, x, y, z))
       w = surrogate_weight(L.V.cfg, G, x, y, z);
     elig = w >= 0.0;
     L.sur[gi] = !elig ? 0 : (w >= 1.0 ? 1 : 2);
@@ -1444,6 +1478,10 @@ __global__ __launch_bounds__(TG) void ls_end_kernel(LockParams L) {
         v1[c] = LKB(45 + c);
         L.T.pos[3 * gi + c] = x1[c];
         L.T.vel[3 * gi + c] = v1[c];
+      }
+      if (L.active[gi]) {  // stage 7 sat at the new state
+        for (int c = 0; c < 3; ++c) LKB(54 + c) = LKB(57 + c);
+        L.fvalid[gi] = 1;
       }
       if (L.active[gi] && kill_point(L.T.G, x1[0], x1[1], x1[2])) L.T.t_death[gi] = t;
       screens_particle(L.T.SP, L.T.events, L.T.ev_cap, L.T.ev_count, g, tid, batch, t_old, t, x0, v0, x1, v1);
@@ -1671,6 +1709,7 @@ struct spt_ctx {
   unsigned int* d_done = nullptr;
   unsigned int* h_done = nullptr;
   uint8_t* d_sur = nullptr;
+  uint8_t* d_fvalid = nullptr;
   // surrogate used by spt_trace
   bool have_sur = false;
   spt_mlp_cfg sur_cfg;
@@ -1781,6 +1820,7 @@ void spt_destroy(spt_ctx* c) {
   if (c->d_done) (void)hipFree(c->d_done);
   if (c->h_done) (void)hipHostFree(c->h_done);
   if (c->d_sur) (void)hipFree(c->d_sur);
+  if (c->d_fvalid) (void)hipFree(c->d_fvalid);
   if (c->d_sur_p) (void)hipFree(c->d_sur_p);
   if (c->d_sur_split) (void)hipFree(c->d_sur_split);
   if (c->d_sur_frag) (void)hipFree(c->d_sur_frag);
@@ -2216,14 +2256,14 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
       for (int st = 0; st < 7; ++st) {
         if (L.sur) {
           if (FPR == SPT_PREC_FP64)
-            ls_surrogate_kernel<true><<<(unsigned)((n + MLP64_PTS - 1) / MLP64_PTS), 64, 0, s>>>(L);
+            ls_surrogate_kernel<true><<<(unsigned)((n + MLP64_PTS - 1) / MLP64_PTS), 64, 0, s>>>(L, st);
           else
-            ls_surrogate_kernel<false><<<(unsigned)((n + MLP_WAVE_PTS - 1) / MLP_WAVE_PTS), 64, 0, s>>>(L);
+            ls_surrogate_kernel<false><<<(unsigned)((n + MLP_WAVE_PTS - 1) / MLP_WAVE_PTS), 64, 0, s>>>(L, st);
         }
         if (fs.T == 192)
-          ls_field_kernel<192, 64, FPR><<<nb_field, 192, 0, s>>>(L, PB);
+          ls_field_kernel<192, 64, FPR><<<nb_field, 192, 0, s>>>(L, PB, st);
         else
-          ls_field_kernel<256, 64, FPR><<<nb_field, 256, 0, s>>>(L, PB);
+          ls_field_kernel<256, 64, FPR><<<nb_field, 256, 0, s>>>(L, PB, st);
         if (L.imgF) {
           const int r = launch_image_force(c, L.T.G, stage_pos_view(L), L.T.charge, L.active, L.T.offsets,
                                            L.g_flags, LS_RUNNING, n_groups, L.T.a2, (double*)L.imgF, s);
@@ -2286,13 +2326,16 @@ static int ensure_lockstep_scratch(spt_ctx* c, int64_t n, int n_groups) {
     if (c->d_pgroup) (void)hipFree(c->d_pgroup);
     if (c->d_active) (void)hipFree(c->d_active);
     if (c->d_sur) (void)hipFree(c->d_sur);
+    if (c->d_fvalid) (void)hipFree(c->d_fvalid);
     c->d_pgroup = nullptr;
     c->d_active = nullptr;
     c->d_sur = nullptr;
+    c->d_fvalid = nullptr;
     c->part_cap = 0;
     HIPCHK(c, hipMalloc(&c->d_pgroup, sizeof(int32_t) * n));
     HIPCHK(c, hipMalloc(&c->d_active, n));
     HIPCHK(c, hipMalloc(&c->d_sur, n));
+    HIPCHK(c, hipMalloc(&c->d_fvalid, n));
     c->part_cap = n;
   }
   if ((size_t)n_groups > c->group_cap) {
@@ -2426,6 +2469,7 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
     L.g_flags = c->d_gflags;
     L.done = c->d_done;
     L.sur = a->use_surrogate ? c->d_sur : nullptr;
+    L.fvalid = c->d_fvalid;
     memset(&L.V, 0, sizeof(L.V));
     memset(&L.V64, 0, sizeof(L.V64));
     L.imgF = nullptr;
