@@ -762,6 +762,8 @@ template <int T, int NMAX>
 struct TraceShared {
   double sx[NMAX], sy[NMAX], sz[NMAX], sq[NMAX];
   double fx[NMAX], fy[NMAX], fz[NMAX];
+  int evalid[NMAX];  // scratch slots 48..50 hold the external field at the state
+  int scomp[NMAX];  // field to be evaluated at this stage
   double cx[NMAX], cy[NMAX], cz[NMAX];
   double ox[NMAX], oy[NMAX], oz[NMAX];
   int sact[NMAX];
@@ -774,16 +776,31 @@ struct TraceShared {
 };
 
 // External field of every active particle of the group at the stage
-// positions in LDS (sx, sy, sz) into (fx, fy, fz).
+// positions in LDS (sx, sy, sz) into (fx, fy, fz). Stage 1 sits at the
+// state, whose field is known after the first attempt (the previous
+// attempt's stage-7 field after an accepted step, its stage-1 field after
+// a rejected one): copied instead of evaluated. The field depends on the
+// position only; the work count stays 7 stages per attempt.
 template <int T, int NMAX, int PR>
-__device__ void stage_field(const TraceParams& P, int n, TraceShared<T, NMAX>& sh) {
+__device__ void stage_field(const TraceParams& P, int64_t off, int n, int st, TraceShared<T, NMAX>& sh) {
   const int tid = threadIdx.x;
+  if (tid < n) {
+    const bool known = sh.sact[tid] && st == 0 && sh.evalid[tid];
+    sh.scomp[tid] = sh.sact[tid] && !known;
+    if (known) {
+      const size_t nt = (size_t)P.n_total, gi = (size_t)(off + tid);
+      sh.fx[tid] = P.kbuf[48 * nt + gi];
+      sh.fy[tid] = P.kbuf[49 * nt + gi];
+      sh.fz[tid] = P.kbuf[50 * nt + gi];
+    }
+  }
   if (!is_bem(P.G)) {
-    if (tid < n && sh.sact[tid])
+    if (tid < n && sh.scomp[tid])
       geometry_field_point(P.G, sh.sx[tid], sh.sy[tid], sh.sz[tid], 0, 0, 0, sh.fx[tid], sh.fy[tid], sh.fz[tid]);
+    __syncthreads();
     return;
   }
-  if (tid < n) sh.need[tid] = sh.sact[tid] && bem_needed(P.G, sh.sx[tid], sh.sy[tid], sh.sz[tid]);
+  if (tid < n) sh.need[tid] = sh.scomp[tid] && bem_needed(P.G, sh.sx[tid], sh.sy[tid], sh.sz[tid]);
   __syncthreads();
   if (tid == 0) {
     int m = 0;
@@ -806,7 +823,7 @@ __device__ void stage_field(const TraceParams& P, int n, TraceShared<T, NMAX>& s
     geometry_field_point(P.G, sh.sx[i], sh.sy[i], sh.sz[i], sh.ox[tid], sh.oy[tid], sh.oz[tid], sh.fx[i], sh.fy[i],
                          sh.fz[i]);
   }
-  if (tid < n && sh.sact[tid] && !sh.need[tid])
+  if (tid < n && sh.scomp[tid] && !sh.need[tid])
     geometry_field_point(P.G, sh.sx[tid], sh.sy[tid], sh.sz[tid], 0, 0, 0, sh.fx[tid], sh.fy[tid], sh.fz[tid]);
   __syncthreads();
 }
@@ -814,8 +831,9 @@ __device__ void stage_field(const TraceParams& P, int n, TraceShared<T, NMAX>& s
 // Per-particle scratch slots (coalesced across a group's threads):
 // 0..41  k of stage st, component c (0-2 position, 3-5 velocity): st*6+c
 // 42..47 candidate end-of-step position and velocity
+// 48..50 external field at the state; 51..53 at the candidate end point
 #define KB(sl) P.kbuf[(size_t)(sl) * (size_t)P.n_total + (size_t)gi]
-constexpr int kSlots = 48;
+constexpr int kSlots = 54;
 
 // Integrate one group from its earliest birth to t_max (reference
 // batched_rk45.py:149-323 for a single group; _attempt_step 326-401).
@@ -833,6 +851,7 @@ __device__ void run_group(const TraceParams& P, int g, TraceShared<T, NMAX>& sh)
   if (mine) {
     tb = P.t_birth[gi];
     sh.sq[tid] = P.charge[gi];
+    sh.evalid[tid] = 0;
   }
   double t = block_min<T>(mine ? tb : INFINITY, sh.red);
   double h = (P.t_max - t) * P.init_frac;
@@ -889,7 +908,19 @@ __device__ void run_group(const TraceParams& P, int g, TraceShared<T, NMAX>& sh)
         for (int c = 0; c < 3; ++c) KB(st * 6 + c) = active ? vs[c] : 0.0;
       }
       __syncthreads();
-      stage_field<T, NMAX, PR>(P, n, sh);
+      stage_field<T, NMAX, PR>(P, off, n, st, sh);
+      if (mine && active) {  // keep stage 1's field (the state's) and stage 7's (the end point's)
+        if (st == 0) {
+          KB(48) = sh.fx[tid];
+          KB(49) = sh.fy[tid];
+          KB(50) = sh.fz[tid];
+          sh.evalid[tid] = 1;
+        } else if (st == 6) {
+          KB(51) = sh.fx[tid];
+          KB(52) = sh.fy[tid];
+          KB(53) = sh.fz[tid];
+        }
+      }
       if (mine) {
         double Fc[3], Fi[3];
         pair_forces_p<PR>(P.G, tid, n, sh.sx, sh.sy, sh.sz, sh.sq, sh.sact, P.a2, Fc, Fi);
@@ -956,6 +987,10 @@ __device__ void run_group(const TraceParams& P, int g, TraceShared<T, NMAX>& sh)
           v1[c] = KB(45 + c);
           P.pos[3 * gi + c] = x1[c];
           P.vel[3 * gi + c] = v1[c];
+        }
+        if (active) {  // stage 7 sat at the new state
+          for (int c = 0; c < 3; ++c) KB(48 + c) = KB(51 + c);
+          sh.evalid[tid] = 1;
         }
         if (active && kill_point(P.G, x1[0], x1[1], x1[2])) td = t;
         screens_particle(P.SP, P.events, P.ev_cap, P.ev_count, g, tid, batch, t_old, t, x0, v0, x1, v1);
