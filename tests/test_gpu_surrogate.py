@@ -289,3 +289,40 @@ def test_surrogate_gain_in_a_trace(gpu):
     assert dt.max() < 1e-3
     # measured 1.18-1.36x as the BEM gather got faster; a real gain is the claim
     assert gain > 1.05
+
+
+def test_surrogate_on_the_hemispherical_tip(gpu):
+    """The same surrogate on the reference's own BEM tip geometry (sphere of
+    radius R on the plane, cap centre at the origin): trained briefly,
+    judged against the BEM (finite band at a loose 5e-2 tolerance), and a
+    lockstep trace of 4096 electrons with it crosses the screen at the
+    plain run's times to 1e-3 relative."""
+    from bem_amd import SpecificParticleTracer, surrogate
+    from bem_amd.bem.geometry import HemisphericalTipBEMGeometry
+    from bem_amd.emission import cap_source
+    from bem_amd.particles import ParticleGroup
+
+    R = 1e-6
+    geo = HemisphericalTipBEMGeometry(-1e8, R)
+    sur = surrogate.train(geo, n_samples=1 << 16, steps=3000, batch=2048, tol=5e-2, n_val=1 << 14, chunk=1000,
+                          d_max=40e-6)
+    rep = sur.report
+    _say(f"tip surrogate: band {rep['band']:.3e} m, field err max {rep['field_err_max']:.2e} "
+         f"median {rep['field_err_median']:.2e}")
+    assert np.isfinite(rep["band"])
+    n = 4096
+    src = cap_source(n, R, 0.0, seed=2)
+    pg = ParticleGroup(dict(src, weight=np.full(n, 1.602176634e-19), status=np.ones(n), species="electron"))
+    runs = {}
+    for name, s in (("plain", None), ("surrogate", sur)):
+        tr = SpecificParticleTracer(pg, 16, geo, [30e-6], rtol=1e-8, z_max=40e-6, mode="lockstep", surrogate=s)
+        runs[name] = tr.trace()
+    a, b = runs["plain"], runs["surrogate"]
+    ea, eb = a["events"], b["events"]
+    assert len(ea) == len(eb) == n
+    ka = np.lexsort((ea["particle"], ea["group"]))
+    kb = np.lexsort((eb["particle"], eb["group"]))
+    dt = np.abs(ea["t"][ka] - eb["t"][kb]) / ea["t"][ka]
+    _say(f"tip surrogate trace: {a['device_ms']:.1f} ms plain vs {b['device_ms']:.1f} ms surrogate; "
+         f"screen-time rel diff max {dt.max():.2e}")
+    assert dt.max() < 1e-3
