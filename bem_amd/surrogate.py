@@ -1,6 +1,6 @@
 """Learned field surrogate for axisymmetric BEM tip geometries (no
 counterpart in the reference, which evaluates the BEM field directly; the
-plain field gather stays the judge of the surrogate's error).
+plain field gather is the reference the surrogate's error is measured against).
 
 A Fourier-feature MLP maps (rho, z) to the BEM perturbation (phi, E_rho,
 E_z). Inputs are polar coordinates about the cap centre (log r, cos
@@ -12,7 +12,7 @@ the device: BF16 MFMA products with FP32 accumulation and FP32 master
 weights under Adam, first with one BF16 term per operand, then with the
 three-term BF16 split so the trained weights match the ~FP32 inference.
 
-After training, the surrogate is judged against the BEM on fresh points,
+After training, the surrogate is validated against the BEM on fresh points,
 and the fallback band (distance from the surface inside which the tracer
 keeps the BEM) is the smallest candidate >= `min_band_panels` panel
 lengths beyond which the worst field error, relative to the magnitude of
@@ -82,7 +82,7 @@ def initial_params(seed=0):
 
 class FieldSurrogate:
     """A trained surrogate for one BEM geometry: its spt_mlp_cfg (as a dict
-    of plain values), FP32 parameters (a device tensor) and the judge's
+    of plain values), FP32 parameters (a device tensor) and the validation's
     report."""
 
     def __init__(self, geometry, cfg, params, report=None):
@@ -184,7 +184,7 @@ def _cosine_lr(i, n, lr0, lr1):
 def train(geometry, n_samples=1 << 20, steps=20000, batch=4096, split_fraction=0.25, lr=2e-3, lr_final=1e-5,
           d_max=None, tol=1e-6, min_band_panels=3.0, n_val=1 << 16, seed=0, chunk=500, log=None,
           theta_max_deg=80.0, sigma_B=1.5):
-    """Generate samples, train on the device and judge against the BEM.
+    """Generate samples, train on the device and validate against the BEM.
     Returns a FieldSurrogate whose report holds losses, validation errors
     and the chosen band."""
     import time
@@ -233,7 +233,7 @@ def train(geometry, n_samples=1 << 20, steps=20000, batch=4096, split_fraction=0
     torch.cuda.synchronize(dev)
     t_train = time.perf_counter() - t1
     losses = np.concatenate(losses)
-    rep = judge(sur, n_val, d_min, d_max, tol, panel, seed + 7, theta_max)
+    rep = validate(sur, n_val, d_min, d_max, tol, panel, seed + 7, theta_max)
     rep.update(t_labels_s=t_labels, t_train_s=t_train, steps=steps, batch=batch, n_samples=n_samples,
                final_loss=float(np.mean(losses[-200:])), panel_length=panel)
     sur.cfg["band"] = rep["band"]
@@ -242,7 +242,7 @@ def train(geometry, n_samples=1 << 20, steps=20000, batch=4096, split_fraction=0
     return sur
 
 
-def judge(sur, n_val, d_min, d_max, tol, panel, seed, theta_max=0.5 * np.pi):
+def validate(sur, n_val, d_min, d_max, tol, panel, seed, theta_max=0.5 * np.pi):
     """Surrogate vs BEM on fresh points: errors relative to the full
     external field magnitude |E_gun e_z + E_pert|, worst case per distance
     band, and the smallest band (>= d_min) whose worst error is <= tol."""
@@ -277,7 +277,7 @@ def band_for(cands, worst, tol):
 
 def with_tolerance(sur, tol):
     """The same network with the band re-chosen for another tolerance from
-    the judge's record."""
+    the validation record."""
     rep = dict(sur.report)
     band = band_for(rep["band_candidates"], rep["worst_beyond"], tol)
     rep.update(tol=tol, band=band, band_panels=band / rep["panel_length"] if np.isfinite(band) else None)

@@ -19,7 +19,7 @@ over segments at least two panel lengths away and FP32 pair forces, both
 Kahan-compensated; FP64 elsewhere) or "pairs32" (FP64 field, FP32 Kahan
 pair forces; lockstep); and ``surrogate``: a trained
 bem_amd.surrogate.FieldSurrogate for the geometry, used in lockstep mode
-for the BEM perturbation wherever its judged band allows.
+for the BEM perturbation wherever its validated band allows.
 """
 import warnings
 

@@ -12,7 +12,7 @@ schedule (every running group one attempt per graph-replayed iteration).
 
 The surrogate (32 Fourier features, 4 x 128 SiLU, outputs phi, E_rho, E_z)
 is trained on the device before the timed region (2^20 BEM samples,
-log-uniform in distance from the surface), judged against the BEM on
+log-uniform in distance from the surface), validated against the BEM on
 fresh points, and used only beyond the distance band where its worst
 field error relative to |E| is <= 1e-6 (and never within 3 panel lengths).
 The plain method (no surrogate) is timed on the same workload; so are the

@@ -11,8 +11,8 @@ trained surrogate gains in a trace.
   output's largest magnitude: FP64 MFMA <= 1e-12; three-term BF16 split
   <= 1e-5.
 * Training drives the loss down; a surrogate trained on the post tip is
-  judged against the BEM and, used in a lockstep trace, cuts the device
-  time; its trajectories stay within the judged error of the plain ones.
+  validated against the BEM and, used in a lockstep trace, cuts the device
+  time; its trajectories stay within the validated error of the plain ones.
 """
 import ctypes
 import sys
@@ -245,9 +245,9 @@ def test_training_reduces_the_loss(gpu):
 
 
 def test_surrogate_gain_in_a_trace(gpu):
-    """Train a small surrogate on the post tip, judge it against the BEM, and
+    """Train a small surrogate on the post tip, validate it against the BEM, and
     trace the same 16384 electrons (1024 groups of 16, rtol 1e-8, FP32
-    pairs) in lockstep with and without it. The band is the judge's for a
+    pairs) in lockstep with and without it. The band is the validation's for a
     loose 5e-2 field tolerance (this short training reaches ~1e-2 near the
     surface), so the test shows the mechanism and its gain; screen-crossing
     times must agree with the plain run to 1e-3 relative."""
@@ -262,7 +262,7 @@ def test_surrogate_gain_in_a_trace(gpu):
     sur = surrogate.train(geo, n_samples=1 << 16, steps=3000, batch=2048, tol=5e-2, n_val=1 << 14, chunk=1000,
                           d_max=110e-6, log=_say)
     rep = sur.report
-    _say(f"judge: band {rep['band']:.3e} m ({rep['band_panels']} panels), field err max {rep['field_err_max']:.2e} "
+    _say(f"validation: band {rep['band']:.3e} m ({rep['band_panels']} panels), field err max {rep['field_err_max']:.2e} "
          f"median {rep['field_err_median']:.2e}, phi err max {rep['phi_err_max']:.2e}; labels {rep['t_labels_s']:.2f}"
          f" s, training {rep['t_train_s']:.2f} s")
     assert np.isfinite(rep["band"])
@@ -294,7 +294,7 @@ def test_surrogate_gain_in_a_trace(gpu):
 def test_surrogate_on_the_hemispherical_tip(gpu):
     """The same surrogate on the reference's own BEM tip geometry (sphere of
     radius R on the plane, cap centre at the origin): trained briefly,
-    judged against the BEM (finite band at a loose 5e-2 tolerance), and a
+    validated against the BEM (finite band at a loose 5e-2 tolerance), and a
     lockstep trace of 4096 electrons with it crosses the screen at the
     plain run's times to 1e-3 relative."""
     from bem_amd import SpecificParticleTracer, surrogate
