@@ -125,32 +125,60 @@ __global__ void node_table_kernel(const double* prof, const double* sigma, int n
 // Near segment (closer than its own length): s = s0 + d sinh(v), Gauss-8 in
 // intervals of width <= 0.5 over [asinh(-s0/d), asinh((L-s0)/d)], density
 // linear between the segment's nodal values (reference
-// bem/axisymmetric.py:571-600).
+// bem/axisymmetric.py:571-600). sinh and cosh of a node come from one
+// exp (their absolute error, ~1e-16 cosh v, moves the node by ~1e-16 d).
+struct NearRule {
+  double d, s0, lo, step, half;
+  int count;
+};
+
+__device__ __forceinline__ NearRule near_rule(const double* g, double proj, double dist) {
+  NearRule r;
+  const double L = g[S_L];
+  r.d = fmax(dist, 1e-15 * L);
+  r.s0 = proj;
+  r.lo = asinh(-r.s0 / r.d);
+  const double hi = asinh((L - r.s0) / r.d);
+  r.count = max(1, (int)ceil((hi - r.lo) / 0.5));
+  r.step = (hi - r.lo) / r.count;
+  r.half = 0.5 * (hi - r.lo) / r.count;
+  return r;
+}
+
+// Gauss node i (interval i / 8, node i % 8) of a segment's near rule.
+__device__ __forceinline__ void near_node(double re, double re2, double ze, const double* g, double sig0, double sig1,
+                                          const NearRule& r, int i, RingAcc& acc) {
+  const int k = i >> 3, q = i & 7;
+  const double v = r.lo + (k + 0.5 * (c_gx8[q] + 1.0)) * r.step;
+  const double ev = exp(v), iv = rcp64(ev);
+  const double sh = 0.5 * (ev - iv), ch = 0.5 * (ev + iv);
+  const double L = g[S_L];
+  const double s = r.s0 + r.d * sh;
+  const double sr = g[S_P0R] + s * g[S_UR];
+  const double sz = g[S_P0Z] + s * g[S_UZ];
+  const double dens = sig0 + (s / L) * (sig1 - sig0);
+  const double w = dens * (2 * kPi * sr) * r.d * ch * r.half * c_gw8[q];
+  ring_accum(re, re2, ze - sz, sr, w, acc);
+}
+
 __device__ __noinline__ void near_segment(double re, double re2, double ze, const double* g, double sig0, double sig1,
                                           double proj, double dist, RingAcc& acc) {
-  const double L = g[S_L];
-  const double d = fmax(dist, 1e-15 * L);
-  const double s0 = proj;
-  const double lo = asinh(-s0 / d);
-  const double hi = asinh((L - s0) / d);
-  const int count = max(1, (int)ceil((hi - lo) / 0.5));
-  const double step = (hi - lo) / count;
-  const double half = 0.5 * (hi - lo) / count;
-  const double p0r = g[S_P0R], p0z = g[S_P0Z], ur = g[S_UR], uz = g[S_UZ];
-  for (int k = 0; k < count; ++k) {
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const double v = lo + (k + 0.5 * (c_gx8[q] + 1.0)) * step;
-      const double sh = sinh(v), ch = cosh(v);
-      const double s = s0 + d * sh;
-      const double sr = p0r + s * ur;
-      const double sz = p0z + s * uz;
-      const double dens = sig0 + (s / L) * (sig1 - sig0);
-      const double w = dens * (2 * kPi * sr) * d * ch * half * c_gw8[q];
-      ring_accum(re, re2, ze - sz, sr, w, acc);
-    }
-  }
+  const NearRule r = near_rule(g, proj, dist);
+  for (int i = 0; i < 8 * r.count; ++i) near_node(re, re2, ze, g, sig0, sig1, r, i, acc);
 }
+
+// Projection of the point (re, ze) onto segment g, clamped to it, and the
+// distance to that point.
+__device__ __forceinline__ void segment_proj(double re, double ze, const double* g, double& proj, double& dist) {
+  const double L = g[S_L], ur = g[S_UR], uz = g[S_UZ];
+  const double dr = re - g[S_P0R], dz = ze - g[S_P0Z];
+  const double along = dr * ur + dz * uz;
+  proj = fmin(fmax(along, 0.0), L);
+  const double qr = dr - proj * ur, qz = dz - proj * uz;
+  dist = sqrt(qr * qr + qz * qz);
+}
+
+constexpr int kNearSlots = 7;  // near segments a slice hands to the point's cooperative pass
 
 template <int T, int NMAX>
 struct BemShared {
@@ -160,6 +188,10 @@ struct BemShared {
       double bd2[T];
       int bseg[T];
     } a;
+    struct {
+      int cnt[T];
+      int seg[kNearSlots][T];
+    } n;
     double acc[4][T];
   } u;
 };
@@ -169,8 +201,9 @@ struct BemShared {
 // taking segments s, s+S, ... Closest segment first (first index wins
 // ties, reference bem/axisymmetric.py:341-389) to place exactly-on-surface
 // points 1e-4 L off the surface (554-557), then per segment either its far
-// Gauss nodes or its near sinh rule. Partial sums are combined in slice
-// order, so results are deterministic. m must be block-uniform.
+// Gauss nodes or its near sinh rule, the latter shared out over the
+// point's slices afterwards. Partial sums are combined in slice order, so
+// results are deterministic. m must be block-uniform.
 template <int T, int NMAX, int PR>
 __device__ void bem_pert_block(const BemView& B, int m, const double* px, const double* py, const double* pz,
                                double* ox, double* oy, double* oz, BemShared<T, NMAX>& sh) {
@@ -232,20 +265,21 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
   acc.zero();
   RingAcc32 acc32;
   acc32.zero();
+  const double re = on ? sh.re[p] : 0.0, ze = on ? sh.ze[p] : 0.0, re2 = re * re;
+  int n_near = 0;  // near segments handed to the point's cooperative pass
   if (on) {
-    const double re = sh.re[p], ze = sh.ze[p], re2 = re * re;
     if (B.n_seg >= 4 * S) {
       // Many segments per slice: stride over whole segments.
       for (int j = s; j < B.n_seg; j += S) {
         const double* g = B.segs + (size_t)j * SEG_STRIDE;
-        const double L = g[S_L], ur = g[S_UR], uz = g[S_UZ];
-        const double dr = re - g[S_P0R], dz = ze - g[S_P0Z];
-        const double along = dr * ur + dz * uz;
-        const double proj = fmin(fmax(along, 0.0), L);
-        const double qr = dr - proj * ur, qz = dz - proj * uz;
-        const double dist = sqrt(qr * qr + qz * qz);
+        const double L = g[S_L];
+        double proj, dist;
+        segment_proj(re, ze, g, proj, dist);
         if (dist < L) {
-          near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, acc);
+          if (n_near < kNearSlots)
+            sh.u.n.seg[n_near++][tid] = j;
+          else
+            near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, acc);
         } else if (PR == SPT_PREC_MIXED && dist >= 2.0 * L) {
           const double4* nd = B.nodes + (size_t)j * B.nq;
 #pragma unroll 1
@@ -283,14 +317,16 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
         const int j = u / n_sub;
         const int sub = u - j * n_sub;
         const double* g = B.segs + (size_t)j * SEG_STRIDE;
-        const double L = g[S_L], ur = g[S_UR], uz = g[S_UZ];
-        const double dr = re - g[S_P0R], dz = ze - g[S_P0Z];
-        const double along = dr * ur + dz * uz;
-        const double proj = fmin(fmax(along, 0.0), L);
-        const double qr = dr - proj * ur, qz = dz - proj * uz;
-        const double dist = sqrt(qr * qr + qz * qz);
+        const double L = g[S_L];
+        double proj, dist;
+        segment_proj(re, ze, g, proj, dist);
         if (dist < L) {
-          if (sub == 0) near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, acc);
+          if (sub == 0) {
+            if (n_near < kNearSlots)
+              sh.u.n.seg[n_near++][tid] = j;
+            else
+              near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, acc);
+          }
         } else if (PR == SPT_PREC_MIXED && dist >= 2.0 * L) {
           const double4* nd = B.nodes + (size_t)u * 4;
 #pragma unroll
@@ -316,12 +352,37 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
         }
       }
     }
+  }
+  sh.u.n.cnt[tid] = n_near;
+  __syncthreads();
+  if (on) {
+    // Cooperative near rule: the Gauss nodes of all the point's handed-over
+    // near segments (slice order, then list order), numbered consecutively,
+    // node f to slice f mod S. Balances the sinh rule, which would
+    // otherwise fall on the one or two slices holding the point's closest
+    // segments.
+    const int t0 = p * S;
+    int base = 0;
+    for (int ss = 0; ss < S; ++ss) {
+      const int c = sh.u.n.cnt[t0 + ss];
+      for (int k = 0; k < c; ++k) {
+        const int j = sh.u.n.seg[k][t0 + ss];
+        const double* g = B.segs + (size_t)j * SEG_STRIDE;
+        double proj, dist;
+        segment_proj(re, ze, g, proj, dist);
+        const NearRule r = near_rule(g, proj, dist);
+        const int nodes = 8 * r.count;
+        for (int i = (s - base % S + S) % S; i < nodes; i += S)
+          near_node(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], r, i, acc);
+        base += nodes;
+      }
+    }
     if (PR == SPT_PREC_MIXED) {
       acc.er_main += acc32.er.value();
       acc.ez_main += acc32.ez.value();
     }
   }
-  __syncthreads();  // bd2/bseg reads done before acc (aliased) is written
+  __syncthreads();  // near-list reads done before acc (aliased) is written
   sh.u.acc[0][tid] = acc.er_main;
   sh.u.acc[1][tid] = acc.ez_main;
   sh.u.acc[2][tid] = acc.er_ax;
