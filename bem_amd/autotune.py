@@ -21,7 +21,7 @@ from . import _lib
 
 
 def _auto_points_per_block(n, n_cu):
-    want = n // (12 * n_cu)
+    want = n // (48 * n_cu)
     pb = 1
     while pb * 2 <= want and pb < 64:
         pb *= 2
@@ -41,7 +41,7 @@ def candidates(tracer, n_cu):
                 out.append(dict(mode="persistent", block_threads=T, field_shape=None))
     pb0 = _auto_points_per_block(n, n_cu)
     for T in (192, 256):
-        for pb in sorted({max(1, pb0 // 2), pb0, min(64, pb0 * 2)}):
+        for pb in sorted({max(1, pb0 // 2), pb0, min(64, pb0 * 2), min(64, pb0 * 4)}):
             out.append(dict(mode="lockstep", block_threads=0, field_shape=(T, pb)))
     return out
 
