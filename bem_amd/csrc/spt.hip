@@ -161,10 +161,15 @@ __device__ __forceinline__ void near_node(double re, double re2, double ze, cons
   ring_accum(re, re2, ze - sz, sr, w, acc);
 }
 
-__device__ __noinline__ void near_segment(double re, double re2, double ze, const double* g, double sig0, double sig1,
-                                          double proj, double dist, RingAcc& acc) {
+// Whole near segment on one thread; its sum comes back by value (the
+// caller adds it), so the caller's accumulator never goes to memory.
+__device__ __noinline__ RingAcc near_segment(double re, double re2, double ze, const double* g, double sig0,
+                                             double sig1, double proj, double dist) {
   const NearRule r = near_rule(g, proj, dist);
+  RingAcc acc;
+  acc.zero();
   for (int i = 0; i < 8 * r.count; ++i) near_node(re, re2, ze, g, sig0, sig1, r, i, acc);
+  return acc;
 }
 
 // Projection of the point (re, ze) onto segment g, clamped to it, and the
@@ -279,7 +284,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
           if (n_near < kNearSlots)
             sh.u.n.seg[n_near++][tid] = j;
           else
-            near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, acc);
+            acc.add(near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist));
         } else if (PR == SPT_PREC_MIXED && dist >= 2.0 * L) {
           const double4* nd = B.nodes + (size_t)j * B.nq;
 #pragma unroll 1
@@ -325,7 +330,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
             if (n_near < kNearSlots)
               sh.u.n.seg[n_near++][tid] = j;
             else
-              near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, acc);
+              acc.add(near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist));
           }
         } else if (PR == SPT_PREC_MIXED && dist >= 2.0 * L) {
           const double4* nd = B.nodes + (size_t)u * 4;

@@ -115,10 +115,14 @@ __device__ __forceinline__ double rsqrt64(double x) {
 
 // Near-axis Taylor terms of the ring field (reference
 // bem/axisymmetric.py:262-264), full constants included.
-__device__ __noinline__ void ring_field_axis(double rho, double z, double a, double& er, double& ez) {
+// The half-integer powers are ad2^k sqrt(ad2) (a few ulps from pow).
+__device__ __forceinline__ double2 ring_field_axis(double rho, double z, double a) {
   const double ad2 = a * a + z * z;
-  er = rho * (2 * z * z - a * a) / (8 * kPi * pow(ad2, 2.5));
-  ez = z / (4 * kPi * pow(ad2, 1.5)) + 3 * rho * rho * z * (3 * a * a - 2 * z * z) / (16 * kPi * pow(ad2, 3.5));
+  const double p15 = ad2 * sqrt(ad2), p25 = ad2 * p15, p35 = ad2 * p25;
+  double2 f;
+  f.x = rho * (2 * z * z - a * a) / (8 * kPi * p25);
+  f.y = z / (4 * kPi * p15) + 3 * rho * rho * z * (3 * a * a - 2 * z * z) / (16 * kPi * p35);
+  return f;
 }
 
 // Weighted ring-field accumulation for one source ring (radius a, axial
@@ -131,6 +135,12 @@ __device__ __noinline__ void ring_field_axis(double rho, double z, double a, dou
 struct RingAcc {
   double er_main, ez_main, er_ax, ez_ax;
   __device__ void zero() { er_main = ez_main = er_ax = ez_ax = 0.0; }
+  __device__ void add(const RingAcc& o) {
+    er_main += o.er_main;
+    ez_main += o.ez_main;
+    er_ax += o.er_ax;
+    ez_ax += o.ez_ax;
+  }
 };
 
 // Main-branch term of ring_accum for a node known not to be near-axis.
@@ -153,10 +163,9 @@ __device__ __forceinline__ void ring_accum_main(double rho, double rho2, double 
 
 __device__ __forceinline__ void ring_accum(double rho, double rho2, double zz, double a, double w, RingAcc& acc) {
   if (rho2 < 1e-10 * (a * a + zz * zz)) {
-    double er, ez;
-    ring_field_axis(rho, zz, a, er, ez);
-    acc.er_ax = fma(w, er, acc.er_ax);
-    acc.ez_ax = fma(w, ez, acc.ez_ax);
+    const double2 f = ring_field_axis(rho, zz, a);
+    acc.er_ax = fma(w, f.x, acc.er_ax);
+    acc.ez_ax = fma(w, f.y, acc.ez_ax);
     return;
   }
   ring_accum_main(rho, rho2, zz, a, w, acc);
@@ -246,10 +255,9 @@ __device__ __forceinline__ void ring_accum32(double rho, double rho2, double zz_
   const float zz2 = zz * zz;
   const float af = (float)a;
   if ((float)rho2 < 1e-10f * (af * af + zz2)) {
-    double er, ez;
-    ring_field_axis(rho, zz_d, a, er, ez);
-    acc64.er_ax = fma((double)w, er, acc64.er_ax);
-    acc64.ez_ax = fma((double)w, ez, acc64.ez_ax);
+    const double2 f = ring_field_axis(rho, zz_d, a);
+    acc64.er_ax = fma((double)w, f.x, acc64.er_ax);
+    acc64.ez_ax = fma((double)w, f.y, acc64.ez_ax);
     return;
   }
   const float sp = (float)(rho + a), sm = (float)(rho - a);
