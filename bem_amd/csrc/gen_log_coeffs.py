@@ -13,8 +13,21 @@ degree-6 polynomial fitted here (relative least squares at 50 digits, Lawson
 reweighting toward minimax) over z in [0, (3 - 2 sqrt 2)^2]. ln2_hi keeps
 42 significant bits so e * ln2_hi is exact for every double exponent.
 
+A second, table-driven form (spt_device.h log_tab) serves the elliptic
+integrals of the field gather, which need ln p only to an absolute
+2^-53 max(1, |ln p|): K(1-p) = P(p) - ln p Q(p) and E are O(1) near p = 1.
+x = 2^e m with m in [1/2, 1) (no folding), the top 8 fraction bits of m
+pick c_i, the centre of one of 256 intervals of width 2^-9, and
+
+    ln x = e ln2 + ln c_i + log1p(r),  r = m / c_i - 1 = fma(m, 1/c_i, -1),
+
+with the table holding the double nearest 1/c_i and ln c_i := -ln(that
+double) rounded, so the identity is exact up to those roundings. |r| <=
+2^-9, so log1p(r) = r + r^2 (-1/2 + r/3 - r^2/4 + r^3/5) leaves a
+truncation error below r^6 / 6 <= 2^-56.6 (Taylor coefficients suffice).
+
 Run once at development time (mpmath needed); the header it writes is
-committed. The script reports the worst error of the FP64 evaluation
+committed. The script reports the worst error of both FP64 evaluations
 (exactly rounded fma emulated with fractions) on random x in (0, 1].
 """
 import os
@@ -109,10 +122,57 @@ def check(C, hi, lo, n=3000):
     return worst / 2.0**-52
 
 
+TAB_N = 256
+TAB_Q = [-0.5, 1.0 / 3.0, -0.25, 0.2]
+
+
+def table():
+    """(1/c_i, ln c_i) for the centres c_i of [1/2 + i/512, 1/2 + (i+1)/512)."""
+    inv, logc = [], []
+    for i in range(TAB_N):
+        c = mp.mpf(0.5) + (i + mp.mpf(0.5)) / (2 * TAB_N)
+        iv = float(1 / c)
+        inv.append(iv)
+        logc.append(float(-mp.log(mp.mpf(iv))))
+    return inv, logc
+
+
+def log_tab(x, inv, logc, ln2):
+    """spt_device.h log_tab in exactly rounded FP64 steps."""
+    m, e = np.frexp(x)
+    m, e = float(m), int(e)
+    i = (int(np.float64(m).view(np.uint64)) >> 44) & (TAB_N - 1)
+    r = fma(m, inv[i], -1.0)
+    r2 = r * r
+    q = TAB_Q[-1]
+    for c in reversed(TAB_Q[:-1]):
+        q = fma(q, r, c)
+    return fma(float(e), ln2, logc[i] + fma(r2, q, r))
+
+
+def check_tab(inv, logc, ln2, n=3000):
+    """Worst |error| / max(1, |ln x|) in units of 2^-52, and the largest |r|."""
+    rmax = max(max(abs((0.5 + i / (2 * TAB_N)) * inv[i] - 1), abs((0.5 + (i + 1) / (2 * TAB_N)) * inv[i] - 1))
+               for i in range(TAB_N))
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([rng.uniform(0, 1, n // 3), 10 ** rng.uniform(-300, 0, n // 3),
+                         1 - 10 ** rng.uniform(-16, 0, n // 3), [2.0**-1022, 0.5, 1.0]])
+    worst = 0.0
+    for x in xs:
+        ref = mp.log(mp.mpf(float(x)))
+        v = log_tab(float(x), inv, logc, ln2)
+        worst = max(worst, float(abs(mp.mpf(v) - ref) / max(1, abs(ref))))
+    return worst / 2.0**-52, rmax
+
+
 def main():
     C = fit()
     hi, lo = ln2_split()
     ulp = check(C, hi, lo)
+    inv, logc = table()
+    ln2 = float(mp.log(2))
+    ulp_t, rmax = check_tab(inv, logc, ln2)
+    assert rmax <= 2.0**-9 and log_tab(1.0, inv, logc, ln2) == 0.0
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "log_coeffs.h")
     with open(out, "w") as f:
         f.write("// Generated by gen_log_coeffs.py -- do not edit by hand.\n")
@@ -122,7 +182,15 @@ def main():
         f.write(f"#define SPT_LOG_T {{ {', '.join(repr(v) for v in C)} }}\n")
         f.write(f"#define SPT_LOG_LN2_HI {hi!r}\n")
         f.write(f"#define SPT_LOG_LN2_LO {lo!r}\n")
-    print(f"wrote {out}: worst {ulp:.2f} x 2^-52", file=sys.stderr)
+        f.write(f"// Table form: {TAB_N} intervals, |r| <= {rmax:.4g}; worst |error| / max(1, |ln x|) on 3000\n")
+        f.write(f"// random x in (0, 1]: {ulp_t:.2f} x 2^-52.\n")
+        f.write(f"#define SPT_LOGT_N {TAB_N}\n")
+        f.write(f"#define SPT_LOGT_INVC {{ {', '.join(repr(v) for v in inv)} }}\n")
+        f.write(f"#define SPT_LOGT_LOGC {{ {', '.join(repr(v) for v in logc)} }}\n")
+        f.write(f"#define SPT_LOGT_Q {{ {', '.join(repr(v) for v in TAB_Q)} }}\n")
+        f.write(f"#define SPT_LOG_LN2 {ln2!r}\n")
+    print(f"wrote {out}: worst {ulp:.2f} x 2^-52, table form {ulp_t:.2f} x 2^-52 (abs / max(1, |ln x|))",
+          file=sys.stderr)
 
 
 if __name__ == "__main__":

@@ -147,7 +147,7 @@ __device__ __forceinline__ NearRule near_rule(const double* g, double proj, doub
 
 // Gauss node i (interval i / 8, node i % 8) of a segment's near rule.
 __device__ __forceinline__ void near_node(double re, double re2, double ze, const double* g, double sig0, double sig1,
-                                          const NearRule& r, int i, RingAcc& acc) {
+                                          const NearRule& r, int i, RingAcc& acc, const double2* lt) {
   const int k = i >> 3, q = i & 7;
   const double v = r.lo + (k + 0.5 * (c_gx8[q] + 1.0)) * r.step;
   const double ev = exp(v), iv = rcp64(ev);
@@ -158,17 +158,17 @@ __device__ __forceinline__ void near_node(double re, double re2, double ze, cons
   const double sz = g[S_P0Z] + s * g[S_UZ];
   const double dens = sig0 + (s / L) * (sig1 - sig0);
   const double w = dens * (2 * kPi * sr) * r.d * ch * r.half * c_gw8[q];
-  ring_accum(re, re2, ze - sz, sr, w, acc);
+  ring_accum(re, re2, ze - sz, sr, w, acc, lt);
 }
 
 // Whole near segment on one thread; its sum comes back by value (the
 // caller adds it), so the caller's accumulator never goes to memory.
 __device__ __noinline__ RingAcc near_segment(double re, double re2, double ze, const double* g, double sig0,
-                                             double sig1, double proj, double dist) {
+                                             double sig1, double proj, double dist, const double2* lt) {
   const NearRule r = near_rule(g, proj, dist);
   RingAcc acc;
   acc.zero();
-  for (int i = 0; i < 8 * r.count; ++i) near_node(re, re2, ze, g, sig0, sig1, r, i, acc);
+  for (int i = 0; i < 8 * r.count; ++i) near_node(re, re2, ze, g, sig0, sig1, r, i, acc, lt);
   return acc;
 }
 
@@ -187,6 +187,7 @@ constexpr int kNearSlots = 7;  // near segments a slice hands to the point's coo
 
 template <int T, int NMAX>
 struct BemShared {
+  double2 logt[SPT_LOGT_N];  // log_tab's table
   double re[NMAX], ze[NMAX], rho[NMAX], cph[NMAX], sph[NMAX];
   union {
     struct {
@@ -213,6 +214,7 @@ template <int T, int NMAX, int PR>
 __device__ void bem_pert_block(const BemView& B, int m, const double* px, const double* py, const double* pz,
                                double* ox, double* oy, double* oz, BemShared<T, NMAX>& sh) {
   if (m <= 0) return;
+  fill_log_table<T>(sh.logt);  // read after the barrier that ends the closest-segment pass
   const int tid = threadIdx.x;
   const int S = m >= T ? 1 : T / m;
   const int p = tid / S, s = tid - p * S;
@@ -284,7 +286,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
           if (n_near < kNearSlots)
             sh.u.n.seg[n_near++][tid] = j;
           else
-            acc.add(near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist));
+            acc.add(near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, sh.logt));
         } else if (PR == SPT_PREC_MIXED && dist >= 2.0 * L) {
           const double4* nd = B.nodes + (size_t)j * B.nq;
 #pragma unroll 1
@@ -300,13 +302,13 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
 #pragma unroll 1
             for (int q = 0; q < B.nq; ++q) {
               const double4 v = nd[q];
-              ring_accum(re, re2, ze - v.y, v.x, v.z, acc);
+              ring_accum(re, re2, ze - v.y, v.x, v.z, acc, sh.logt);
             }
           } else {
 #pragma unroll 2
             for (int q = 0; q < B.nq; ++q) {
               const double4 v = nd[q];
-              ring_accum_main(re, re2, ze - v.y, v.x, v.z, acc);
+              ring_accum_main(re, re2, ze - v.y, v.x, v.z, acc, sh.logt);
             }
           }
         }
@@ -330,7 +332,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
             if (n_near < kNearSlots)
               sh.u.n.seg[n_near++][tid] = j;
             else
-              acc.add(near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist));
+              acc.add(near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, sh.logt));
           }
         } else if (PR == SPT_PREC_MIXED && dist >= 2.0 * L) {
           const double4* nd = B.nodes + (size_t)u * 4;
@@ -345,13 +347,13 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
               const double4 v = nd[q];
-              ring_accum(re, re2, ze - v.y, v.x, v.z, acc);
+              ring_accum(re, re2, ze - v.y, v.x, v.z, acc, sh.logt);
             }
           } else {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
               const double4 v = nd[q];
-              ring_accum_main(re, re2, ze - v.y, v.x, v.z, acc);
+              ring_accum_main(re, re2, ze - v.y, v.x, v.z, acc, sh.logt);
             }
           }
         }
@@ -378,7 +380,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
         const NearRule r = near_rule(g, proj, dist);
         const int nodes = 8 * r.count;
         for (int i = (s - base % S + S) % S; i < nodes; i += S)
-          near_node(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], r, i, acc);
+          near_node(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], r, i, acc, sh.logt);
         base += nodes;
       }
     }

@@ -69,14 +69,42 @@ __device__ __forceinline__ double log_pos(double x) {
   return x == 0.0 ? -INFINITY : v;
 }
 
-// K(1-p), E(1-p) for p in [0, 1].
+// (1/c_i, ln c_i) of log_tab's 256 intervals; kernels that call log_tab
+// copy them into LDS (fill_log_table) and pass the LDS copy.
+__constant__ double c_logt_inv[SPT_LOGT_N] = SPT_LOGT_INVC;
+__constant__ double c_logt_logc[SPT_LOGT_N] = SPT_LOGT_LOGC;
+
+template <int T>
+__device__ __forceinline__ void fill_log_table(double2* tab) {
+  for (int k = threadIdx.x; k < SPT_LOGT_N; k += T) tab[k] = make_double2(c_logt_inv[k], c_logt_logc[k]);
+}
+
+// Natural logarithm of a normal x > 0 to an absolute 2^-52 max(1, |ln x|)
+// (0.74 x 2^-52 measured), the accuracy the elliptic integrals below need
+// of ln p (K and E are O(1) where ln p is small). x = 2^e m, m in [1/2, 1);
+// the top 8 fraction bits of m pick the interval centre c_i, r = m/c_i - 1
+// by one fma (|r| <= 2^-9), ln x = e ln2 + ln c_i + log1p(r) with log1p to
+// the r^5 term (truncation below 2^-56). tab = (1/c_i, ln c_i), normally in
+// LDS. 13 instructions against log_pos's ~30.
+__device__ __forceinline__ double log_tab(double x, const double2* tab) {
+  constexpr double Q[4] = SPT_LOGT_Q;
+  const double m = __builtin_amdgcn_frexp_mant(x);
+  const int e = __builtin_amdgcn_frexp_exp(x);
+  const double2 t = tab[(__double2hiint(m) >> 12) & (SPT_LOGT_N - 1)];
+  const double r = fma(m, t.x, -1.0);
+  double q = fma(Q[3], r, Q[2]);
+  q = fma(q, r, Q[1]);
+  q = fma(q, r, Q[0]);
+  return fma((double)e, SPT_LOG_LN2, t.y + fma(r * r, q, r));
+}
+
+// K(1-p), E(1-p) for p in [0, 1], given lp = ln p.
 //   K(1-p) = PK(p) - ln(p) QK(p)  (logarithmic form of DLMF 19.12.1,
 //   coefficients fitted by gen_ellip_coeffs.py, ~1.2 ulp);
 //   E from Legendre's derivative relation dK/dm = (E - (1-m)K)/(2m(1-m))
 //   (DLMF 19.4.1) with m = 1 - p:  E = p K - 2 (1-p) p dK/dp,
 //   p dK/dp = p PK'(p) - QK(p) - p ln(p) QK'(p)  (~1.6 ulp).
-__device__ __forceinline__ void ellip_ke_c(double p, double& K, double& E) {
-  const double lp = log_pos(p);
+__device__ __forceinline__ void ellip_ke_lp(double p, double lp, double& K, double& E) {
   double P, dP, Q, dQ;
   horner11_d(c_PK, p, P, dP);
   horner11_d(c_QK, p, Q, dQ);
@@ -84,6 +112,8 @@ __device__ __forceinline__ void ellip_ke_c(double p, double& K, double& E) {
   const double pkp = fma(p, dP, -Q) - (p * lp) * dQ;
   E = fma(p, K, -2.0 * (1.0 - p) * pkp);
 }
+
+__device__ __forceinline__ void ellip_ke_c(double p, double& K, double& E) { ellip_ke_lp(p, log_pos(p), K, E); }
 
 __device__ __forceinline__ double ellip_k_c(double p) {
   const double lp = log_pos(p);
@@ -144,7 +174,10 @@ struct RingAcc {
 };
 
 // Main-branch term of ring_accum for a node known not to be near-axis.
-__device__ __forceinline__ void ring_accum_main(double rho, double rho2, double zz, double a, double w, RingAcc& acc) {
+// lt: log_tab's table (p > 0 here: the gather never places a node on the
+// point).
+__device__ __forceinline__ void ring_accum_main(double rho, double rho2, double zz, double a, double w, RingAcc& acc,
+                                                const double2* lt) {
   const double zz2 = zz * zz;
   const double a2 = a * a;
   const double sp = rho + a, sm = rho - a;
@@ -153,7 +186,8 @@ __device__ __forceinline__ void ring_accum_main(double rho, double rho2, double 
   const double rD = rsqrt64(D);
   const double p = d2 * (rD * rD);
   double K, E;
-  ellip_ke_c(fmin(p, 1.0), K, E);
+  const double pc = fmin(p, 1.0);
+  ellip_ke_lp(pc, log_tab(pc, lt), K, E);
   const double inv_d2 = rcp64(d2);
   const double c = (a2 - rho2) + zz2;
   const double wr = w * rD;
@@ -161,14 +195,15 @@ __device__ __forceinline__ void ring_accum_main(double rho, double rho2, double 
   acc.ez_main = fma(wr * zz, E * inv_d2, acc.ez_main);
 }
 
-__device__ __forceinline__ void ring_accum(double rho, double rho2, double zz, double a, double w, RingAcc& acc) {
+__device__ __forceinline__ void ring_accum(double rho, double rho2, double zz, double a, double w, RingAcc& acc,
+                                           const double2* lt) {
   if (rho2 < 1e-10 * (a * a + zz * zz)) {
     const double2 f = ring_field_axis(rho, zz, a);
     acc.er_ax = fma(w, f.x, acc.er_ax);
     acc.ez_ax = fma(w, f.y, acc.ez_ax);
     return;
   }
-  ring_accum_main(rho, rho2, zz, a, w, acc);
+  ring_accum_main(rho, rho2, zz, a, w, acc, lt);
 }
 
 // Can any node of the segment (p0r, p0z) + t (abr, abz), t in [0, 1],

@@ -134,8 +134,55 @@ def test_device_log_against_numpy():
         assert np.max(np.abs(bad[nz] / ref[nz] - 1)) > 2 * 2.0**-52, kw
 
 
-def _device_ke(p):
-    """The device formula (spt_device.h ellip_ke_c) restated in NumPy."""
+def _log_table():
+    src = open(os.path.join(ROOT, "bem_amd", "csrc", "log_coeffs.h")).read()
+
+    def arr(name):
+        return np.array([float(v) for v in re.search(r"%s \{ (.*) \}" % name, src).group(1).split(",")])
+
+    return arr("SPT_LOGT_INVC"), arr("SPT_LOGT_LOGC"), arr("SPT_LOGT_Q"), float(
+        re.search(r"SPT_LOG_LN2 (\S+)", src).group(1))
+
+
+def _device_log_tab(x, index_shift=0, n_q=4):
+    """spt_device.h log_tab restated in NumPy (x > 0). index_shift / n_q
+    plant mistakes for the test's own check."""
+    inv, logc, Q, ln2 = _log_table()
+    x = np.asarray(x, dtype=np.float64)
+    m, e = np.frexp(x)
+    i = ((m.view(np.uint64) >> np.uint64(44)).astype(np.int64) + index_shift) & (len(inv) - 1)
+    r = _fma(m, inv[i], -1.0)
+    Q = Q[:n_q]
+    q = np.full_like(r, Q[-1])
+    for c in reversed(Q[:-1]):
+        q = _fma(q, r, c)
+    return _fma(e.astype(np.float64), ln2, logc[i] + _fma(r * r, q, r))
+
+
+def test_device_table_log_against_numpy():
+    """log_tab's contract: |error| <= 2^-52 max(1, |ln x|), ln 1 = 0, the
+    table spanning [1/2, 1) in 256 intervals with |m/c_i - 1| <= 2^-9."""
+    inv, logc, Q, ln2 = _log_table()
+    assert len(inv) == 256 and ln2 == np.log(2.0)
+    lo = 0.5 + np.arange(256) / 512.0
+    assert np.all(np.abs(lo * inv - 1) <= 2.0**-9) and np.all(np.abs((lo + 1 / 512.0) * inv - 1) <= 2.0**-9)
+    np.testing.assert_allclose(logc, -np.log(inv), rtol=2e-16, atol=1e-19)
+    rng = np.random.default_rng(4)
+    x = np.concatenate([rng.random(20000), 10 ** rng.uniform(-300, 0, 20000), 1 - 10 ** rng.uniform(-16, 0, 20000),
+                        [2.0**-1022, 0.5, np.nextafter(1.0, 0.0)]])
+    ref = np.log(x)
+    scale = np.maximum(1.0, np.abs(ref))
+    assert np.max(np.abs(_device_log_tab(x) - ref) / scale) < 2.0**-52
+    assert _device_log_tab(np.array([1.0]))[0] == 0.0
+    # planted slips fail the same bound: the neighbouring interval's entry,
+    # log1p one term short
+    for kw in (dict(index_shift=1), dict(n_q=3)):
+        assert np.max(np.abs(_device_log_tab(x, **kw) - ref) / scale) > 2.0**-52, kw
+
+
+def _device_ke(p, log=None):
+    """The device formula (spt_device.h ellip_ke_lp) restated in NumPy, ln p
+    by log_pos (default) or the given logarithm."""
     PK, QK = _coeffs("PK"), _coeffs("QK")
 
     def hd(c, x):
@@ -146,7 +193,7 @@ def _device_ke(p):
             s = s * x + a
         return s, d
 
-    lp = _device_log(p)
+    lp = (log or _device_log)(p)
     P, dP = hd(PK, p)
     Q, dQ = hd(QK, p)
     K = P - lp * Q
@@ -154,17 +201,22 @@ def _device_ke(p):
     return K, p * K - 2.0 * (1.0 - p) * pkp
 
 
-def test_device_ellip_coefficients_against_scipy_and_identities():
+@pytest.mark.parametrize("log", ["log_pos", "log_tab"])
+def test_device_ellip_coefficients_against_scipy_and_identities(log):
+    """K and E with either device logarithm (log_pos: toroidal seeds and
+    potentials; log_tab: the field gather's ring kernel)."""
+    lg = _device_log if log == "log_pos" else _device_log_tab
     rng = np.random.default_rng(0)
-    p = np.concatenate([10 ** rng.uniform(-300, 0, 4000), rng.random(4000), [1.0]])
-    K, E = _device_ke(p)
+    p = np.concatenate([10 ** rng.uniform(-300, 0, 4000), rng.random(4000), 1 - 10 ** rng.uniform(-16, 0, 2000),
+                        [1.0]])
+    K, E = _device_ke(p, lg)
     np.testing.assert_allclose(K, special.ellipkm1(p), rtol=4e-16 * 4)
     np.testing.assert_allclose(E, special.ellipe(1 - p), rtol=4e-16 * 4)
     # K(0) = E(0) = pi/2; E(1) = 1 (DLMF 19.6.1)
-    K1, E1 = _device_ke(np.array([1.0]))
+    K1, E1 = _device_ke(np.array([1.0]), lg)
     assert abs(K1[0] - np.pi / 2) < 4e-16 and abs(E1[0] - np.pi / 2) < 4e-16
     # Legendre relation at m = 1/2 (DLMF 19.7.1): 2 E K - K^2 = pi/2
-    Kh, Eh = _device_ke(np.array([0.5]))
+    Kh, Eh = _device_ke(np.array([0.5]), lg)
     assert abs(2 * Eh[0] * Kh[0] - Kh[0] ** 2 - np.pi / 2) < 2e-15
 
 
