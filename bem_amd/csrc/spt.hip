@@ -174,7 +174,8 @@ __device__ __noinline__ RingAcc near_segment(double re, double re2, double ze, c
 
 // Projection of the point (re, ze) onto segment g, clamped to it, and the
 // distance to that point.
-__device__ __forceinline__ void segment_proj(double re, double ze, const double* g, double& proj, double& dist) {
+template <class G>
+__device__ __forceinline__ void segment_proj(double re, double ze, G g, double& proj, double& dist) {
   const double L = g[S_L], ur = g[S_UR], uz = g[S_UZ];
   const double dr = re - g[S_P0R], dz = ze - g[S_P0Z];
   const double along = dr * ur + dz * uz;
@@ -202,23 +203,44 @@ struct BemShared {
   } u;
 };
 
+// Node and segment tables read through the constant address space: where
+// the index is uniform over the wavefront these become scalar loads.
+typedef const __attribute__((address_space(4))) double4* cnode_ptr;
+typedef const __attribute__((address_space(4))) double* cseg_ptr;
+
+// Node q of a table read through cnode_ptr: (a, z, w), component by component.
+__device__ __forceinline__ double4 load_node(cnode_ptr nd, int q) {
+  const cseg_ptr c = (cseg_ptr)(nd + q);
+  return make_double4(c[0], c[1], c[2], 0.0);
+}
+
 // Perturbation field -grad S[sigma] at m <= NMAX points held in LDS, by all
-// T threads of the block: S = T/m threads ("slices") per point, slice s
-// taking segments s, s+S, ... Closest segment first (first index wins
-// ties, reference bem/axisymmetric.py:341-389) to place exactly-on-surface
-// points 1e-4 L off the surface (554-557), then per segment either its far
-// Gauss nodes or its near sinh rule, the latter shared out over the
-// point's slices afterwards. Partial sums are combined in slice order, so
-// results are deterministic. m must be block-uniform.
+// T threads of the block: S threads ("slices") per point, slice s taking
+// segments s, s+S, ... Two lane layouts: slice-major (S = T/m, point p's
+// slices are threads p*S .. p*S+S-1) and, for 32 < m <= 64, wave-major
+// (lane = point, wavefront = slice, S = T/64): there every wavefront walks
+// one segment sequence, so its node and segment reads are uniform scalar
+// loads instead of 64 scattered vector reads per node. Closest segment
+// first (first index wins ties, reference bem/axisymmetric.py:341-389) to
+// place exactly-on-surface points 1e-4 L off the surface (554-557), then
+// per segment either its far Gauss nodes or its near sinh rule, the latter
+// shared out over the point's slices afterwards. Partial sums are combined
+// in slice order, so results are deterministic. m must be block-uniform.
 template <int T, int NMAX, int PR>
 __device__ void bem_pert_block(const BemView& B, int m, const double* px, const double* py, const double* pz,
                                double* ox, double* oy, double* oz, BemShared<T, NMAX>& sh) {
+  static_assert(T % 64 == 0, "whole wavefronts");
   if (m <= 0) return;
   fill_log_table<T>(sh.logt);  // read after the barrier that ends the closest-segment pass
   const int tid = threadIdx.x;
-  const int S = m >= T ? 1 : T / m;
-  const int p = tid / S, s = tid - p * S;
+  const bool wave_major = m > 32 && m <= 64 && B.n_seg >= 4 * (T / 64);
+  const int S = wave_major ? T / 64 : (m >= T ? 1 : T / m);
+  const int p = wave_major ? (tid & 63) : tid / S;
+  const int s = wave_major ? (tid >> 6) : tid - p * S;
   const bool on = p < m;
+  // thread holding slice k of this thread's point: slot0 + k * sstep
+  const int slot0 = wave_major ? p : p * S, sstep = wave_major ? 64 : 1;
+  auto slot = [=](int k) { return slot0 + k * sstep; };
   if (on) {
     const double x = px[p], y = py[p], z = pz[p];
     const double rho = hypot(x, y);
@@ -245,8 +267,8 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
     double bd2 = sh.u.a.bd2[tid];
     int bseg = sh.u.a.bseg[tid];
     for (int k = 1; k < S; ++k) {
-      const double d = sh.u.a.bd2[tid + k];
-      const int sg = sh.u.a.bseg[tid + k];
+      const double d = sh.u.a.bd2[slot(k)];
+      const int sg = sh.u.a.bseg[slot(k)];
       if (d < bd2 || (d == bd2 && sg < bseg)) {
         bd2 = d;
         bseg = sg;
@@ -275,10 +297,12 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
   const double re = on ? sh.re[p] : 0.0, ze = on ? sh.ze[p] : 0.0, re2 = re * re;
   int n_near = 0;  // near segments handed to the point's cooperative pass
   if (on) {
-    if (B.n_seg >= 4 * S) {
-      // Many segments per slice: stride over whole segments.
-      for (int j = s; j < B.n_seg; j += S) {
-        const double* g = B.segs + (size_t)j * SEG_STRIDE;
+    // Many segments per slice: stride over whole segments from j0. In the
+    // wave-major layout j0 is made wavefront-uniform (readfirstlane), so
+    // the table reads below compile to scalar loads.
+    auto whole_segments = [&](int j0) {
+      for (int j = j0; j < B.n_seg; j += S) {
+        const cseg_ptr g = (cseg_ptr)B.segs + (size_t)j * SEG_STRIDE;
         const double L = g[S_L];
         double proj, dist;
         segment_proj(re, ze, g, proj, dist);
@@ -286,33 +310,40 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
           if (n_near < kNearSlots)
             sh.u.n.seg[n_near++][tid] = j;
           else
-            acc.add(near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, sh.logt));
+            acc.add(near_segment(re, re2, ze, B.segs + (size_t)j * SEG_STRIDE, B.sigma[j], B.sigma[j + 1], proj,
+                                 dist, sh.logt));
         } else if (PR == SPT_PREC_MIXED && dist >= 2.0 * L) {
-          const double4* nd = B.nodes + (size_t)j * B.nq;
+          const cnode_ptr nd = (cnode_ptr)B.nodes + (size_t)j * B.nq;
 #pragma unroll 1
           for (int q = 0; q < B.nq; ++q) {
-            const double4 v = nd[q];
+            const double4 v = load_node(nd, q);
             ring_accum32(re, re2, ze - v.y, v.x, (float)v.z, acc32, acc);
           }
         } else {
           // Segments that cannot meet the near-axis branch take the
           // branch-free main term, two nodes per trip (nq is a multiple of 4).
-          const double4* nd = B.nodes + (size_t)j * B.nq;
+          const cnode_ptr nd = (cnode_ptr)B.nodes + (size_t)j * B.nq;
           if (ring_axis_possible(re2, ze, g[S_P0R], g[S_P0Z], g[S_ABR], g[S_ABZ])) {
 #pragma unroll 1
             for (int q = 0; q < B.nq; ++q) {
-              const double4 v = nd[q];
+              const double4 v = load_node(nd, q);
               ring_accum(re, re2, ze - v.y, v.x, v.z, acc, sh.logt);
             }
           } else {
 #pragma unroll 2
             for (int q = 0; q < B.nq; ++q) {
-              const double4 v = nd[q];
+              const double4 v = load_node(nd, q);
               ring_accum_main(re, re2, ze - v.y, v.x, v.z, acc, sh.logt);
             }
           }
         }
       }
+    };
+    if (B.n_seg >= 4 * S) {
+      if (wave_major)
+        whole_segments(__builtin_amdgcn_readfirstlane(s));
+      else
+        whole_segments(s);
     } else {
       // Few segments per slice: stride over sub-panels (4 Gauss nodes) so
       // slices stay balanced. The near test is per segment; the unit with
@@ -368,12 +399,11 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
     // node f to slice f mod S. Balances the sinh rule, which would
     // otherwise fall on the one or two slices holding the point's closest
     // segments.
-    const int t0 = p * S;
     int base = 0;
     for (int ss = 0; ss < S; ++ss) {
-      const int c = sh.u.n.cnt[t0 + ss];
+      const int c = sh.u.n.cnt[slot(ss)];
       for (int k = 0; k < c; ++k) {
-        const int j = sh.u.n.seg[k][t0 + ss];
+        const int j = sh.u.n.seg[k][slot(ss)];
         const double* g = B.segs + (size_t)j * SEG_STRIDE;
         double proj, dist;
         segment_proj(re, ze, g, proj, dist);
@@ -398,10 +428,10 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
   if (on && s == 0) {
     RingAcc tot = {sh.u.acc[0][tid], sh.u.acc[1][tid], sh.u.acc[2][tid], sh.u.acc[3][tid]};
     for (int k = 1; k < S; ++k) {
-      tot.er_main += sh.u.acc[0][tid + k];
-      tot.ez_main += sh.u.acc[1][tid + k];
-      tot.er_ax += sh.u.acc[2][tid + k];
-      tot.ez_ax += sh.u.acc[3][tid + k];
+      tot.er_main += sh.u.acc[0][slot(k)];
+      tot.ez_main += sh.u.acc[1][slot(k)];
+      tot.er_ax += sh.u.acc[2][slot(k)];
+      tot.ez_ax += sh.u.acc[3][slot(k)];
     }
     double er, ez;
     ring_finish(tot, sh.re[p], er, ez);
