@@ -21,9 +21,12 @@ from . import _lib
 
 
 def _auto_points_per_block(n, n_cu):
+    """The library's automatic points per block (field_points_per_block)."""
+    if n >= 64 * 12 * n_cu:
+        return 64
     want = n // (48 * n_cu)
     pb = 1
-    while pb * 2 <= want and pb < 64:
+    while pb * 2 <= want and pb < 32:
         pb *= 2
     return pb
 
@@ -41,7 +44,7 @@ def candidates(tracer, n_cu):
                 out.append(dict(mode="persistent", block_threads=T, field_shape=None))
     pb0 = _auto_points_per_block(n, n_cu)
     for T in (192, 256):
-        for pb in sorted({max(1, pb0 // 2), pb0, min(64, pb0 * 2), min(64, pb0 * 4)}):
+        for pb in sorted({max(1, pb0 // 4), max(1, pb0 // 2), pb0, min(64, pb0 * 2), min(64, pb0 * 4)}):
             out.append(dict(mode="lockstep", block_threads=0, field_shape=(T, pb)))
     return out
 

@@ -2105,10 +2105,15 @@ int spt_bem_solve(spt_ctx* c, const double* prof, int32_t n, double E_gun, doubl
 // resident workgroups, so the last round's imbalance stays small; measured
 // at 262,144 points: 16 per block 13.2 ms, 32 13.9 ms, 64 17.4 ms), at
 // most FIELD_NMAX.
+// Points per block for n points: 64, the gather's wave-major layout (node
+// tables read by scalar loads), when that still gives every CU four rounds
+// of three blocks; otherwise the largest power of two up to 32 with at
+// least 48 blocks per CU.
 static int field_points_per_block(const spt_ctx* c, int64_t n) {
+  if (n >= (int64_t)FIELD_NMAX * 12 * c->n_cu) return FIELD_NMAX;
   const int64_t want = n / (48 * (int64_t)c->n_cu);
   int PB = 1;
-  while (PB * 2 <= want && PB < FIELD_NMAX) PB *= 2;
+  while (PB * 2 <= want && PB < FIELD_NMAX / 2) PB *= 2;
   return PB;
 }
 
