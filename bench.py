@@ -42,7 +42,7 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-RK-stages/sec, 1 GPU, surrogate+batched vs plain at field err ≤1e-6"
 PEAK_FP64_TFLOPS = 78.6
 C_RING = 120          # FP64 flops per ring-kernel node (K, E, log, kernel algebra, accumulation)
-TRAFFIC_BYTES = 44.0e6  # HBM bytes per 262,144-point field-gather launch at (256, 8), from the rocprofv3 counter runs
+TRAFFIC_BYTES = 19.8e6  # HBM bytes per 262,144-point field-gather launch at (256, 64), from the rocprofv3 counter runs
 
 
 def workload(n_el=262144, group=16, n_segments=400, seed=0, rtol=1e-8):

@@ -47,21 +47,65 @@ def test_bem_field_matches_golden(torch_cuda, name):
     assert _rel_rows(f, d["pert"]) < 1e-11
 
 
-def test_bem_field_post400_against_oracle(torch_cuda):
+@pytest.mark.parametrize("shape", [None, (256, 64), (192, 64), (256, 8), (192, 1)])
+def test_bem_field_post400_against_oracle(torch_cuda, shape):
+    """Every gather layout: the automatic shape, 64 points per block (blocks
+    of 64, 64 and 22 points: wave-major and slice-major side by side), 8
+    points per block and one point per block (sub-panel striding)."""
     from bem_amd.bem.solver import AxisymmetricBEMSolution
     from oracle import bem as ob
 
-    _say("bem field post400")
+    _say("bem field post400", shape)
     d = golden("bem_post400.npz")
     rng = np.random.default_rng(7)
     # points near and on the cap, along the wall and far away
-    th = np.deg2rad(rng.uniform(0, 90, 40))
-    r = 1e-6 * (1 + 10 ** rng.uniform(-5, 0.5, 40))
-    pts = np.stack([r * np.sin(th), np.zeros(40), 10e-6 + r * np.cos(th)], -1)
+    th = np.deg2rad(rng.uniform(0, 90, 146))
+    r = 1e-6 * (1 + 10 ** rng.uniform(-5, 0.5, 146))
+    phi = rng.uniform(0, 2 * np.pi, 146)
+    pts = np.stack([r * np.sin(th) * np.cos(phi), r * np.sin(th) * np.sin(phi), 10e-6 + r * np.cos(th)], -1)
     pts = np.concatenate([pts, [[1e-6, 0, 5e-6], [1.00001e-6, 0, 3e-6], [0, 0, 11e-6], [2e-6, 1e-6, 30e-6]]])
-    got = AxisymmetricBEMSolution(d["profile"], d["sigma"]).field(pts)
+    sol = AxisymmetricBEMSolution(d["profile"], d["sigma"])
+    ctx = sol._upload(8)
+    if shape:
+        ctx.check(ctx.lib.spt_set_field_shape(ctx.h, *shape), "spt_set_field_shape")
+    try:
+        got = sol.field(pts)
+    finally:
+        ctx.lib.spt_set_field_shape(ctx.h, 0, 0)
     ref = ob.evaluate_field(d["profile"], d["sigma"], pts)
     assert _rel_rows(got, ref) < 1e-11
+
+
+def test_wave_major_trace_matches_oracle(torch_cuda):
+    """A lockstep BEM trace whose field blocks hold 48 particles (64 points
+    per block: the wave-major gather, scalar node loads) against the oracle
+    run of the same 48 cap electrons in 3 groups of 16 on the 80-panel post:
+    identical attempt/accept counts per group, state 1e-9."""
+    from bem_amd import SpecificParticleTracer
+    from bem_amd.bem.geometry import PostTipBEMGeometry
+    from bem_amd.emission import cap_source
+    from bem_amd.particles import ParticleGroup
+    from oracle import tracker as otr
+
+    _say("wave-major trace")
+    p = golden("bem_post80.npz")
+    R, H = float(p["R"]), float(p["H"])
+    src = cap_source(48, R, H, seed=5)
+    pg = ParticleGroup(dict(src, weight=np.full(48, 1.602176634e-19), status=np.ones(48), species="electron"))
+    geo = PostTipBEMGeometry(-1e8, R, H, n_segments=80, sigma=p["sigma"])
+    screens, t_max = [12e-6, 20e-6], 1.5e-12
+    tr = SpecificParticleTracer(pg, 16, geo, screens, rtol=1e-9, atol=1e-12, t_max=t_max, mode="lockstep")
+    tr.field_shape = (256, 64)
+    tr.run()
+    out = tr.last_run
+    st = otr.setup(*(src[k] for k in ("x", "y", "z", "px", "py", "pz", "t", "id")), 16)
+    o = otr.run(st, dict(kind="post_bem", E_gun=-1e8, R=R, H=H, profile=p["profile"], sigma=p["sigma"],
+                         z_handoff=20 * (R + H), kill_z_below=0.0), screens, t_max, rtol=1e-9, atol=1e-12)
+    _say(f"wave-major trace: attempts {out['attempts'].tolist()} vs oracle {o['attempts'].tolist()}")
+    np.testing.assert_array_equal(out["attempts"], o["attempts"])
+    np.testing.assert_array_equal(out["accepted"], o["accepted"])
+    pos = out["pos"].reshape(3, 16, 3)
+    np.testing.assert_allclose(pos, o["pos"], rtol=1e-9, atol=1e-9 * np.abs(o["pos"]).max())
 
 
 def test_geometry_fields(torch_cuda):
