@@ -96,7 +96,8 @@ __global__ void seg_table_kernel(const double* prof, int n_seg, double* segs) {
 }
 
 // Fixed quadrature node table (reference bem/axisymmetric.py:305-326):
-// weight = area_q * sigma_q with area_q = 2 pi rho_q (L / n_sub) w_g / 2.
+// (rho_q, z_q, weight, rho_q^2), weight = area_q * sigma_q with
+// area_q = 2 pi rho_q (L / n_sub) w_g / 2.
 __global__ void node_table_kernel(const double* prof, const double* sigma, int n_seg, int n_sub, double4* nodes) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int per = n_sub * 4;
@@ -113,7 +114,7 @@ __global__ void node_table_kernel(const double* prof, const double* sigma, int n
   const double seg_len = (L / n_sub) * 0.5 * c_gw4[gq];
   const double area = 2.0 * kPi * rho * seg_len;
   const double sq = sigma[j] + t * (sigma[j + 1] - sigma[j]);
-  nodes[k] = make_double4(rho, z, area * sq, 0.0);
+  nodes[k] = make_double4(rho, z, area * sq, rho * rho);
 }
 
 // ------------------------------------------------------- BEM field gather
@@ -208,10 +209,10 @@ struct BemShared {
 typedef const __attribute__((address_space(4))) double4* cnode_ptr;
 typedef const __attribute__((address_space(4))) double* cseg_ptr;
 
-// Node q of a table read through cnode_ptr: (a, z, w), component by component.
+// Node q of a table read through cnode_ptr, component by component.
 __device__ __forceinline__ double4 load_node(cnode_ptr nd, int q) {
   const cseg_ptr c = (cseg_ptr)(nd + q);
-  return make_double4(c[0], c[1], c[2], 0.0);
+  return make_double4(c[0], c[1], c[2], c[3]);
 }
 
 // Perturbation field -grad S[sigma] at m <= NMAX points held in LDS, by all
@@ -333,7 +334,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
 #pragma unroll 2
             for (int q = 0; q < B.nq; ++q) {
               const double4 v = load_node(nd, q);
-              ring_accum_main(re, re2, ze - v.y, v.x, v.z, acc, sh.logt);
+              ring_accum_main(re, re2, ze - v.y, v.x, v.w, v.z, acc, sh.logt);
             }
           }
         }
@@ -384,7 +385,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
               const double4 v = nd[q];
-              ring_accum_main(re, re2, ze - v.y, v.x, v.z, acc, sh.logt);
+              ring_accum_main(re, re2, ze - v.y, v.x, v.w, v.z, acc, sh.logt);
             }
           }
         }

@@ -173,13 +173,12 @@ struct RingAcc {
   }
 };
 
-// Main-branch term of ring_accum for a node known not to be near-axis.
-// lt: log_tab's table (p > 0 here: the gather never places a node on the
-// point).
-__device__ __forceinline__ void ring_accum_main(double rho, double rho2, double zz, double a, double w, RingAcc& acc,
-                                                const double2* lt) {
+// Main-branch term of ring_accum for a node known not to be near-axis
+// (a2 = a^2). lt: log_tab's table (p > 0 here: the gather never places a
+// node on the point).
+__device__ __forceinline__ void ring_accum_main(double rho, double rho2, double zz, double a, double a2, double w,
+                                                RingAcc& acc, const double2* lt) {
   const double zz2 = zz * zz;
-  const double a2 = a * a;
   const double sp = rho + a, sm = rho - a;
   const double D = fma(sp, sp, zz2);
   const double d2 = fma(sm, sm, zz2);
@@ -188,22 +187,23 @@ __device__ __forceinline__ void ring_accum_main(double rho, double rho2, double 
   double K, E;
   const double pc = fmin(p, 1.0);
   ellip_ke_lp(pc, log_tab(pc, lt), K, E);
-  const double inv_d2 = rcp64(d2);
+  const double u = E * rcp64(d2);  // E / d2
   const double c = (a2 - rho2) + zz2;
   const double wr = w * rD;
-  acc.er_main = fma(wr, fma(-c * E, inv_d2, K), acc.er_main);
-  acc.ez_main = fma(wr * zz, E * inv_d2, acc.ez_main);
+  acc.er_main = fma(wr, fma(-c, u, K), acc.er_main);
+  acc.ez_main = fma(wr * zz, u, acc.ez_main);
 }
 
 __device__ __forceinline__ void ring_accum(double rho, double rho2, double zz, double a, double w, RingAcc& acc,
                                            const double2* lt) {
-  if (rho2 < 1e-10 * (a * a + zz * zz)) {
+  const double a2 = a * a;
+  if (rho2 < 1e-10 * (a2 + zz * zz)) {
     const double2 f = ring_field_axis(rho, zz, a);
     acc.er_ax = fma(w, f.x, acc.er_ax);
     acc.ez_ax = fma(w, f.y, acc.ez_ax);
     return;
   }
-  ring_accum_main(rho, rho2, zz, a, w, acc, lt);
+  ring_accum_main(rho, rho2, zz, a, a2, w, acc, lt);
 }
 
 // Can any node of the segment (p0r, p0z) + t (abr, abz), t in [0, 1],
