@@ -146,20 +146,47 @@ __device__ __forceinline__ NearRule near_rule(const double* g, double proj, doub
   return r;
 }
 
-// Gauss node i (interval i / 8, node i % 8) of a segment's near rule.
-__device__ __forceinline__ void near_node(double re, double re2, double ze, const double* g, double sig0, double sig1,
-                                          const NearRule& r, int i, RingAcc& acc, const double2* lt) {
+// Per-segment constants of the near rule's nodes.
+struct NearSeg {
+  double p0r, p0z, ur, uz, inv_L, sig0, dsig;
+};
+
+__device__ __forceinline__ NearSeg near_seg(const double* g, double sig0, double sig1) {
+  return {g[S_P0R], g[S_P0Z], g[S_UR], g[S_UZ], 1.0 / g[S_L], sig0, sig1 - sig0};
+}
+
+// Gauss node i (interval i / 8, node i % 8) of a segment's near rule. AXIS:
+// the segment may meet the near-axis branch (ring_axis_possible), so each
+// node takes ring_accum's per-node test; otherwise the main term directly
+// (no Taylor-branch code in the loop).
+template <bool AXIS>
+__device__ __forceinline__ void near_node(double re, double re2, double ze, const NearSeg& sg, const NearRule& r,
+                                          int i, RingAcc& acc, const double2* lt) {
   const int k = i >> 3, q = i & 7;
   const double v = r.lo + (k + 0.5 * (c_gx8[q] + 1.0)) * r.step;
   const double ev = exp(v), iv = rcp64(ev);
   const double sh = 0.5 * (ev - iv), ch = 0.5 * (ev + iv);
-  const double L = g[S_L];
   const double s = r.s0 + r.d * sh;
-  const double sr = g[S_P0R] + s * g[S_UR];
-  const double sz = g[S_P0Z] + s * g[S_UZ];
-  const double dens = sig0 + (s / L) * (sig1 - sig0);
+  const double sr = sg.p0r + s * sg.ur;
+  const double sz = sg.p0z + s * sg.uz;
+  const double dens = sg.sig0 + (s * sg.inv_L) * sg.dsig;
   const double w = dens * (2 * kPi * sr) * r.d * ch * r.half * c_gw8[q];
-  ring_accum(re, re2, ze - sz, sr, w, acc, lt);
+  if (AXIS)
+    ring_accum(re, re2, ze - sz, sr, w, acc, lt);
+  else
+    ring_accum_main(re, re2, ze - sz, sr, sr * sr, w, acc, lt);
+}
+
+// Nodes i0, i0 + stride, ... of segment g's near rule.
+__device__ __forceinline__ void near_nodes(double re, double re2, double ze, const double* g, double sig0, double sig1,
+                                           const NearRule& r, int i0, int stride, RingAcc& acc, const double2* lt) {
+  const NearSeg sg = near_seg(g, sig0, sig1);
+  const int nodes = 8 * r.count;
+  if (ring_axis_possible(re2, ze, g[S_P0R], g[S_P0Z], g[S_ABR], g[S_ABZ])) {
+    for (int i = i0; i < nodes; i += stride) near_node<true>(re, re2, ze, sg, r, i, acc, lt);
+  } else {
+    for (int i = i0; i < nodes; i += stride) near_node<false>(re, re2, ze, sg, r, i, acc, lt);
+  }
 }
 
 // Whole near segment on one thread; its sum comes back by value (the
@@ -169,7 +196,7 @@ __device__ __noinline__ RingAcc near_segment(double re, double re2, double ze, c
   const NearRule r = near_rule(g, proj, dist);
   RingAcc acc;
   acc.zero();
-  for (int i = 0; i < 8 * r.count; ++i) near_node(re, re2, ze, g, sig0, sig1, r, i, acc, lt);
+  near_nodes(re, re2, ze, g, sig0, sig1, r, 0, 1, acc, lt);
   return acc;
 }
 
@@ -409,10 +436,8 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
         double proj, dist;
         segment_proj(re, ze, g, proj, dist);
         const NearRule r = near_rule(g, proj, dist);
-        const int nodes = 8 * r.count;
-        for (int i = (s - base % S + S) % S; i < nodes; i += S)
-          near_node(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], r, i, acc, sh.logt);
-        base += nodes;
+        near_nodes(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], r, (s - base % S + S) % S, S, acc, sh.logt);
+        base += 8 * r.count;
       }
     }
     if (PR == SPT_PREC_MIXED) {
