@@ -180,8 +180,9 @@ __device__ __forceinline__ void near_node(double re, double re2, double ze, cons
 // Nodes i0, i0 + stride, ... of segment g's near rule.
 __device__ __forceinline__ void near_nodes(double re, double re2, double ze, const double* g, double sig0, double sig1,
                                            const NearRule& r, int i0, int stride, RingAcc& acc, const double2* lt) {
-  const NearSeg sg = near_seg(g, sig0, sig1);
   const int nodes = 8 * r.count;
+  if (i0 >= nodes) return;  // wide point blocks: most slices hold no node of a given segment
+  const NearSeg sg = near_seg(g, sig0, sig1);
   if (ring_axis_possible(re2, ze, g[S_P0R], g[S_P0Z], g[S_ABR], g[S_ABZ])) {
     for (int i = i0; i < nodes; i += stride) near_node<true>(re, re2, ze, sg, r, i, acc, lt);
   } else {
@@ -412,7 +413,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
               const double4 v = nd[q];
-              ring_accum_main(re, re2, ze - v.y, v.x, v.w, v.z, acc, sh.logt);
+              ring_accum_main(re, re2, ze - v.y, v.x, v.x * v.x, v.z, acc, sh.logt);  // a^2 formed here: 24-byte loads
             }
           }
         }

@@ -91,6 +91,12 @@ class FieldSurrogate:
         self.params = params
         self.report = report or {}
 
+    @property
+    def inert(self):
+        """True when validation found no band where the tolerance holds: the
+        network is then never used, and a trace skips its eligibility pass."""
+        return not np.isfinite(self.cfg["band"])
+
     # -- the C struct ------------------------------------------------------
     def cfg_struct(self, band=None):
         c = _lib.MlpCfg_t()
@@ -256,7 +262,9 @@ def validate(sur, n_val, d_min, d_max, tol, panel, seed, theta_max=0.5 * np.pi):
     phi_scale = np.abs(geometry.E_gun) * np.maximum(np.hypot(rho, z), geometry.R)
     errP = np.abs(got[:, 0] - ref[:, 0]) / phi_scale
     cands = d_min * np.geomspace(1.0, d_max / d_min, 60)
-    worst = np.array([errE[d >= b].max() if np.any(d >= b) else 0.0 for b in cands])
+    # only candidates with validation points beyond them can be chosen
+    cands = cands[np.array([np.any(d >= b) for b in cands])]
+    worst = np.array([errE[d >= b].max() for b in cands])
     band = band_for(cands, worst, tol)
     return dict(tol=tol, band=band, band_panels=band / panel if np.isfinite(band) else None,
                 field_err_max=float(errE.max()), field_err_median=float(np.median(errE)),

@@ -226,7 +226,7 @@ class SpecificParticleTracer:
         a.mode = self._mode_code(ctx)
         a.precision = self._precision
         a.use_surrogate = 0
-        if self.surrogate is not None:
+        if self.surrogate is not None and not self.surrogate.inert:
             self.surrogate.install(ctx)
             a.use_surrogate = 1
         for i, z in enumerate(self.screen_positions):

@@ -341,7 +341,9 @@ def main():
                    "n_quad": n_quad, "field_shape": list(plain_tr.field_shape or (0, 0))},
         "plain_value": plain_value,
         "gain_vs_plain": value / plain_value,
-        "surrogate": {"tol": args.tol, "band_m": rep["band"], "band_panels": rep["band_panels"],
+        "surrogate": {"tol": args.tol, "band_m": rep["band"] if np.isfinite(rep["band"]) else None,
+                      "band_panels": rep["band_panels"],
+                      "inert": bool(sur.inert),
                       "field_err_max": rep["field_err_max"], "field_err_median": rep["field_err_median"],
                       "field_err_p99": rep["field_err_p99"], "phi_err_max": rep["phi_err_max"],
                       "train_s": rep["t_train_s"], "labels_s": rep["t_labels_s"], "train_steps": rep["steps"],

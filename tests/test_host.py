@@ -295,3 +295,18 @@ def test_weight_override_warning():
     with pytest.warns(UserWarning):
         tr = SpecificParticleTracer(pg, 2, FlatCathode(-1e8), [1e-6])
     np.testing.assert_array_equal(tr.weight, 1.602176634e-19)
+
+
+def test_surrogate_band_rule():
+    """The band is the smallest validated distance beyond which every
+    validated distance's worst error is within the tolerance; none -> inf,
+    and a surrogate with an infinite band is inert (never installed)."""
+    from bem_amd.surrogate import FieldSurrogate, band_for
+
+    cands, worst = [1.0, 2.0, 4.0, 8.0], [3e-4, 2e-5, 8e-7, 4e-7]
+    assert band_for(cands, worst, 1e-6) == 4.0
+    assert band_for(cands, worst, 1e-4) == 2.0
+    assert band_for(cands, worst, 1e-3) == 1.0
+    assert band_for(cands, [3e-4, 2e-5, 8e-7, 2e-6], 1e-6) == np.inf
+    assert FieldSurrogate(None, dict(band=np.inf), None).inert
+    assert not FieldSurrogate(None, dict(band=4.0), None).inert
