@@ -217,7 +217,8 @@ constexpr int kNearSlots = 7;  // near segments a slice hands to the point's coo
 
 template <int T, int NMAX>
 struct BemShared {
-  double2 logt[SPT_LOGT_N];  // log_tab's table
+  double2 logt[SPT_LOGT_N];              // log_tab's table
+  unsigned long long nmask[T / 64];      // per wavefront: lanes that handed over near segments
   double re[NMAX], ze[NMAX], rho[NMAX], cph[NMAX], sph[NMAX];
   union {
     struct {
@@ -291,18 +292,24 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
     sh.u.a.bd2[tid] = bd2;
     sh.u.a.bseg[tid] = bseg;
   }
-  __syncthreads();
-  if (on && s == 0) {
-    double bd2 = sh.u.a.bd2[tid];
-    int bseg = sh.u.a.bseg[tid];
-    for (int k = 1; k < S; ++k) {
-      const double d = sh.u.a.bd2[slot(k)];
-      const int sg = sh.u.a.bseg[slot(k)];
-      if (d < bd2 || (d == bd2 && sg < bseg)) {
-        bd2 = d;
-        bseg = sg;
+  // Closest segment over the point's slices: a fixed-pairing tree (slot(s)
+  // is this thread; the (distance, index) minimum does not depend on the
+  // pairing).
+  for (int h = 1; h < S; h <<= 1) {
+    __syncthreads();
+    if (on && (s & (2 * h - 1)) == 0 && s + h < S) {
+      const double d = sh.u.a.bd2[slot(s + h)];
+      const int sg = sh.u.a.bseg[slot(s + h)];
+      if (d < sh.u.a.bd2[tid] || (d == sh.u.a.bd2[tid] && sg < sh.u.a.bseg[tid])) {
+        sh.u.a.bd2[tid] = d;
+        sh.u.a.bseg[tid] = sg;
       }
     }
+  }
+  __syncthreads();
+  if (on && s == 0) {
+    const double bd2 = sh.u.a.bd2[tid];
+    const int bseg = sh.u.a.bseg[tid];
     const double x = px[p], y = py[p], z = pz[p];
     const double rho = hypot(x, y);
     const double* g = B.segs + (size_t)bseg * SEG_STRIDE;
@@ -421,6 +428,10 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
     }
   }
   sh.u.n.cnt[tid] = n_near;
+  {
+    const unsigned long long lanes = __ballot(n_near > 0);
+    if ((tid & 63) == 0) sh.nmask[tid >> 6] = lanes;
+  }
   __syncthreads();
   if (on) {
     // Cooperative near rule: the Gauss nodes of all the point's handed-over
@@ -429,16 +440,34 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
     // otherwise fall on the one or two slices holding the point's closest
     // segments.
     int base = 0;
-    for (int ss = 0; ss < S; ++ss) {
-      const int c = sh.u.n.cnt[slot(ss)];
+    auto near_slice = [&](int t) {
+      const int c = sh.u.n.cnt[t];
       for (int k = 0; k < c; ++k) {
-        const int j = sh.u.n.seg[k][slot(ss)];
+        const int j = sh.u.n.seg[k][t];
         const double* g = B.segs + (size_t)j * SEG_STRIDE;
         double proj, dist;
         segment_proj(re, ze, g, proj, dist);
         const NearRule r = near_rule(g, proj, dist);
         near_nodes(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], r, (s - base % S + S) % S, S, acc, sh.logt);
         base += 8 * r.count;
+      }
+    };
+    if (wave_major) {
+      for (int ss = 0; ss < S; ++ss) near_slice(slot(ss));
+    } else {
+      // The point's slices are threads slot0 .. slot0 + S - 1: visit only
+      // those whose wavefront ballot bit is set, in thread (= slice) order.
+      const int t_end = slot0 + S;
+      for (int w = slot0 >> 6; w <= (t_end - 1) >> 6; ++w) {
+        unsigned long long lanes = sh.nmask[w];
+        const int lo = slot0 - 64 * w, hi = t_end - 64 * w;
+        if (lo > 0) lanes &= ~0ull << lo;
+        if (hi < 64) lanes &= (1ull << hi) - 1ull;
+        while (lanes) {
+          const int b = __builtin_ctzll(lanes);
+          lanes &= lanes - 1ull;
+          near_slice(64 * w + b);
+        }
       }
     }
     if (PR == SPT_PREC_MIXED) {
@@ -451,15 +480,17 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
   sh.u.acc[1][tid] = acc.ez_main;
   sh.u.acc[2][tid] = acc.er_ax;
   sh.u.acc[3][tid] = acc.ez_ax;
+  // Slice partials summed by a fixed-pairing tree: deterministic.
+  for (int h = 1; h < S; h <<= 1) {
+    __syncthreads();
+    if (on && (s & (2 * h - 1)) == 0 && s + h < S) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sh.u.acc[c][tid] += sh.u.acc[c][slot(s + h)];
+    }
+  }
   __syncthreads();
   if (on && s == 0) {
-    RingAcc tot = {sh.u.acc[0][tid], sh.u.acc[1][tid], sh.u.acc[2][tid], sh.u.acc[3][tid]};
-    for (int k = 1; k < S; ++k) {
-      tot.er_main += sh.u.acc[0][slot(k)];
-      tot.ez_main += sh.u.acc[1][slot(k)];
-      tot.er_ax += sh.u.acc[2][slot(k)];
-      tot.ez_ax += sh.u.acc[3][slot(k)];
-    }
+    const RingAcc tot = {sh.u.acc[0][tid], sh.u.acc[1][tid], sh.u.acc[2][tid], sh.u.acc[3][tid]};
     double er, ez;
     ring_finish(tot, sh.re[p], er, ez);
     const bool axis = sh.rho[p] == 0.0;
