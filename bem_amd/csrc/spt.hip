@@ -254,8 +254,10 @@ __device__ __forceinline__ double4 load_node(cnode_ptr nd, int q) {
 // first (first index wins ties, reference bem/axisymmetric.py:341-389) to
 // place exactly-on-surface points 1e-4 L off the surface (554-557), then
 // per segment either its far Gauss nodes or its near sinh rule, the latter
-// shared out over the point's slices afterwards. Partial sums are combined
-// in slice order, so results are deterministic. m must be block-uniform.
+// shared out over the point's slices afterwards. Slice partials are
+// combined by a fixed-pairing tree, so results are deterministic (they
+// depend on S, i.e. on the launch shape, in the last bits). m must be
+// block-uniform.
 template <int T, int NMAX, int PR>
 __device__ void bem_pert_block(const BemView& B, int m, const double* px, const double* py, const double* pz,
                                double* ox, double* oy, double* oz, BemShared<T, NMAX>& sh) {
@@ -383,8 +385,8 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
     } else {
       // Few segments per slice: stride over sub-panels (4 Gauss nodes) so
       // slices stay balanced. The near test is per segment; the unit with
-      // sub == 0 of a near segment integrates that whole segment with the
-      // sinh rule, the segment's other units skip.
+      // sub == 0 of a near segment hands the segment to the cooperative
+      // near pass, the segment's other units skip.
       const int n_sub = B.nq >> 2;
       const int n_units = B.n_seg * n_sub;
       for (int u = s; u < n_units; u += S) {
