@@ -7,6 +7,7 @@ library uploads once. ``field``, ``image_force`` and ``kill_mask`` accept
 NumPy arrays or torch tensors of shape (..., 3) and return the same kind;
 the arithmetic runs in the HIP library.
 """
+import contextlib
 from abc import ABC, abstractmethod
 
 import numpy as np
@@ -64,6 +65,19 @@ class Geometry(ABC):
                 ctx.check(ctx.lib.spt_bem_upload(ctx.h, pp, prof_c.shape[0], sp, int(nsub)), "spt_bem_upload")
                 ctx.bem_key = key
         return ctx
+
+    @contextlib.contextmanager
+    def far_rule(self, k4=2.0, k2=16.0):
+        """Within the block, every field gather on this geometry's context
+        sums a segment at distance >= k4 L by Gauss-4 over the whole segment
+        and at >= k2 L by Gauss-2 (spt_bem_far_rule), instead of the
+        reference's fixed sub-panel table; restored on exit."""
+        ctx = self._ctx()
+        ctx.check(ctx.lib.spt_bem_far_rule(ctx.h, float(k4), float(k2)), "spt_bem_far_rule")
+        try:
+            yield
+        finally:
+            ctx.lib.spt_bem_far_rule(ctx.h, 0.0, 0.0)
 
     def _device_field(self, position, precision="fp64"):
         return _DeviceField(self.spt_geometry(), self._ctx(), _precision_code(precision)).evaluate(position)

@@ -19,7 +19,10 @@ over segments at least two panel lengths away and FP32 pair forces, both
 Kahan-compensated; FP64 elsewhere) or "pairs32" (FP64 field, FP32 Kahan
 pair forces; lockstep); and ``surrogate``: a trained
 bem_amd.surrogate.FieldSurrogate for the geometry, used in lockstep mode
-for the BEM perturbation wherever its validated band allows.
+for the BEM perturbation wherever its validated band allows; and
+``far_rule``: None (the reference's fixed far rule) or (k4, k2), the
+distance-adaptive far rule of spt_bem_far_rule for the BEM gathers of the
+trace (a field-error budget, not the plain method).
 """
 import warnings
 
@@ -45,7 +48,7 @@ class SpecificParticleTracer:
     def __init__(self, initial_particles, n_emit, geometry, screens, t_out=None,
                  plummer_radius=DEFAULT_PLUMMER_RADIUS, backend="mi355x", n_workers=1, rtol=None, atol=None,
                  t_max=None, z_max=None, group_sizes=None, block_threads=0, device=0, max_steps=10_000_000,
-                 mode="auto", precision="fp64", surrogate=None):
+                 mode="auto", precision="fp64", surrogate=None, far_rule=None):
         if n_workers != 1:
             raise ValueError(f"n_workers is only supported for backend='cpu' (got backend={backend!r}, "
                              f"n_workers={n_workers!r})")
@@ -88,6 +91,13 @@ class SpecificParticleTracer:
             if surrogate.geometry is not geometry:
                 raise ValueError("the surrogate was trained for a different geometry object")
         self.surrogate = surrogate
+        if far_rule is not None:
+            far_rule = (float(far_rule[0]), float(far_rule[1]))
+            if not (far_rule[0] >= 1.0 and far_rule[1] >= far_rule[0]):
+                raise ValueError(f"far_rule needs 1 <= k4 <= k2, got {far_rule!r}")
+            if geometry.bem_table() is None:
+                raise ValueError("far_rule applies to BEM geometries only")
+        self.far_rule = far_rule
         self.field_shape = None  # (threads, points per block) of the field gather; None = automatic
         self.max_steps = int(max_steps)
         self._setup_state(group_sizes)
@@ -245,6 +255,8 @@ class SpecificParticleTracer:
         ctx.bind_stream()
         shape = getattr(self, "field_shape", None) or (0, 0)
         ctx.check(ctx.lib.spt_set_field_shape(ctx.h, int(shape[0]), int(shape[1])), "spt_set_field_shape")
+        k4, k2 = self.far_rule or (0.0, 0.0)
+        ctx.check(ctx.lib.spt_bem_far_rule(ctx.h, k4, k2), "spt_bem_far_rule")
         t0 = torch.cuda.Event(enable_timing=True)
         t1 = torch.cuda.Event(enable_timing=True)
         t0.record()
@@ -252,6 +264,7 @@ class SpecificParticleTracer:
             ctx.check(ctx.lib.spt_trace(ctx.h, a), "spt_trace")
         finally:
             ctx.lib.spt_set_field_shape(ctx.h, 0, 0)
+            ctx.lib.spt_bem_far_rule(ctx.h, 0.0, 0.0)
         t1.record()
         ctx.sync()
         torch.cuda.synchronize(dev)

@@ -43,6 +43,7 @@ METRIC = "particle-RK-stages/sec, 1 GPU, surrogate+batched vs plain at field err
 PEAK_FP64_TFLOPS = 78.6
 C_RING = 120          # FP64 flops per ring-kernel node (K, E, log, kernel algebra, accumulation)
 TRAFFIC_BYTES = 19.8e6  # HBM bytes per 262,144-point field-gather launch at (256, 64), from the rocprofv3 counter runs
+FAR_TRAFFIC_BYTES = None  # the same with the far rule (2, 16)
 
 
 def workload(n_el=262144, group=16, n_segments=400, seed=0, rtol=1e-8):
@@ -54,7 +55,7 @@ def workload(n_el=262144, group=16, n_segments=400, seed=0, rtol=1e-8):
                 rtol=rtol, atol=1e-12, n_el=n_el)
 
 
-def make_tracer(wl, geo, surrogate=None, precision="pairs32", mode="lockstep"):
+def make_tracer(wl, geo, surrogate=None, precision="pairs32", mode="lockstep", far_rule=None):
     from bem_amd import SpecificParticleTracer
     from bem_amd.particles import ParticleGroup
 
@@ -62,7 +63,8 @@ def make_tracer(wl, geo, surrogate=None, precision="pairs32", mode="lockstep"):
     n = len(src["x"])
     pg = ParticleGroup(dict(src, weight=np.full(n, 1.602176634e-19), status=np.ones(n), species="electron"))
     return SpecificParticleTracer(pg, wl["group"], geo, wl["screens"], rtol=wl["rtol"], atol=wl["atol"],
-                                  z_max=wl["z_max"], mode=mode, precision=precision, surrogate=surrogate)
+                                  z_max=wl["z_max"], mode=mode, precision=precision, surrogate=surrogate,
+                                  far_rule=far_rule)
 
 
 def timed_traces(tr, steps, warmup):
@@ -87,14 +89,8 @@ def screen_time_diff(a, b):
     return dict(max=float(dt.max()), median=float(np.median(dt)))
 
 
-def field_gather_timing(wl, geo, n_pts=262144, reps=10, shape=None):
-    """Average launch duration of the BEM field gather over n_pts points
-    displaced 10 nm .. 10 um off the cap, timed with HIP events on the
-    stream the kernel is launched on (the library is bound to torch's
-    current stream)."""
-    import torch
-
-    from bem_amd import _lib
+def gather_points(wl, n_pts=262144):
+    """n_pts points displaced 10 nm .. 10 um off the cap (log-uniform)."""
     from bem_amd.emission import cap_source
 
     src = cap_source(n_pts, wl["R"], wl["H"], theta_max_deg=60.0, mean_ke_ev=0.2, seed=1)
@@ -104,7 +100,45 @@ def field_gather_timing(wl, geo, n_pts=262144, reps=10, shape=None):
     d = 10 ** np.random.default_rng(1).uniform(-8, -5, (n_pts, 1))
     pts = p / rc * (R + d)
     pts[:, 2] += H
+    return pts
+
+
+def far_rule_nodes_per_point(geo, pts, k4, k2, sample=8192):
+    """Mean ring-kernel nodes the far rule sums per point (32 per segment at
+    L <= D < k4 L, 4 at k4 L <= D < k2 L, 2 beyond; near segments, D < L,
+    take the sinh rule and are not counted), over the first `sample`
+    points."""
+    prof = np.asarray(geo.solution.profile)
+    p0, t = prof[:-1], np.diff(prof, axis=0)
+    L = np.linalg.norm(t, axis=1)
+    q = pts[:sample]
+    rho, z = np.hypot(q[:, 0], q[:, 1]), q[:, 2]
+    tt = np.clip(((rho[:, None] - p0[None, :, 0]) * t[None, :, 0] + (z[:, None] - p0[None, :, 1]) * t[None, :, 1])
+                 / (t ** 2).sum(1)[None], 0.0, 1.0)
+    D = np.hypot(rho[:, None] - (p0[None, :, 0] + tt * t[None, :, 0]), z[:, None] - (p0[None, :, 1] + tt * t[None, :, 1]))
+    Lr = L[None]
+    nodes = np.where(D < Lr, 0, np.where(D < k4 * Lr, 32, np.where(D < k2 * Lr, 4, 2)))
+    return float(nodes.sum(1).mean())
+
+
+def field_gather_timing(wl, geo, n_pts=262144, reps=10, shape=None, far_rule=None):
+    """Average launch duration of the BEM field gather over gather_points(),
+    timed with HIP events on the stream the kernel is launched on (the
+    library is bound to torch's current stream); far_rule = (k4, k2) times
+    the distance-adaptive far rule instead of the fixed one."""
+    import contextlib
+
+    pts = gather_points(wl, n_pts)
     ctx = geo._ctx()
+    with (geo.far_rule(*far_rule) if far_rule else contextlib.nullcontext()):
+        return _gather_timing(ctx, pts, n_pts, reps, shape)
+
+
+def _gather_timing(ctx, pts, n_pts, reps, shape):
+    import torch
+
+    from bem_amd import _lib
+
     pt = torch.as_tensor(pts, device=ctx.torch_device).contiguous()
     out = torch.empty_like(pt)
     ctx.bind_stream()
@@ -253,7 +287,12 @@ def main():
                     help="only the field-gather timing and a 256 MiB device copy (for hardware-counter runs)")
     ap.add_argument("--field-shape", default="0,0",
                     help="threads,points-per-block of the field gather for --field-only (0,0 = automatic)")
+    ap.add_argument("--far-rule", default="2,16",
+                    help="k4,k2 of the distance-adaptive far rule in the surrogate + batched traces "
+                         "(and in --field-only); 0 = the fixed rule")
     args = ap.parse_args()
+    far_rule = tuple(float(v) for v in args.far_rule.split(","))
+    far_rule = None if far_rule[0] == 0.0 else far_rule
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the tracing path is benchmarked per GPU (groups shard across GPUs "
               "with no exchange during integration); --gpus above 1 is not supported.", file=sys.stderr)
@@ -277,12 +316,13 @@ def main():
         import torch
 
         shape = tuple(int(v) for v in args.field_shape.split(","))
-        field_ms, n_pts = field_gather_timing(wl, geo, reps=3, shape=shape)
+        field_ms, n_pts = field_gather_timing(wl, geo, reps=3, shape=shape, far_rule=far_rule)
         src = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-        print(json.dumps(dict(field_ms=field_ms, points=n_pts, field_shape=shape, copy_bytes_read=src.numel() * 4,
+        print(json.dumps(dict(field_ms=field_ms, points=n_pts, field_shape=shape, far_rule=far_rule,
+                              copy_bytes_read=src.numel() * 4,
                               copy_bytes_written=dst.numel() * 4)))
         return
     say(f"bench: BEM solve {t_solve:.2f} s; training the surrogate ({args.train_steps} steps)")
@@ -294,7 +334,14 @@ def main():
     plain_tr = make_tracer(wl, geo)
     tuning = autotune.tune(plain_tr, attempts=4)
     say(f"bench: autotuned field shape {tuning['best']}")
-    tr = make_tracer(wl, geo, surrogate=sur)
+    far_err = None
+    if far_rule:
+        far_err = surrogate.far_rule_error(geo, *far_rule, n=1 << 16)
+        say(f"bench: far rule {far_rule}: field err max {far_err['field_err_max']:.2e}")
+        if far_err["field_err_max"] > args.tol:
+            say("bench: the far rule misses the tolerance; the surrogate + batched traces use the fixed rule")
+            far_rule = None
+    tr = make_tracer(wl, geo, surrogate=sur, far_rule=far_rule)
     tr.field_shape = plain_tr.field_shape
     outs, wall = timed_traces(tr, args.steps, args.warmup)
     say(f"bench: {args.steps} surrogate traces in {wall:.2f} s")
@@ -303,6 +350,16 @@ def main():
     plain_outs, plain_wall = timed_traces(plain_tr, 1, 0)
     plain_value = plain_outs[0]["particle_stages"] / plain_wall
     say(f"bench: plain trace {plain_wall:.2f} s")
+    net_off = None  # the same batched trace without the network: what the network itself gains
+    if far_rule:
+        t3 = make_tracer(wl, geo, far_rule=far_rule)
+        t3.field_shape = plain_tr.field_shape
+        o3, w3 = timed_traces(t3, 1, 0)
+        net_off = dict(value=o3[0]["particle_stages"] / w3, wall_s=w3,
+                       attempts_mean=float(o3[0]["attempts"].mean()),
+                       screen_t_rel_diff_vs_plain=screen_time_diff(plain_outs[0], o3[0]))
+        say(f"bench: far-rule trace without the network {w3:.2f} s")
+    base_wall = net_off["wall_s"] if net_off else plain_wall
     sweep = []
     if not args.no_sweep:
         for tol in (1e-4, 1e-3):
@@ -310,17 +367,46 @@ def main():
             if not np.isfinite(s.cfg["band"]):
                 sweep.append(dict(tol=tol, band_m=None))
                 continue
-            t2 = make_tracer(wl, geo, surrogate=s)
+            t2 = make_tracer(wl, geo, surrogate=s, far_rule=far_rule)
             t2.field_shape = plain_tr.field_shape
             o, w = timed_traces(t2, 1, 0)
             sweep.append(dict(tol=tol, band_m=s.cfg["band"], value=o[0]["particle_stages"] / w,
-                              wall_s=w, gain_wall=plain_wall / w, attempts_mean=float(o[0]["attempts"].mean()),
+                              wall_s=w, gain_wall_vs_plain=plain_wall / w, network_gain_wall=base_wall / w,
+                              attempts_mean=float(o[0]["attempts"].mean()),
                               screen_t_rel_diff_vs_plain=screen_time_diff(plain_outs[0], o[0])))
             say(f"bench: tol {tol:g} trace {w:.2f} s")
     field_ms, n_pts = field_gather_timing(wl, geo, shape=plain_tr.field_shape)
     n_quad = wl["n_segments"] * 8 * 4
-    achieved = n_pts * n_quad * C_RING / (field_ms / 1e3) / 1e12
+    plain_achieved = n_pts * n_quad * C_RING / (field_ms / 1e3) / 1e12
+    far_gather = None
+    if far_rule:
+        far_ms, _ = field_gather_timing(wl, geo, shape=plain_tr.field_shape, far_rule=far_rule)
+        npp = far_rule_nodes_per_point(geo, gather_points(wl, n_pts), *far_rule)
+        far_gather = dict(kernel_ms_mean=far_ms, nodes_per_point_mean=npp, points_per_launch=n_pts,
+                          flops_per_launch=n_pts * npp * C_RING)
     value = stages / wall
+    plain_roof = {"bound": "mfma", "achieved": plain_achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                  "frac": plain_achieved / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_BYTES,
+                  "traffic_source": "FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB copy), "
+                                    "profiles/bem_field_gather_post400_262k.md",
+                  "kernel": "bem_pert_kernel, fixed rule (BEM field gather; the same device code as the plain "
+                            "trace's ls_field_kernel)",
+                  "kernel_ms_mean": field_ms, "points_per_launch": n_pts, "flops_per_launch": n_pts * n_quad * C_RING,
+                  "note": "FP64 VALU-bound; MI355X FP64 vector peak = FP64 MFMA peak = 78.6 TFLOP/s"}
+    far_roof = None
+    if far_gather:
+        fa = far_gather["flops_per_launch"] / (far_gather["kernel_ms_mean"] / 1e3) / 1e12
+        far_roof = {"bound": "mfma", "achieved": fa, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                    "frac": fa / PEAK_FP64_TFLOPS, "traffic": FAR_TRAFFIC_BYTES,
+                    "traffic_source": "FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB copy), "
+                                      "profiles/bem_field_gather_far_rule_post400_262k.md",
+                    "kernel": f"bem_pert_kernel, far rule (k4, k2) = {far_rule} (the same device code as the "
+                              "value trace's ls_field_kernel)",
+                    "kernel_ms_mean": far_gather["kernel_ms_mean"], "points_per_launch": n_pts,
+                    "nodes_per_point_mean": far_gather["nodes_per_point_mean"],
+                    "flops_per_launch": far_gather["flops_per_launch"],
+                    "note": "algorithmic flops = ring-kernel nodes the far rule sums (counted on the host for the "
+                            "same points) x 120; near-rule nodes not counted"}
     res = {
         "metric": METRIC,
         "value": value,
@@ -336,11 +422,15 @@ def main():
         "data": "synthetic (seeded cap emission, numpy default_rng(0)); surrogate trained on BEM samples",
         "config": {"workload": f"post-tip 1 um on 10 um post, 400-panel BEM, {wl['n_el']} e- in "
                                f"{wl['n_el'] // wl['group']} groups of 16, rtol 1e-8, screen 90 um, anode kill 100 um",
-                   "form": f"surrogate + batched lockstep, band at field err <= {args.tol:g}; FP64 field and step "
-                           "control, FP32 + Kahan pair forces",
+                   "form": f"surrogate + batched lockstep, band at field err <= {args.tol:g}; BEM gathers with "
+                           f"the far rule {far_rule} (fixed rule if None); FP64 field and step control, FP32 + Kahan "
+                           "pair forces",
                    "n_quad": n_quad, "field_shape": list(plain_tr.field_shape or (0, 0))},
         "plain_value": plain_value,
         "gain_vs_plain": value / plain_value,
+        "far_rule": None if not far_rule else dict(far_err, screen_t_rel_diff_vs_plain=screen_time_diff(
+            plain_outs[0], outs[-1])),
+        "without_network": net_off,
         "surrogate": {"tol": args.tol, "band_m": rep["band"] if np.isfinite(rep["band"]) else None,
                       "band_panels": rep["band_panels"],
                       "inert": bool(sur.inert),
@@ -354,15 +444,8 @@ def main():
         "attempts_per_group_mean": float(np.mean(outs[-1]["attempts"])),
         "plain_attempts_per_group_mean": float(np.mean(plain_outs[0]["attempts"])),
         "active_particle_stages_per_s": act / wall,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_BYTES,
-                     "traffic_source": "FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB copy), "
-                                       "profiles/bem_field_gather_post400_262k.md",
-                     "kernel": "bem_pert_kernel (BEM field gather; the same device code as the trace's "
-                               "ls_field_kernel)",
-                     "kernel_ms_mean": field_ms, "points_per_launch": n_pts,
-                     "flops_per_launch": n_pts * n_quad * C_RING,
-                     "note": "FP64 VALU-bound; MI355X FP64 vector peak = FP64 MFMA peak = 78.6 TFLOP/s"},
+        "roofline": far_roof if far_gather else plain_roof,
+        "roofline_plain_gather": plain_roof,
         "c2_plain_fp64": c2_plain_rate(),
     }
     if not args.no_c3:

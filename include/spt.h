@@ -217,6 +217,17 @@ int spt_device_cus(spt_ctx* ctx, int* n_cu);
 int spt_bem_upload(spt_ctx* ctx, const double* profile_rz, int32_t n_nodes, const double* sigma,
                    int32_t n_subdiv);
 
+/* Distance-adaptive far rule for every later field gather on this context
+ * (field evaluation, traces). No counterpart in the reference, whose far
+ * rule is always the fixed sub-panel table of spt_bem_upload (reference
+ * bem/axisymmetric.py:560-564): with k4 > 0, a segment at distance >= k4 L
+ * from the point is summed by Gauss-4 over the whole segment, at >= k2 L by
+ * Gauss-2; segments closer than that keep the fixed table, closer than L
+ * the near rule. k4 = 0 restores the reference rule (the default). Needs
+ * 1 <= k4 <= k2 (k2 = INFINITY: no Gauss-2). A field-error budget, for the
+ * traces that state one, not a parity mode. */
+int spt_bem_far_rule(spt_ctx* ctx, double k4, double k2);
+
 /* Collocation matrix of the P1 hat basis (reference
  * bem/axisymmetric.py:275-302,430-441), row-major into host A_out
  * (n_nodes x n_nodes). */
