@@ -31,6 +31,7 @@ EXPORTS = (
     "spt_pair_forces", "spt_kill_mask", "spt_trace", "spt_bem_potential", "spt_mlp_nparams", "spt_mlp_grad",
     "spt_mlp_train", "spt_mlp_infer", "spt_surrogate_set", "spt_toroidal_q", "spt_ring_modes",
     "spt_image_upload", "spt_image_force", "spt_set_field_shape", "spt_bem_far_rule",
+    "spt_bem_multipole",
 )
 MLP_FOURIER, MLP_WIDTH, MLP_DEPTH = 32, 128, 4
 
@@ -119,6 +120,7 @@ def load():
             "spt_device_cus": (ctypes.c_int, [_p, P(ctypes.c_int)]),
             "spt_bem_upload": (ctypes.c_int, [_p, _p, ctypes.c_int32, _p, ctypes.c_int32]),
             "spt_bem_far_rule": (ctypes.c_int, [_p, ctypes.c_double, ctypes.c_double]),
+            "spt_bem_multipole": (ctypes.c_int, [_p, _p, ctypes.c_int32, ctypes.c_double, ctypes.c_double]),
             "spt_bem_assemble": (ctypes.c_int, [_p, _p, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _p]),
             "spt_bem_solve": (ctypes.c_int, [_p, _p, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_int32, _p]),

@@ -71,13 +71,21 @@ struct BemView {
   const double* segs;     // n_seg * SEG_STRIDE
   const double* sigma;    // n_seg + 1
   // Distance-adaptive far rule (nullptr: the reference's fixed rule).
-  // Per segment kCoarsePer nodes: Gauss-4 over the whole segment, then
-  // Gauss-2; a segment at distance >= k4 L from the point takes the former,
-  // >= k2 L the latter, instead of its nq sub-panel nodes.
+  // Per segment kCoarsePer nodes: Gauss-4 over the whole segment, Gauss-2,
+  // Gauss-8; a segment at distance D >= L from the point takes Gauss-8
+  // below k4 L, Gauss-4 below k2 L and Gauss-2 beyond, instead of its nq
+  // sub-panel nodes.
   const double4* coarse;
   double k4, k2;
+  // Exterior multipole expansion of the surface charge about the origin
+  // (nullptr: off): per term l, (m_l, (2l+1)/(l+1), l/(l+1), 0); used for
+  // points at r^2 >= mp_rmin2 instead of the segment sums.
+  const double4* mp;
+  int mp_n;
+  double mp_rref, mp_rmin2;
 };
-constexpr int kCoarsePer = 6;
+constexpr int kCoarsePer = 14;
+constexpr int kMultipoleMaxTerms = 1024;
 
 // Segment geometry (reference bem/axisymmetric.py:329-338,490-498).
 __global__ void seg_table_kernel(const double* prof, int n_seg, double* segs) {
@@ -126,17 +134,18 @@ __global__ void node_table_kernel(const double* prof, const double* sigma, int n
 
 // Coarse far-rule nodes per segment (no counterpart in the reference, whose
 // far rule is the fixed sub-panel table above): Gauss-4 over the whole
-// segment (4 nodes), then Gauss-2 (2 nodes), as (rho, z, area * sigma,
-// rho^2) with sigma linear along the segment. Gauss-n over a segment of
-// length L at distance D converges like (4D/L)^(-2n): at D >= 2L (Gauss-4)
-// and D >= 16L (Gauss-2) the field changes by < 3e-8 of |E| on the 400-panel
-// post against the fixed rule.
+// segment (nodes 0-3), Gauss-2 (4-5), Gauss-8 (6-13), as (rho, z,
+// area * sigma, rho^2) with sigma linear along the segment. Gauss-n over a
+// segment of length L at distance D converges like (4D/L)^(-2n): with
+// Gauss-8 at L <= D < 2L, Gauss-4 at 2L <= D < 16L and Gauss-2 beyond, the
+// field changes by < 3.5e-8 of |E| on the 400-panel post against the fixed
+// rule.
 __global__ void coarse_table_kernel(const double* prof, const double* sigma, int n_seg, double4* out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n_seg * kCoarsePer) return;
   const int j = k / kCoarsePer, r = k - j * kCoarsePer;
-  const double x = r < 4 ? c_gx4[r] : (r == 4 ? -0.5773502691896257645 : 0.5773502691896257645);
-  const double w = r < 4 ? c_gw4[r] : 1.0;
+  const double x = r < 4 ? c_gx4[r] : (r < 6 ? (r == 4 ? -0.5773502691896257645 : 0.5773502691896257645) : c_gx8[r - 6]);
+  const double w = r < 4 ? c_gw4[r] : (r < 6 ? 1.0 : c_gw8[r - 6]);
   const double p0r = prof[2 * j], p0z = prof[2 * j + 1];
   const double p1r = prof[2 * j + 2], p1z = prof[2 * j + 3];
   const double dr = p1r - p0r, dz = p1z - p0z;
@@ -276,13 +285,67 @@ __device__ __forceinline__ double4 load_node(cnode_ptr nd, int q) {
 }
 __device__ __forceinline__ double4 load_node(const double4* nd, int q) { return nd[q]; }
 
-// One segment's coarse far-rule nodes (Gauss-2 if g2, else Gauss-4); the
-// near-axis Taylor branch is tested per node only where the segment allows
-// it (axis). nd is the segment's first coarse node.
+// BEM perturbation field from the exterior multipole expansion of the
+// surface charge (no counterpart in the reference, which sums the segments
+// at every point): phi = (1/4 pi) sum_l m_l (r_ref/r)^l P_l(cos theta) / r
+// about the origin, valid outside the sphere r = r_ref holding every
+// quadrature node, so E_r = (1/4 pi r^2) sum m_l (l+1) q^l P_l and
+// E_theta = (sin theta / 4 pi r^2) sum m_l q^l P_l' (q = r_ref / r), with
+// P_{l+1} = a_l c P_l - b_l P_{l-1} and P'_{l+1} = P'_{l-1} + (2l+1) P_l.
+// Returns false (nothing written) where r^2 < mp_rmin2.
+__device__ __forceinline__ bool multipole_pert(const BemView& B, double x, double y, double z, double& ox, double& oy,
+                                               double& oz) {
+  const double rho2 = x * x + y * y;
+  const double r2 = rho2 + z * z;
+  if (!(r2 >= B.mp_rmin2)) return false;
+  const double r = sqrt(r2), ir = 1.0 / r, rho = sqrt(rho2);
+  const double c = z * ir, sn = rho * ir, q = B.mp_rref * ir;
+  double pl = 1.0, plm = 0.0, dl = 0.0, dlm = 0.0, qp = 1.0, sa = 0.0, sb = 0.0;
+  const cseg_ptr t = (cseg_ptr)B.mp;
+#pragma unroll 2
+  for (int l = 0; l < B.mp_n; ++l) {
+    const double m = t[4 * l], a = t[4 * l + 1], b = t[4 * l + 2];
+    const double mq = m * qp;
+    sa = fma(mq * (double)(l + 1), pl, sa);
+    sb = fma(mq, dl, sb);
+    const double pn = fma(a * c, pl, -b * plm);
+    const double dn = fma((double)(2 * l + 1), pl, dlm);
+    plm = pl;
+    pl = pn;
+    dlm = dl;
+    dl = dn;
+    qp *= q;
+  }
+  const double k = 1.0 / (4.0 * kPi * r2);
+  const double er = k * sa, et = k * sn * sb;
+  const double erho = er * sn + et * c;
+  ox = rho > 0.0 ? erho * (x / rho) : 0.0;
+  oy = rho > 0.0 ? erho * (y / rho) : 0.0;
+  oz = er * c - et * sn;
+  return true;
+}
+
+// One segment's coarse far-rule nodes: Gauss-2 (rule 2), Gauss-4 (1) or
+// Gauss-8 (0); the near-axis Taylor branch is tested per node only where the
+// segment allows it (axis). nd is the segment's first coarse node.
 template <class P>
-__device__ __forceinline__ void coarse_segment(double re, double re2, double ze, P nd, bool g2, bool axis, RingAcc& acc,
+__device__ __forceinline__ void coarse_segment(double re, double re2, double ze, P nd, int rule, bool axis, RingAcc& acc,
                                                const double2* lt) {
-  if (g2) {
+  if (rule == 0) {
+    if (axis) {
+#pragma unroll 1
+      for (int q = 6; q < 14; ++q) {
+        const double4 v = load_node(nd, q);
+        ring_accum(re, re2, ze - v.y, v.x, v.z, acc, lt);
+      }
+    } else {
+#pragma unroll 2
+      for (int q = 6; q < 14; ++q) {
+        const double4 v = load_node(nd, q);
+        ring_accum_main(re, re2, ze - v.y, v.x, v.w, v.z, acc, lt);
+      }
+    }
+  } else if (rule == 2) {
     const double4 v0 = load_node(nd, 4), v1 = load_node(nd, 5);
     if (axis) {
       ring_accum(re, re2, ze - v0.y, v0.x, v0.z, acc, lt);
@@ -412,8 +475,9 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
           else
             acc.add(near_segment(re, re2, ze, B.segs + (size_t)j * SEG_STRIDE, B.sigma[j], B.sigma[j + 1], proj,
                                  dist, sh.logt));
-        } else if (B.coarse && dist >= B.k4 * L) {
-          coarse_segment(re, re2, ze, (cnode_ptr)B.coarse + (size_t)j * kCoarsePer, dist >= B.k2 * L,
+        } else if (B.coarse) {
+          coarse_segment(re, re2, ze, (cnode_ptr)B.coarse + (size_t)j * kCoarsePer,
+                         dist >= B.k2 * L ? 2 : (dist >= B.k4 * L ? 1 : 0),
                          ring_axis_possible(re2, ze, g[S_P0R], g[S_P0Z], g[S_ABR], g[S_ABZ]), acc, sh.logt);
         } else if (PR == SPT_PREC_MIXED && dist >= 2.0 * L) {
           const cnode_ptr nd = (cnode_ptr)B.nodes + (size_t)j * B.nq;
@@ -468,9 +532,10 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
             else
               acc.add(near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, sh.logt));
           }
-        } else if (B.coarse && dist >= B.k4 * L) {
+        } else if (B.coarse) {
           if (sub == 0)
-            coarse_segment(re, re2, ze, B.coarse + (size_t)j * kCoarsePer, dist >= B.k2 * L,
+            coarse_segment(re, re2, ze, B.coarse + (size_t)j * kCoarsePer,
+                           dist >= B.k2 * L ? 2 : (dist >= B.k4 * L ? 1 : 0),
                            ring_axis_possible(re2, ze, g[S_P0R], g[S_P0Z], g[S_ABR], g[S_ABZ]), acc, sh.logt);
         } else if (PR == SPT_PREC_MIXED && dist >= 2.0 * L) {
           const double4* nd = B.nodes + (size_t)u * 4;
@@ -1027,7 +1092,16 @@ __device__ void stage_field(const TraceParams& P, int64_t off, int n, int st, Tr
     __syncthreads();
     return;
   }
-  if (tid < n) sh.need[tid] = sh.scomp[tid] && bem_needed(P.G, sh.sx[tid], sh.sy[tid], sh.sz[tid]);
+  if (tid < n) {
+    bool need = sh.scomp[tid] && bem_needed(P.G, sh.sx[tid], sh.sy[tid], sh.sz[tid]);
+    double qx, qy, qz;
+    if (need && P.bem.mp && multipole_pert(P.bem, sh.sx[tid], sh.sy[tid], sh.sz[tid], qx, qy, qz)) {
+      geometry_field_point(P.G, sh.sx[tid], sh.sy[tid], sh.sz[tid], qx, qy, qz, sh.fx[tid], sh.fy[tid], sh.fz[tid]);
+      sh.scomp[tid] = 0;
+      need = false;
+    }
+    sh.need[tid] = need;
+  }
   __syncthreads();
   if (tid == 0) {
     int m = 0;
@@ -1274,19 +1348,57 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void bem_pert_kernel(BemVie
                                                                           int PB) {
   __shared__ FieldShared<T, NMAX> sh;
   const int64_t base = (int64_t)blockIdx.x * PB;
-  const int m = (int)min((int64_t)PB, n - base);
+  const int m_all = (int)min((int64_t)PB, n - base);
   const int tid = threadIdx.x;
+  if (!B.mp) {
+    if (tid < m_all) {
+      sh.px[tid] = pos[3 * (base + tid)];
+      sh.py[tid] = pos[3 * (base + tid) + 1];
+      sh.pz[tid] = pos[3 * (base + tid) + 2];
+    }
+    __syncthreads();
+    bem_pert_block<T, NMAX, PR>(B, m_all, sh.px, sh.py, sh.pz, sh.ox, sh.oy, sh.oz, sh.bem);
+    if (tid < m_all) {
+      E[3 * (base + tid)] = sh.ox[tid];
+      E[3 * (base + tid) + 1] = sh.oy[tid];
+      E[3 * (base + tid) + 2] = sh.oz[tid];
+    }
+    return;
+  }
+  // Points in the multipole zone are done here; the rest go to the gather.
+  if (tid < m_all) {
+    const double x = pos[3 * (base + tid)], y = pos[3 * (base + tid) + 1], z = pos[3 * (base + tid) + 2];
+    double qx, qy, qz;
+    const bool far = multipole_pert(B, x, y, z, qx, qy, qz);
+    if (far) {
+      E[3 * (base + tid)] = qx;
+      E[3 * (base + tid) + 1] = qy;
+      E[3 * (base + tid) + 2] = qz;
+    }
+    sh.need[tid] = !far;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int m = 0;
+    for (int i = 0; i < m_all; ++i)
+      if (sh.need[i]) sh.cidx[m++] = i;
+    sh.m = m;
+  }
+  __syncthreads();
+  const int m = sh.m;
   if (tid < m) {
-    sh.px[tid] = pos[3 * (base + tid)];
-    sh.py[tid] = pos[3 * (base + tid) + 1];
-    sh.pz[tid] = pos[3 * (base + tid) + 2];
+    const int64_t gi = base + sh.cidx[tid];
+    sh.px[tid] = pos[3 * gi];
+    sh.py[tid] = pos[3 * gi + 1];
+    sh.pz[tid] = pos[3 * gi + 2];
   }
   __syncthreads();
   bem_pert_block<T, NMAX, PR>(B, m, sh.px, sh.py, sh.pz, sh.ox, sh.oy, sh.oz, sh.bem);
   if (tid < m) {
-    E[3 * (base + tid)] = sh.ox[tid];
-    E[3 * (base + tid) + 1] = sh.oy[tid];
-    E[3 * (base + tid) + 2] = sh.oz[tid];
+    const int64_t gi = base + sh.cidx[tid];
+    E[3 * gi] = sh.ox[tid];
+    E[3 * gi + 1] = sh.oy[tid];
+    E[3 * gi + 2] = sh.oz[tid];
   }
 }
 
@@ -1305,7 +1417,18 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void geometry_field_kernel(
     sx[tid] = pos[3 * (base + tid)];
     sy[tid] = pos[3 * (base + tid) + 1];
     sz[tid] = pos[3 * (base + tid) + 2];
-    sh.need[tid] = is_bem(G) && bem_needed(G, sx[tid], sy[tid], sz[tid]);
+    bool need = is_bem(G) && bem_needed(G, sx[tid], sy[tid], sz[tid]);
+    double qx, qy, qz;
+    if (need && B.mp && multipole_pert(B, sx[tid], sy[tid], sz[tid], qx, qy, qz)) {
+      double ex, ey, ez;
+      geometry_field_point(G, sx[tid], sy[tid], sz[tid], qx, qy, qz, ex, ey, ez);
+      E[3 * (base + tid)] = ex;
+      E[3 * (base + tid) + 1] = ey;
+      E[3 * (base + tid) + 2] = ez;
+      need = false;
+      sx[tid] = NAN;  // done: skipped below
+    }
+    sh.need[tid] = need;
   }
   __syncthreads();
   if (tid == 0) {
@@ -1332,7 +1455,7 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void geometry_field_kernel(
     E[3 * (base + i) + 1] = ey;
     E[3 * (base + i) + 2] = ez;
   }
-  if (tid < m_all && !sh.need[tid]) {
+  if (tid < m_all && !sh.need[tid] && !isnan(sx[tid])) {
     double ex, ey, ez;
     geometry_field_point(G, sx[tid], sy[tid], sz[tid], 0, 0, 0, ex, ey, ez);
     E[3 * (base + tid)] = ex;
@@ -1499,7 +1622,25 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_field_kernel(LockPa
       ls_keep_field(L, gi, st);
       act[tid] = 0;
     }
-    sh.need[tid] = act[tid] && is_bem(G) && bem_needed(G, sx[tid], sy[tid], sz[tid]);
+    bool need = act[tid] && is_bem(G) && bem_needed(G, sx[tid], sy[tid], sz[tid]);
+    double qx, qy, qz;
+    if (need && L.T.bem.mp && multipole_pert(L.T.bem, sx[tid], sy[tid], sz[tid], qx, qy, qz)) {
+      if (L.sur && L.sur[gi] == 2) {  // blend with the parked surrogate perturbation
+        const double w = surrogate_weight(L.V.cfg, G, sx[tid], sy[tid], sz[tid]);
+        qx = (1.0 - w) * qx + w * LKB(51);
+        qy = (1.0 - w) * qy + w * LKB(52);
+        qz = (1.0 - w) * qz + w * LKB(53);
+      }
+      double ex, ey, ez;
+      geometry_field_point(G, sx[tid], sy[tid], sz[tid], qx, qy, qz, ex, ey, ez);
+      LKB(51) = ex;
+      LKB(52) = ey;
+      LKB(53) = ez;
+      ls_keep_field(L, gi, st);
+      need = false;
+      act[tid] = 0;
+    }
+    sh.need[tid] = need;
   }
   __syncthreads();
   if (tid == 0) {
@@ -1579,7 +1720,10 @@ __global__ __launch_bounds__(64) void ls_surrogate_kernel(LockParams L, int st) 
     y = LKB(49);
     z = LKB(50);
     const bool reuse = st == 0 && L.fvalid[gi];  // ls_field_kernel copies the state's field
-    if (!reuse && (L.g_flags[g] & LS_RUNNING) && L.active[gi] && bem_needed(G, x, y, z))
+    // the multipole zone (exact to its validation tolerance, and cheaper)
+    // takes precedence over the network
+    const bool mp_zone = L.T.bem.mp && x * x + y * y + z * z >= L.T.bem.mp_rmin2;
+    if (!reuse && (L.g_flags[g] & LS_RUNNING) && L.active[gi] && bem_needed(G, x, y, z) && !mp_zone)
       w = surrogate_weight(L.V.cfg, G, x, y, z);
     elig = w >= 0.0;
     L.sur[gi] = !elig ? 0 : (w >= 1.0 ? 1 : 2);
@@ -1956,6 +2100,9 @@ struct spt_ctx {
   double4* d_nodes = nullptr;
   double4* d_coarse = nullptr;      // coarse far-rule nodes (kCoarsePer per segment)
   double far_k4 = 0.0, far_k2 = 0.0;  // spt_bem_far_rule; 0 = the reference's fixed rule
+  double4* d_mp = nullptr;            // spt_bem_multipole table (kMultipoleMaxTerms entries)
+  int mp_n = 0;
+  double mp_rref = 0.0, mp_rmin = 0.0;
   // scratch
   double* d_kbuf = nullptr;
   size_t kbuf_elems = 0;
@@ -2034,6 +2181,10 @@ static BemView bem_view(const spt_ctx* c) {
   B.coarse = c->far_k4 > 0.0 ? c->d_coarse : nullptr;
   B.k4 = c->far_k4;
   B.k2 = c->far_k2;
+  B.mp = c->mp_n > 0 ? c->d_mp : nullptr;
+  B.mp_n = c->mp_n;
+  B.mp_rref = c->mp_rref;
+  B.mp_rmin2 = c->mp_rmin * c->mp_rmin;
   return B;
 }
 
@@ -2045,6 +2196,7 @@ static void free_bem(spt_ctx* c) {
   if (c->d_coarse) (void)hipFree(c->d_coarse);
   c->d_prof = c->d_sigma = c->d_segs = nullptr;
   c->d_nodes = c->d_coarse = nullptr;
+  c->mp_n = 0;  // the expansion belongs to the uploaded charge
   c->bem_nodes = 0;
 }
 
@@ -2079,6 +2231,7 @@ void spt_destroy(spt_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   free_bem(c);
+  if (c->d_mp) (void)hipFree(c->d_mp);
   if (c->d_kbuf) (void)hipFree(c->d_kbuf);
   if (c->d_queue) (void)hipFree(c->d_queue);
   if (c->d_pgroup) (void)hipFree(c->d_pgroup);
@@ -2174,6 +2327,28 @@ int spt_bem_far_rule(spt_ctx* c, double k4, double k2) {
   if (!(k4 >= 1.0 && k2 >= k4)) return fail(c, SPT_EINVAL, "spt_bem_far_rule: need k4 = 0, or 1 <= k4 <= k2");
   c->far_k4 = k4;
   c->far_k2 = k2;
+  return SPT_OK;
+}
+
+int spt_bem_multipole(spt_ctx* c, const double* moments, int32_t n_terms, double r_ref, double r_min) {
+  if (!c) return SPT_EINVAL;
+  if (n_terms == 0) {
+    c->mp_n = 0;
+    return SPT_OK;
+  }
+  if (!moments || n_terms < 1 || n_terms > kMultipoleMaxTerms || !(r_ref > 0.0) || !(r_min > r_ref))
+    return fail(c, SPT_EINVAL, "spt_bem_multipole: need 1 <= n_terms <= 1024 and 0 < r_ref < r_min (or n_terms = 0)");
+  if (c->bem_nodes == 0) return fail(c, SPT_ENOBEM, "spt_bem_multipole: no BEM table uploaded");
+  HIPCHK(c, spt_begin(c));
+  if (!c->d_mp) HIPCHK(c, hipMalloc(&c->d_mp, sizeof(double4) * kMultipoleMaxTerms));
+  std::vector<double4> tab(n_terms);
+  for (int l = 0; l < n_terms; ++l)
+    tab[l] = make_double4(moments[l], (2.0 * l + 1.0) / (l + 1.0), (double)l / (l + 1.0), 0.0);
+  HIPCHK(c, hipMemcpyAsync(c->d_mp, tab.data(), sizeof(double4) * n_terms, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->mp_n = n_terms;
+  c->mp_rref = r_ref;
+  c->mp_rmin = r_min;
   return SPT_OK;
 }
 

@@ -69,9 +69,9 @@ class Geometry(ABC):
     @contextlib.contextmanager
     def far_rule(self, k4=2.0, k2=16.0):
         """Within the block, every field gather on this geometry's context
-        sums a segment at distance >= k4 L by Gauss-4 over the whole segment
-        and at >= k2 L by Gauss-2 (spt_bem_far_rule), instead of the
-        reference's fixed sub-panel table; restored on exit."""
+        sums a segment at distance D >= L over the whole segment by Gauss-8
+        below k4 L, Gauss-4 below k2 L and Gauss-2 beyond (spt_bem_far_rule),
+        instead of the reference's fixed sub-panel table; restored on exit."""
         ctx = self._ctx()
         ctx.check(ctx.lib.spt_bem_far_rule(ctx.h, float(k4), float(k2)), "spt_bem_far_rule")
         try:

@@ -22,7 +22,9 @@ bem_amd.surrogate.FieldSurrogate for the geometry, used in lockstep mode
 for the BEM perturbation wherever its validated band allows; and
 ``far_rule``: None (the reference's fixed far rule) or (k4, k2), the
 distance-adaptive far rule of spt_bem_far_rule for the BEM gathers of the
-trace (a field-error budget, not the plain method).
+trace; and ``multipole``: None or a validated bem_amd.bem.multipole.Multipole
+of the geometry, whose exterior expansion replaces the segment sums beyond
+its radius (both field-error budgets, not the plain method).
 """
 import warnings
 
@@ -48,7 +50,7 @@ class SpecificParticleTracer:
     def __init__(self, initial_particles, n_emit, geometry, screens, t_out=None,
                  plummer_radius=DEFAULT_PLUMMER_RADIUS, backend="mi355x", n_workers=1, rtol=None, atol=None,
                  t_max=None, z_max=None, group_sizes=None, block_threads=0, device=0, max_steps=10_000_000,
-                 mode="auto", precision="fp64", surrogate=None, far_rule=None):
+                 mode="auto", precision="fp64", surrogate=None, far_rule=None, multipole=None):
         if n_workers != 1:
             raise ValueError(f"n_workers is only supported for backend='cpu' (got backend={backend!r}, "
                              f"n_workers={n_workers!r})")
@@ -98,6 +100,9 @@ class SpecificParticleTracer:
             if geometry.bem_table() is None:
                 raise ValueError("far_rule applies to BEM geometries only")
         self.far_rule = far_rule
+        if multipole is not None and multipole.geometry is not geometry:
+            raise ValueError("the multipole expansion was fitted for a different geometry object")
+        self.multipole = multipole
         self.field_shape = None  # (threads, points per block) of the field gather; None = automatic
         self.max_steps = int(max_steps)
         self._setup_state(group_sizes)
@@ -257,6 +262,8 @@ class SpecificParticleTracer:
         ctx.check(ctx.lib.spt_set_field_shape(ctx.h, int(shape[0]), int(shape[1])), "spt_set_field_shape")
         k4, k2 = self.far_rule or (0.0, 0.0)
         ctx.check(ctx.lib.spt_bem_far_rule(ctx.h, k4, k2), "spt_bem_far_rule")
+        if self.multipole is not None and not self.multipole.inert:
+            self.multipole.install(ctx)
         t0 = torch.cuda.Event(enable_timing=True)
         t1 = torch.cuda.Event(enable_timing=True)
         t0.record()
@@ -265,6 +272,7 @@ class SpecificParticleTracer:
         finally:
             ctx.lib.spt_set_field_shape(ctx.h, 0, 0)
             ctx.lib.spt_bem_far_rule(ctx.h, 0.0, 0.0)
+            ctx.lib.spt_bem_multipole(ctx.h, None, 0, 0.0, 0.0)
         t1.record()
         ctx.sync()
         torch.cuda.synchronize(dev)

@@ -55,7 +55,7 @@ def workload(n_el=262144, group=16, n_segments=400, seed=0, rtol=1e-8):
                 rtol=rtol, atol=1e-12, n_el=n_el)
 
 
-def make_tracer(wl, geo, surrogate=None, precision="pairs32", mode="lockstep", far_rule=None):
+def make_tracer(wl, geo, surrogate=None, precision="pairs32", mode="lockstep", far_rule=None, multipole=None):
     from bem_amd import SpecificParticleTracer
     from bem_amd.particles import ParticleGroup
 
@@ -64,7 +64,7 @@ def make_tracer(wl, geo, surrogate=None, precision="pairs32", mode="lockstep", f
     pg = ParticleGroup(dict(src, weight=np.full(n, 1.602176634e-19), status=np.ones(n), species="electron"))
     return SpecificParticleTracer(pg, wl["group"], geo, wl["screens"], rtol=wl["rtol"], atol=wl["atol"],
                                   z_max=wl["z_max"], mode=mode, precision=precision, surrogate=surrogate,
-                                  far_rule=far_rule)
+                                  far_rule=far_rule, multipole=multipole)
 
 
 def timed_traces(tr, steps, warmup):
@@ -104,7 +104,7 @@ def gather_points(wl, n_pts=262144):
 
 
 def far_rule_nodes_per_point(geo, pts, k4, k2, sample=8192):
-    """Mean ring-kernel nodes the far rule sums per point (32 per segment at
+    """Mean ring-kernel nodes the far rule sums per point (8 per segment at
     L <= D < k4 L, 4 at k4 L <= D < k2 L, 2 beyond; near segments, D < L,
     take the sinh rule and are not counted), over the first `sample`
     points."""
@@ -117,7 +117,7 @@ def far_rule_nodes_per_point(geo, pts, k4, k2, sample=8192):
                  / (t ** 2).sum(1)[None], 0.0, 1.0)
     D = np.hypot(rho[:, None] - (p0[None, :, 0] + tt * t[None, :, 0]), z[:, None] - (p0[None, :, 1] + tt * t[None, :, 1]))
     Lr = L[None]
-    nodes = np.where(D < Lr, 0, np.where(D < k4 * Lr, 32, np.where(D < k2 * Lr, 4, 2)))
+    nodes = np.where(D < Lr, 0, np.where(D < k4 * Lr, 8, np.where(D < k2 * Lr, 4, 2)))
     return float(nodes.sum(1).mean())
 
 
@@ -287,6 +287,8 @@ def main():
                     help="only the field-gather timing and a 256 MiB device copy (for hardware-counter runs)")
     ap.add_argument("--field-shape", default="0,0",
                     help="threads,points-per-block of the field gather for --field-only (0,0 = automatic)")
+    ap.add_argument("--no-multipole", action="store_true",
+                    help="no exterior multipole zone in the surrogate + batched traces")
     ap.add_argument("--far-rule", default="2,16",
                     help="k4,k2 of the distance-adaptive far rule in the surrogate + batched traces "
                          "(and in --field-only); 0 = the fixed rule")
@@ -341,7 +343,15 @@ def main():
         if far_err["field_err_max"] > args.tol:
             say("bench: the far rule misses the tolerance; the surrogate + batched traces use the fixed rule")
             far_rule = None
-    tr = make_tracer(wl, geo, surrogate=sur, far_rule=far_rule)
+    mp = None
+    if not args.no_multipole:
+        from bem_amd.bem import multipole as mpl
+
+        mp = mpl.fit(geo, n_terms=128, tol=min(1e-8, args.tol))
+        say(f"bench: multipole r_min {mp.r_min:.3e} m ({mp.report['r_min_over_r_ref']:.3f} r_ref)")
+        if mp.inert:
+            mp = None
+    tr = make_tracer(wl, geo, surrogate=sur, far_rule=far_rule, multipole=mp)
     tr.field_shape = plain_tr.field_shape
     outs, wall = timed_traces(tr, args.steps, args.warmup)
     say(f"bench: {args.steps} surrogate traces in {wall:.2f} s")
@@ -351,8 +361,8 @@ def main():
     plain_value = plain_outs[0]["particle_stages"] / plain_wall
     say(f"bench: plain trace {plain_wall:.2f} s")
     net_off = None  # the same batched trace without the network: what the network itself gains
-    if far_rule:
-        t3 = make_tracer(wl, geo, far_rule=far_rule)
+    if far_rule or mp:
+        t3 = make_tracer(wl, geo, far_rule=far_rule, multipole=mp)
         t3.field_shape = plain_tr.field_shape
         o3, w3 = timed_traces(t3, 1, 0)
         net_off = dict(value=o3[0]["particle_stages"] / w3, wall_s=w3,
@@ -367,7 +377,7 @@ def main():
             if not np.isfinite(s.cfg["band"]):
                 sweep.append(dict(tol=tol, band_m=None))
                 continue
-            t2 = make_tracer(wl, geo, surrogate=s, far_rule=far_rule)
+            t2 = make_tracer(wl, geo, surrogate=s, far_rule=far_rule, multipole=mp)
             t2.field_shape = plain_tr.field_shape
             o, w = timed_traces(t2, 1, 0)
             sweep.append(dict(tol=tol, band_m=s.cfg["band"], value=o[0]["particle_stages"] / w,
@@ -422,14 +432,17 @@ def main():
         "data": "synthetic (seeded cap emission, numpy default_rng(0)); surrogate trained on BEM samples",
         "config": {"workload": f"post-tip 1 um on 10 um post, 400-panel BEM, {wl['n_el']} e- in "
                                f"{wl['n_el'] // wl['group']} groups of 16, rtol 1e-8, screen 90 um, anode kill 100 um",
-                   "form": f"surrogate + batched lockstep, band at field err <= {args.tol:g}; BEM gathers with "
-                           f"the far rule {far_rule} (fixed rule if None); FP64 field and step control, FP32 + Kahan "
-                           "pair forces",
+                   "form": f"surrogate + batched lockstep at field err <= {args.tol:g}: validated exterior "
+                           "multipole expansion of the BEM charge beyond its radius, learned network beyond its "
+                           f"band (inert if none), BEM gathers with the far rule {far_rule} (fixed rule if None) "
+                           "elsewhere; FP64 field and step control, FP32 + Kahan pair forces",
                    "n_quad": n_quad, "field_shape": list(plain_tr.field_shape or (0, 0))},
         "plain_value": plain_value,
         "gain_vs_plain": value / plain_value,
         "far_rule": None if not far_rule else dict(far_err, screen_t_rel_diff_vs_plain=screen_time_diff(
             plain_outs[0], outs[-1])),
+        "multipole": None if mp is None else {k: v for k, v in mp.report.items()
+                                              if k not in ("candidates", "worst_beyond")},
         "without_network": net_off,
         "surrogate": {"tol": args.tol, "band_m": rep["band"] if np.isfinite(rep["band"]) else None,
                       "band_panels": rep["band_panels"],

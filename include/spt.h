@@ -220,13 +220,28 @@ int spt_bem_upload(spt_ctx* ctx, const double* profile_rz, int32_t n_nodes, cons
 /* Distance-adaptive far rule for every later field gather on this context
  * (field evaluation, traces). No counterpart in the reference, whose far
  * rule is always the fixed sub-panel table of spt_bem_upload (reference
- * bem/axisymmetric.py:560-564): with k4 > 0, a segment at distance >= k4 L
- * from the point is summed by Gauss-4 over the whole segment, at >= k2 L by
- * Gauss-2; segments closer than that keep the fixed table, closer than L
- * the near rule. k4 = 0 restores the reference rule (the default). Needs
+ * bem/axisymmetric.py:560-564): with k4 > 0, a segment at distance D >= L
+ * from the point is summed over the whole segment by Gauss-8 below k4 L,
+ * Gauss-4 below k2 L and Gauss-2 beyond; closer than L the near rule
+ * applies as before. k4 = 0 restores the reference rule (the default). Needs
  * 1 <= k4 <= k2 (k2 = INFINITY: no Gauss-2). A field-error budget, for the
  * traces that state one, not a parity mode. */
 int spt_bem_far_rule(spt_ctx* ctx, double k4, double k2);
+
+/* Exterior multipole expansion of the uploaded BEM charge for every later
+ * field gather on this context (field evaluation, traces). No counterpart
+ * in the reference, which sums every segment at every point (reference
+ * bem/axisymmetric.py:560-605). moments[l] = sum over the quadrature nodes
+ * of (area * sigma)_q (r_q / r_ref)^l P_l(z_q / r_q), r_q = |(rho_q, z_q)|,
+ * for l < n_terms (host array); r_ref >= every r_q. Points at distance
+ * >= r_min from the origin take the perturbation
+ *   phi = (1/4 pi) sum_l moments[l] (r_ref / r)^l P_l(cos theta) / r
+ * (its field) instead of the segment sums. n_terms = 0 turns it off (the
+ * default); spt_bem_upload also turns it off. Needs 1 <= n_terms <= 1024,
+ * 0 < r_ref < r_min. A field-error budget, for the traces that state one
+ * (r_min and n_terms come from a validation against the segment sums), not
+ * a parity mode. */
+int spt_bem_multipole(spt_ctx* ctx, const double* moments, int32_t n_terms, double r_ref, double r_min);
 
 /* Collocation matrix of the P1 hat basis (reference
  * bem/axisymmetric.py:275-302,430-441), row-major into host A_out

@@ -2,11 +2,12 @@
 CPU with the oracle's ring kernel (the mathematics the GPU far rule relies
 on; the GPU gather itself is checked in test_gpu_far_rule.py).
 
-On the 400-panel post (reference-solved sigma): Gauss-4 over a whole
-segment at distance >= 2 L and Gauss-2 at >= 16 L change the field by
-< 1e-7 of |E| from the 8 x Gauss-4 sub-panel rule, while the midpoint
-rule at 16 L does not (> 1e-5): the choice of rule matters and the test
-would fail for a dropped Gauss node or a wrong weight.
+On the 400-panel post (reference-solved sigma): Gauss-8 over a whole
+segment at distance L <= D < 2 L, Gauss-4 at 2 L <= D < 16 L and Gauss-2
+beyond change the field by < 1e-7 of |E| from the 8 x Gauss-4 sub-panel
+rule, while the midpoint rule at 16 L does not (> 1e-5): the choice of
+rule matters and the test would fail for a dropped Gauss node or a wrong
+weight.
 """
 import numpy as np
 
@@ -49,27 +50,31 @@ def test_far_rule_against_fixed_rule_on_the_post():
     L = np.linalg.norm(np.diff(prof, axis=0), axis=1)
     r, zq, a, s = ob.fixed_quadrature(prof, sig, 8)
     fine = (r.reshape(n_seg, -1), zq.reshape(n_seg, -1), (a * s).reshape(n_seg, -1))
+    g8 = _segment_nodes(prof, sig, *np.polynomial.legendre.leggauss(8))
     g4 = _segment_nodes(prof, sig, G4X, G4W)
     g2 = _segment_nodes(prof, sig, np.array([-1.0, 1.0]) / np.sqrt(3.0), np.array([1.0, 1.0]))
     g1 = _segment_nodes(prof, sig, np.array([0.0]), np.array([2.0]))
     rng = np.random.default_rng(5)
     n = 240
     th = rng.uniform(0.0, np.deg2rad(100.0), n)
-    dd = np.exp(rng.uniform(np.log(0.1e-6), np.log(50e-6), n))
+    dd = np.exp(rng.uniform(np.log(0.1e-9), np.log(50e-6), n))
     rho, z = (R + dd) * np.sin(th), H + (R + dd) * np.cos(th)
     D = _seg_dist(prof, rho, z)
     fr, fz = _field_parts(fine, rho, z)
     far = D >= L[None]
     Et = np.hypot(np.where(far, fr, 0).sum(1), np.where(far, fz, 0).sum(1) + Ez)
+    m8 = far & (D < 2 * L[None])
     m4 = (D >= 2 * L[None]) & (D < 16 * L[None])
     m2 = D >= 16 * L[None]
+    c8r, c8z = _field_parts(g8, rho, z)
     c4r, c4z = _field_parts(g4, rho, z)
     c2r, c2z = _field_parts(g2, rho, z)
     c1r, c1z = _field_parts(g1, rho, z)
-    er = np.where(m4, c4r - fr, 0).sum(1) + np.where(m2, c2r - fr, 0).sum(1)
-    ez = np.where(m4, c4z - fz, 0).sum(1) + np.where(m2, c2z - fz, 0).sum(1)
+    er = sum(np.where(m, cr - fr, 0).sum(1) for m, cr in ((m8, c8r), (m4, c4r), (m2, c2r)))
+    ez = sum(np.where(m, cz - fz, 0).sum(1) for m, cz in ((m8, c8z), (m4, c4z), (m2, c2z)))
     err = np.hypot(er, ez) / Et
     assert err.max() < 1e-7
+    assert np.count_nonzero(m8) > 0  # every band is exercised
     assert np.count_nonzero(m2) > 0.5 * m2.size  # the coarse rules do most of the work
     mid = np.hypot(np.where(m2, c1r - fr, 0).sum(1), np.where(m2, c1z - fz, 0).sum(1)) / Et
     assert mid.max() > 1e-5

@@ -1,9 +1,9 @@
 """Distance-adaptive far rule (spt_bem_far_rule) on the GPU.
 
 The far rule is a field-error budget for the fast trace, not part of the
-plain method: a segment at distance >= k4 L from the point is summed by
-Gauss-4 over the whole segment, at >= k2 L by Gauss-2, instead of the
-reference's 8 x Gauss-4 sub-panel table. Checks:
+plain method: a segment at distance D >= L from the point is summed over
+the whole segment by Gauss-8 below k4 L, Gauss-4 below k2 L and Gauss-2
+beyond, instead of the reference's 8 x Gauss-4 sub-panel table. Checks:
 
 * off by default and after the context ends: the gather is bit-identical to
   the fixed rule;

@@ -1,0 +1,138 @@
+"""Exterior multipole expansion (spt_bem_multipole) on the GPU.
+
+A field-error budget for the fast trace, not part of the plain method:
+points at r >= r_min from the origin take the expansion of the BEM charge
+instead of the segment sums. Checks:
+
+* fit() on the 400-panel post finds r_min <= 1.3 r_ref at 1e-8 of |E|;
+* the device expansion equals the host formula (bem_amd.bem.multipole.
+  field, itself pinned to the oracle in test_multipole_math.py) to 1e-12;
+* points inside r_min, and every point once it is off, are bit-identical to
+  the segment sums;
+* a lockstep trace with the far rule and the expansion: every electron
+  crosses the screen, crossing times within 1e-6 relative of the fixed-rule
+  trace, and faster than the far rule alone;
+* bad parameters are rejected.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from bem_amd import _lib
+
+    _lib.load()
+    return torch
+
+
+def _say(*a):
+    print(*a, flush=True, file=sys.stderr)
+
+
+@pytest.fixture(scope="module")
+def post(gpu):
+    from bem_amd.bem import multipole as mpl
+    from bem_amd.bem.geometry import PostTipBEMGeometry
+
+    d = golden("bem_post400.npz")
+    geo = PostTipBEMGeometry(-1e8, float(d["R"]), float(d["H"]), n_segments=400, sigma=d["sigma"])
+    mp = mpl.fit(geo, n_terms=128, tol=1e-8, n_val=1 << 14)
+    rep = mp.report
+    _say(f"multipole: r_ref {rep['r_ref']:.4e} m, r_min {rep['r_min']:.4e} m ({rep['r_min_over_r_ref']:.3f} r_ref), "
+         f"field err beyond max {rep['field_err_max_beyond']:.2e} median {rep['field_err_median_beyond']:.2e}")
+    return geo, mp
+
+
+def test_fit_on_the_post(post):
+    geo, mp = post
+    assert not mp.inert
+    assert mp.r_min <= 1.3 * mp.r_ref
+    assert mp.report["field_err_max_beyond"] <= 1e-8
+
+
+def test_device_expansion_matches_host_formula(post):
+    from bem_amd.bem import multipole as mpl
+
+    geo, mp = post
+    rho, z, r = mpl.validation_points(geo, mp.r_min, 4096, 8 * mp.r_ref, seed=21)
+    with mp.active():
+        got = mpl._pert_field(geo, rho, z)
+    hr, hz = mpl.field(mp.m, mp.r_ref, rho, z)
+    scale = np.hypot(hr, hz).max()
+    assert np.max(np.abs(got[:, 0] - hr)) < 1e-12 * scale
+    assert np.max(np.abs(got[:, 2] - hz)) < 1e-12 * scale
+    assert np.all(got[:, 1] == 0.0)
+
+
+def test_inside_and_off_are_the_segment_sums(post):
+    from bem_amd.bem import multipole as mpl
+
+    geo, mp = post
+    rng = np.random.default_rng(4)
+    th = rng.uniform(0.0, 0.5 * np.pi, 4096)
+    r = mp.r_ref * rng.uniform(0.5, 2.0, 4096)
+    rho, z = r * np.sin(th), r * np.cos(th)
+    a = mpl._pert_field(geo, rho, z)
+    with mp.active():
+        b = mpl._pert_field(geo, rho, z)
+    c = mpl._pert_field(geo, rho, z)
+    inside = r < mp.r_min
+    assert inside.any() and (~inside).any()
+    assert np.array_equal(a, c)
+    assert np.array_equal(a[inside], b[inside])
+    assert not np.array_equal(a[~inside], b[~inside])
+
+
+def test_multipole_rejects_bad_parameters(post):
+    from bem_amd import _lib
+
+    geo, mp = post
+    ctx = geo._ctx()
+    m = mp.m
+    p = m.ctypes.data_as(_lib.ctypes.c_void_p)
+    assert ctx.lib.spt_bem_multipole(ctx.h, p, len(m), mp.r_ref, 0.5 * mp.r_ref) == _lib.SPT_EINVAL
+    assert ctx.lib.spt_bem_multipole(ctx.h, p, 2000, mp.r_ref, 2 * mp.r_ref) == _lib.SPT_EINVAL
+    assert ctx.lib.spt_bem_multipole(ctx.h, None, 0, 0.0, 0.0) == _lib.SPT_OK
+
+
+def test_multipole_trace(post):
+    """4096 electrons (256 groups of 16, rtol 1e-8, FP32 pairs) in lockstep:
+    fixed rule, far rule, far rule + expansion."""
+    from bem_amd import SpecificParticleTracer
+    from bem_amd.emission import cap_source
+    from bem_amd.particles import ParticleGroup
+
+    geo, mp = post
+    n = 4096
+    src = cap_source(n, geo.R, geo.H, seed=2)
+    pg = ParticleGroup(dict(src, weight=np.full(n, 1.602176634e-19), status=np.ones(n), species="electron"))
+    runs = {}
+    for name, fr, m in (("fixed", None, None), ("far", (2.0, 16.0), None), ("far+mp", (2.0, 16.0), mp)):
+        tr = SpecificParticleTracer(pg, 16, geo, [90e-6], rtol=1e-8, z_max=100e-6, mode="lockstep",
+                                    precision="pairs32", far_rule=fr, multipole=m)
+        tr.trace()  # warm-up (graph capture, first launches)
+        out = tr.trace()
+        runs[name] = out
+        _say(f"trace [{name}]: {out['device_ms']:.1f} ms, attempts mean {out['attempts'].mean():.2f}")
+    a = runs["fixed"]
+    ea = a["events"]
+    ka = np.lexsort((ea["particle"], ea["group"]))
+    for name in ("far", "far+mp"):
+        b = runs[name]
+        eb = b["events"]
+        assert len(eb) == len(ea) == n
+        kb = np.lexsort((eb["particle"], eb["group"]))
+        dt = np.abs(ea["t"][ka] - eb["t"][kb]) / ea["t"][ka]
+        _say(f"{name}: screen-time rel diff max {dt.max():.2e} median {np.median(dt):.2e}")
+        assert dt.max() < 1e-6
+    assert runs["far+mp"]["device_ms"] < runs["far"]["device_ms"]
