@@ -383,7 +383,7 @@ __device__ __forceinline__ void coarse_segment(double re, double re2, double ze,
 // combined by a fixed-pairing tree, so results are deterministic (they
 // depend on S, i.e. on the launch shape, in the last bits). m must be
 // block-uniform.
-template <int T, int NMAX, int PR>
+template <int T, int NMAX, int PR, bool FAR>
 __device__ void bem_pert_block(const BemView& B, int m, const double* px, const double* py, const double* pz,
                                double* ox, double* oy, double* oz, BemShared<T, NMAX>& sh) {
   static_assert(T % 64 == 0, "whole wavefronts");
@@ -469,16 +469,31 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
         const double L = g[S_L];
         double proj, dist;
         segment_proj(re, ze, g, proj, dist);
-        if (dist < L) {
+        if (FAR) {
+          // Far rule: one rule per wavefront, the finest any of its lanes
+          // needs (a finer rule than a lane's own only adds accuracy), so
+          // the node loop is a uniform branch with scalar table loads.
+          const bool near = dist < L;
+          if (near) {
+            if (n_near < kNearSlots)
+              sh.u.n.seg[n_near++][tid] = j;
+            else
+              acc.add(near_segment(re, re2, ze, B.segs + (size_t)j * SEG_STRIDE, B.sigma[j], B.sigma[j + 1], proj,
+                                   dist, sh.logt));
+          }
+          const int rl = near ? 3 : (dist >= B.k2 * L ? 2 : (dist >= B.k4 * L ? 1 : 0));
+          const unsigned long long b0 = __ballot(rl == 0), b1 = __ballot(rl == 1), b2 = __ballot(rl == 2);
+          const bool axw =
+              __ballot(!near && ring_axis_possible(re2, ze, g[S_P0R], g[S_P0Z], g[S_ABR], g[S_ABZ])) != 0ull;
+          if ((b0 | b1 | b2) && !near)
+            coarse_segment(re, re2, ze, (cnode_ptr)B.coarse + (size_t)j * kCoarsePer, b0 ? 0 : (b1 ? 1 : 2), axw,
+                           acc, sh.logt);
+        } else if (dist < L) {
           if (n_near < kNearSlots)
             sh.u.n.seg[n_near++][tid] = j;
           else
             acc.add(near_segment(re, re2, ze, B.segs + (size_t)j * SEG_STRIDE, B.sigma[j], B.sigma[j + 1], proj,
                                  dist, sh.logt));
-        } else if (B.coarse) {
-          coarse_segment(re, re2, ze, (cnode_ptr)B.coarse + (size_t)j * kCoarsePer,
-                         dist >= B.k2 * L ? 2 : (dist >= B.k4 * L ? 1 : 0),
-                         ring_axis_possible(re2, ze, g[S_P0R], g[S_P0Z], g[S_ABR], g[S_ABZ]), acc, sh.logt);
         } else if (PR == SPT_PREC_MIXED && dist >= 2.0 * L) {
           const cnode_ptr nd = (cnode_ptr)B.nodes + (size_t)j * B.nq;
 #pragma unroll 1
@@ -532,7 +547,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
             else
               acc.add(near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, sh.logt));
           }
-        } else if (B.coarse) {
+        } else if (FAR) {
           if (sub == 0)
             coarse_segment(re, re2, ze, B.coarse + (size_t)j * kCoarsePer,
                            dist >= B.k2 * L ? 2 : (dist >= B.k4 * L ? 1 : 0),
@@ -635,6 +650,19 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
     oz[p] = ez;
   }
   __syncthreads();
+}
+
+// The gather with the reference's fixed rule, or with the far rule when the
+// view carries its coarse table (two instantiations, so the fixed rule's
+// code is unchanged by the far rule's).
+template <int T, int NMAX, int PR>
+__device__ __forceinline__ void bem_pert_any(const BemView& B, int m, const double* px, const double* py,
+                                             const double* pz, double* ox, double* oy, double* oz,
+                                             BemShared<T, NMAX>& sh) {
+  if (B.coarse)
+    bem_pert_block<T, NMAX, PR, true>(B, m, px, py, pz, ox, oy, oz, sh);
+  else
+    bem_pert_block<T, NMAX, PR, false>(B, m, px, py, pz, ox, oy, oz, sh);
 }
 
 // ------------------------------------------------------ geometry helpers
@@ -1118,7 +1146,7 @@ __device__ void stage_field(const TraceParams& P, int64_t off, int n, int st, Tr
     sh.cz[tid] = sh.sz[i];
   }
   __syncthreads();
-  bem_pert_block<T, NMAX, PR>(P.bem, m, sh.cx, sh.cy, sh.cz, sh.ox, sh.oy, sh.oz, sh.bem);
+  bem_pert_any<T, NMAX, PR>(P.bem, m, sh.cx, sh.cy, sh.cz, sh.ox, sh.oy, sh.oz, sh.bem);
   if (tid < m) {
     const int i = sh.cidx[tid];
     geometry_field_point(P.G, sh.sx[i], sh.sy[i], sh.sz[i], sh.ox[tid], sh.oy[tid], sh.oz[tid], sh.fx[i], sh.fy[i],
@@ -1350,26 +1378,12 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void bem_pert_kernel(BemVie
   const int64_t base = (int64_t)blockIdx.x * PB;
   const int m_all = (int)min((int64_t)PB, n - base);
   const int tid = threadIdx.x;
-  if (!B.mp) {
-    if (tid < m_all) {
-      sh.px[tid] = pos[3 * (base + tid)];
-      sh.py[tid] = pos[3 * (base + tid) + 1];
-      sh.pz[tid] = pos[3 * (base + tid) + 2];
-    }
-    __syncthreads();
-    bem_pert_block<T, NMAX, PR>(B, m_all, sh.px, sh.py, sh.pz, sh.ox, sh.oy, sh.oz, sh.bem);
-    if (tid < m_all) {
-      E[3 * (base + tid)] = sh.ox[tid];
-      E[3 * (base + tid) + 1] = sh.oy[tid];
-      E[3 * (base + tid) + 2] = sh.oz[tid];
-    }
-    return;
-  }
-  // Points in the multipole zone are done here; the rest go to the gather.
+  // Points in the multipole zone (if any) are done here; the rest go to
+  // the gather in their original order.
   if (tid < m_all) {
     const double x = pos[3 * (base + tid)], y = pos[3 * (base + tid) + 1], z = pos[3 * (base + tid) + 2];
     double qx, qy, qz;
-    const bool far = multipole_pert(B, x, y, z, qx, qy, qz);
+    const bool far = B.mp && multipole_pert(B, x, y, z, qx, qy, qz);
     if (far) {
       E[3 * (base + tid)] = qx;
       E[3 * (base + tid) + 1] = qy;
@@ -1393,7 +1407,7 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void bem_pert_kernel(BemVie
     sh.pz[tid] = pos[3 * gi + 2];
   }
   __syncthreads();
-  bem_pert_block<T, NMAX, PR>(B, m, sh.px, sh.py, sh.pz, sh.ox, sh.oy, sh.oz, sh.bem);
+  bem_pert_any<T, NMAX, PR>(B, m, sh.px, sh.py, sh.pz, sh.ox, sh.oy, sh.oz, sh.bem);
   if (tid < m) {
     const int64_t gi = base + sh.cidx[tid];
     E[3 * gi] = sh.ox[tid];
@@ -1446,7 +1460,7 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void geometry_field_kernel(
     sh.pz[tid] = sz[i];
   }
   __syncthreads();
-  bem_pert_block<T, NMAX, PR>(B, m, sh.px, sh.py, sh.pz, sh.ox, sh.oy, sh.oz, sh.bem);
+  bem_pert_any<T, NMAX, PR>(B, m, sh.px, sh.py, sh.pz, sh.ox, sh.oy, sh.oz, sh.bem);
   if (tid < m) {
     const int i = sh.cidx[tid];
     double ex, ey, ez;
@@ -1658,7 +1672,7 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_field_kernel(LockPa
     sh.pz[tid] = sz[i];
   }
   __syncthreads();
-  bem_pert_block<T, NMAX, PR>(L.T.bem, m, sh.px, sh.py, sh.pz, sh.ox, sh.oy, sh.oz, sh.bem);
+  bem_pert_any<T, NMAX, PR>(L.T.bem, m, sh.px, sh.py, sh.pz, sh.ox, sh.oy, sh.oz, sh.bem);
   if (tid < m) {
     const int i = sh.cidx[tid];
     const int64_t gi = base + i;

@@ -287,6 +287,9 @@ def main():
                     help="only the field-gather timing and a 256 MiB device copy (for hardware-counter runs)")
     ap.add_argument("--field-shape", default="0,0",
                     help="threads,points-per-block of the field gather for --field-only (0,0 = automatic)")
+    ap.add_argument("--value-only", action="store_true",
+                    help="only the headline trace (far rule and multipole zone, no network: the network is inert "
+                         "at 1e-6) after one warm-up trace, for profiler runs")
     ap.add_argument("--no-multipole", action="store_true",
                     help="no exterior multipole zone in the surrogate + batched traces")
     ap.add_argument("--far-rule", default="2,16",
@@ -326,6 +329,16 @@ def main():
         print(json.dumps(dict(field_ms=field_ms, points=n_pts, field_shape=shape, far_rule=far_rule,
                               copy_bytes_read=src.numel() * 4,
                               copy_bytes_written=dst.numel() * 4)))
+        return
+    if args.value_only:
+        from bem_amd.bem import multipole as mpl
+
+        mp = None if args.no_multipole else mpl.fit(geo, n_terms=128, tol=min(1e-8, args.tol))
+        vt = make_tracer(wl, geo, far_rule=far_rule, multipole=mp)
+        vt.field_shape = (256, 64)
+        o, w = timed_traces(vt, args.steps, args.warmup)
+        print(json.dumps(dict(value=sum(x["particle_stages"] for x in o) / w, wall_s=w, steps=args.steps,
+                              far_rule=far_rule, multipole_r_min=None if mp is None else mp.r_min)))
         return
     say(f"bench: BEM solve {t_solve:.2f} s; training the surrogate ({args.train_steps} steps)")
     sur = surrogate.train(geo, n_samples=1 << 20, steps=args.train_steps, batch=8192, tol=args.tol, chunk=10000)
