@@ -310,8 +310,20 @@ __device__ __forceinline__ void ring_accum32(double rho, double rho2, double zz_
 }
 
 // ---- block reductions (deterministic: fixed tree over thread index) ----
+// One-wavefront blocks (T = 64) reduce with shuffles in the same pairing
+// (lane i takes lane i + s, s = 32 .. 1, the result is lane 0's), so the
+// sum is bit-identical to the LDS tree, without its seven LDS round trips;
+// the barriers at both ends keep the tree's ordering of other LDS traffic.
 template <int T>
 __device__ double block_sum(double v, double* red) {
+  if constexpr (T == 64) {
+    __syncthreads();
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) v = v + __shfl_down(v, s, 64);
+    const double r = __shfl(v, 0, 64);
+    __syncthreads();
+    return r;
+  }
   const int tid = threadIdx.x;
   red[tid] = v;
   __syncthreads();
@@ -327,6 +339,14 @@ __device__ double block_sum(double v, double* red) {
 
 template <int T>
 __device__ double block_min(double v, double* red) {
+  if constexpr (T == 64) {
+    __syncthreads();
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) v = fmin(v, __shfl_down(v, s, 64));
+    const double r = __shfl(v, 0, 64);
+    __syncthreads();
+    return r;
+  }
   const int tid = threadIdx.x;
   red[tid] = v;
   __syncthreads();

@@ -310,7 +310,7 @@ def main():
 
     wl = workload(n_el=args.electrons)
     cpu = None
-    if not args.no_cpu_baseline and not args.field_only:
+    if not args.no_cpu_baseline and not args.field_only and not args.value_only:
         say(f"bench: CPU baseline (oracle, {args.cpu_seconds:.0f} s per process)")
         cpu = cpu_baseline(wl, args.cpu_seconds)  # before the GPU is touched: the workers are forked
         say(f"bench: CPU baseline {cpu['value']:.0f} particle-stages/s on {cpu['cores']} cores")
