@@ -43,7 +43,7 @@ METRIC = "particle-RK-stages/sec, 1 GPU, surrogate+batched vs plain at field err
 PEAK_FP64_TFLOPS = 78.6
 C_RING = 120          # FP64 flops per ring-kernel node (K, E, log, kernel algebra, accumulation)
 TRAFFIC_BYTES = 19.8e6  # HBM bytes per 262,144-point field-gather launch at (256, 64), from the rocprofv3 counter runs
-FAR_TRAFFIC_BYTES = None  # the same with the far rule (2, 16)
+FAR_TRAFFIC_BYTES = 13.9e6  # the same with the far rule (2, 16)
 
 
 def workload(n_el=262144, group=16, n_segments=400, seed=0, rtol=1e-8):
