@@ -10,20 +10,26 @@ cos-weighted, numpy default_rng(0)), RK45 rtol 1e-8, atol 1e-12, screen at
 90 um; FP64 field and step control, FP32 + Kahan pair forces; lockstep
 schedule (every running group one attempt per graph-replayed iteration).
 
-The surrogate (32 Fourier features, 4 x 128 SiLU, outputs phi, E_rho, E_z)
-is trained on the device before the timed region (2^20 BEM samples,
-log-uniform in distance from the surface), validated against the BEM on
-fresh points, and used only beyond the distance band where its worst
-field error relative to |E| is <= 1e-6 (and never within 3 panel lengths).
-The plain method (no surrogate) is timed on the same workload; so are the
-surrogate bands for looser tolerances (1e-4, 1e-3).
+The surrogate + batched form spends the 1e-6 field budget on three
+approximations of the BEM field, each validated against the device's
+fixed-rule BEM on fresh points before the timed region: the exterior
+multipole expansion of the BEM charge beyond its validated radius (1e-8),
+the distance-adaptive far rule (Gauss-8/4/2 per segment) in every remaining
+gather, and the learned surrogate (32 Fourier features, 4 x 128 SiLU,
+outputs phi, E_rho, E_z; trained on the device on 2^20 BEM samples) beyond
+the distance band where its worst field error relative to |E| is <= 1e-6
+(none at 1e-6: inert). The plain method (the reference's rule everywhere)
+is timed on the same workload; so are the same fast trace without the
+network and the network's bands for looser tolerances (1e-4, 1e-3).
 
 value = 7 x sum_g(size_g x attempts_g) / wall time (all particle slots of
 running groups, the reference's accel_fn shape). The roofline entry is for
-the dominant kernel, the BEM field gather (99% of device time in a plain
-trace): algorithmic FP64 flops = points x 12,800 nodes x 120 flops per
-node, over its average launch time from HIP events on the launch stream,
-on 262,144 points 10 nm .. 10 um off the cap. cpu_baseline: the oracle
+the dominant kernel of the value trace, the far-rule BEM field gather:
+algorithmic FP64 flops = points x ring nodes the far rule sums (counted on
+the host for the same points) x 120 flops per node, over its average
+launch time from HIP events on the launch stream, on 262,144 points
+10 nm .. 10 um off the cap; roofline_plain_gather gives the same for the
+fixed rule (12,800 nodes per point). cpu_baseline: the oracle
 (NumPy FP64 restatement of the reference) on a time-bounded sample of the
 same workload, one forked process per host core (up to 16), run before the
 GPU is touched.
