@@ -936,9 +936,10 @@ struct ScreenParams {
 // Screen crossings of one particle over one accepted step: cubic Hermite
 // z(s), split at its real extrema, half-open (t_old, t_new] sign test on
 // moving particles, 60-step bisection (reference screens.py:43-168).
-__device__ __noinline__ void screens_particle(const ScreenParams& SP, spt_event* events, int64_t cap, unsigned long long* count,
-                                 int group, int particle, int64_t batch, double t0, double t1, const double* p0,
-                                 const double* v0, const double* p1, const double* v1) {
+__device__ __forceinline__ void screens_particle_body(const ScreenParams& SP, spt_event* events, int64_t cap,
+                                                     unsigned long long* count, int group, int particle, int64_t batch,
+                                                     double t0, double t1, const double* p0, const double* v0,
+                                                     const double* p1, const double* v1) {
   if (SP.n == 0) return;
   const double dt = t1 - t0;
   const bool moving = p0[0] != p1[0] || p0[1] != p1[1] || p0[2] != p1[2] || v0[0] != v1[0] || v0[1] != v1[1] ||
@@ -1004,6 +1005,16 @@ __device__ __noinline__ void screens_particle(const ScreenParams& SP, spt_event*
       }
     }
   }
+}
+
+// Out of line for the persistent kernel (its register budget); the
+// lockstep end kernel inlines the body instead (an out-of-line call there
+// spilled the caller's registers to 1.6 KB of scratch per lane).
+__device__ __noinline__ void screens_particle(const ScreenParams& SP, spt_event* events, int64_t cap,
+                                              unsigned long long* count, int group, int particle, int64_t batch,
+                                              double t0, double t1, const double* p0, const double* v0,
+                                              const double* p1, const double* v1) {
+  screens_particle_body(SP, events, cap, count, group, particle, batch, t0, t1, p0, v0, p1, v1);
 }
 
 // ------------------------------------------------------------ trace kernel
@@ -1904,7 +1915,7 @@ __global__ __launch_bounds__(TG) void ls_end_kernel(LockParams L) {
         L.fvalid[gi] = 1;
       }
       if (L.active[gi] && kill_point(L.T.G, x1[0], x1[1], x1[2])) L.T.t_death[gi] = t;
-      screens_particle(L.T.SP, L.T.events, L.T.ev_cap, L.T.ev_count, g, tid, batch, t_old, t, x0, v0, x1, v1);
+      screens_particle_body(L.T.SP, L.T.events, L.T.ev_cap, L.T.ev_count, g, tid, batch, t_old, t, x0, v0, x1, v1);
       if (!oexh && t >= nout - tol && L.T.t_birth[gi] <= t) record_snapshot(L.T, optr, g, tid, batch, t, x1, v1);
     }
     running = t < L.T.t_max * (1 - 1e-14);
