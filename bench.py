@@ -371,7 +371,11 @@ def main():
         if mp.inert:
             mp = None
     tr = make_tracer(wl, geo, surrogate=sur, far_rule=far_rule, multipole=mp)
-    tr.field_shape = plain_tr.field_shape
+    tuning_value = autotune.tune(tr, attempts=4) if (far_rule or mp) else None  # its own launch shape
+    if tuning_value is None:
+        tr.field_shape = plain_tr.field_shape
+    else:
+        say(f"bench: autotuned field shape of the fast trace {tuning_value['best']}")
     outs, wall = timed_traces(tr, args.steps, args.warmup)
     say(f"bench: {args.steps} surrogate traces in {wall:.2f} s")
     stages = sum(o["particle_stages"] for o in outs)
@@ -382,7 +386,7 @@ def main():
     net_off = None  # the same batched trace without the network: what the network itself gains
     if far_rule or mp:
         t3 = make_tracer(wl, geo, far_rule=far_rule, multipole=mp)
-        t3.field_shape = plain_tr.field_shape
+        t3.field_shape = tr.field_shape
         o3, w3 = timed_traces(t3, 1, 0)
         net_off = dict(value=o3[0]["particle_stages"] / w3, wall_s=w3,
                        attempts_mean=float(o3[0]["attempts"].mean()),
@@ -397,7 +401,7 @@ def main():
                 sweep.append(dict(tol=tol, band_m=None))
                 continue
             t2 = make_tracer(wl, geo, surrogate=s, far_rule=far_rule, multipole=mp)
-            t2.field_shape = plain_tr.field_shape
+            t2.field_shape = tr.field_shape
             o, w = timed_traces(t2, 1, 0)
             sweep.append(dict(tol=tol, band_m=s.cfg["band"], value=o[0]["particle_stages"] / w,
                               wall_s=w, gain_wall_vs_plain=plain_wall / w, network_gain_wall=base_wall / w,
@@ -409,7 +413,7 @@ def main():
     plain_achieved = n_pts * n_quad * C_RING / (field_ms / 1e3) / 1e12
     far_gather = None
     if far_rule:
-        far_ms, _ = field_gather_timing(wl, geo, shape=plain_tr.field_shape, far_rule=far_rule)
+        far_ms, _ = field_gather_timing(wl, geo, shape=tr.field_shape, far_rule=far_rule)
         npp = far_rule_nodes_per_point(geo, gather_points(wl, n_pts), *far_rule)
         far_gather = dict(kernel_ms_mean=far_ms, nodes_per_point_mean=npp, points_per_launch=n_pts,
                           flops_per_launch=n_pts * npp * C_RING)
@@ -432,6 +436,7 @@ def main():
                     "kernel": f"bem_pert_kernel, far rule (k4, k2) = {far_rule} (the same device code as the "
                               "value trace's ls_field_kernel)",
                     "kernel_ms_mean": far_gather["kernel_ms_mean"], "points_per_launch": n_pts,
+                    "field_shape": list(tr.field_shape or (0, 0)), "traffic_field_shape": [256, 64],
                     "nodes_per_point_mean": far_gather["nodes_per_point_mean"],
                     "flops_per_launch": far_gather["flops_per_launch"],
                     "note": "algorithmic flops = ring-kernel nodes the far rule sums (counted on the host for the "
@@ -455,7 +460,7 @@ def main():
                            "multipole expansion of the BEM charge beyond its radius, learned network beyond its "
                            f"band (inert if none), BEM gathers with the far rule {far_rule} (fixed rule if None) "
                            "elsewhere; FP64 field and step control, FP32 + Kahan pair forces",
-                   "n_quad": n_quad, "field_shape": list(plain_tr.field_shape or (0, 0))},
+                   "n_quad": n_quad, "field_shape": list(tr.field_shape or (0, 0)), "plain_field_shape": list(plain_tr.field_shape or (0, 0))},
         "plain_value": plain_value,
         "gain_vs_plain": value / plain_value,
         "far_rule": None if not far_rule else dict(far_err, screen_t_rel_diff_vs_plain=screen_time_diff(
@@ -473,6 +478,8 @@ def main():
                       "screen_t_rel_diff_vs_plain": screen_time_diff(plain_outs[0], outs[-1])},
         "tolerance_sweep": sweep,
         "autotune": {"best": tuning["best"], "table": tuning["table"], "sample_attempts": tuning["attempts"]},
+        "autotune_value": None if tuning_value is None else {"best": tuning_value["best"],
+                                                             "table": tuning_value["table"]},
         "attempts_per_group_mean": float(np.mean(outs[-1]["attempts"])),
         "plain_attempts_per_group_mean": float(np.mean(plain_outs[0]["attempts"])),
         "active_particle_stages_per_s": act / wall,
