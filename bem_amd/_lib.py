@@ -9,6 +9,7 @@ There is no CPU fallback: if the library or a GPU is missing, every entry
 point raises.
 """
 import ctypes
+import itertools
 import os
 import threading
 
@@ -21,7 +22,7 @@ SPT_OK, SPT_EINVAL, SPT_ENOMEM, SPT_EHIP, SPT_ENOBEM, SPT_ESTEPCAP = 0, -1, -2, 
 SPT_MAX_SCREENS = 16
 GEOM_FLAT, GEOM_TIP, GEOM_BEM_POST, GEOM_BEM_WELL = 0, 1, 2, 3
 IMAGE_NONE, IMAGE_PLANE, IMAGE_TIP3, IMAGE_BEM, IMAGE_BEM_MIRROR = 0, 1, 2, 3, 4
-STATUS_DONE, STATUS_STEPCAP = 0, 1
+STATUS_DONE, STATUS_STEPCAP, STATUS_STALLED = 0, 1, 2
 MODE_PERSISTENT, MODE_LOCKSTEP = 0, 1
 PREC_FP64, PREC_MIXED, PREC_PAIR32 = 0, 1, 2
 
@@ -94,11 +95,23 @@ class TraceArgs_t(ctypes.Structure):
         ("events", _p), ("event_capacity", ctypes.c_int64), ("event_count", _p),
         ("t_out", _p), ("n_out", ctypes.c_int32), ("pad1", ctypes.c_int32),
         ("snapshots", _p), ("snapshot_capacity", ctypes.c_int64), ("snapshot_count", _p),
+        ("t_end", _p),
     ]
 
 
 _lib = None
 _lock = threading.Lock()
+_tokens = itertools.count(1)
+
+
+def token(obj):
+    """A process-unique number for obj, never reused (unlike id(), which
+    CPython hands to the next object at the same address once obj is freed):
+    the key under which a context remembers which object's tables it holds."""
+    t = obj.__dict__.get("_spt_token")
+    if t is None:
+        t = obj.__dict__["_spt_token"] = next(_tokens)
+    return t
 
 
 def load():

@@ -53,7 +53,7 @@ class _AxisymmetricBEMGeometry(geometry_module.Geometry):
     def _ctx(self):
         ctx = super()._ctx()
         if self.image_profile is not None:
-            key = ("image", id(self), self.image_profile.shape[0], self.image_n_max)
+            key = ("image", _lib.token(self), self.image_profile.shape[0], self.image_n_max)
             if getattr(ctx, "image_key", None) != key:
                 prof, pp = _lib.host_ptr(np.asarray(self.image_profile, dtype=np.float64))
                 ctx.check(ctx.lib.spt_image_upload(ctx.h, pp, prof.shape[0], self.image_n_max, 4,
@@ -71,7 +71,7 @@ class _AxisymmetricBEMGeometry(geometry_module.Geometry):
         _, ap = _lib.host_ptr(A)
         ctx.check(ctx.lib.spt_image_upload(ctx.h, pp, P, self.image_n_max, 4, self.image_d_lo * self.R,
                                            self.image_d_hi * self.R, ap), "spt_image_upload")
-        ctx.image_key = ("image", id(self), P, self.image_n_max)
+        ctx.image_key = ("image", _lib.token(self), P, self.image_n_max)
         return A
 
     def spt_geometry(self, z_max=None):
@@ -252,8 +252,9 @@ class CylindricalWellBEMGeometry(_AxisymmetricBEMGeometry):
 
     def __init__(self, E_gun, R, H, field_plane_radius=None, field_max_length=None, n_subdiv=8,
                  bottom_fillet_radius=None, rim_fillet_radius=None, z0=None, image_plane_radius=None,
-                 image_max_length=None, image_n_max=16, image_d_lo=0.1, image_d_hi=0.5, z_handoff=None,
-                 sigma=None, device=0):
+                 image_max_length=None, image_bottom_max_length=None, image_fillet_max_length=None,
+                 image_wall_max_length=None, image_rim_fillet_max_length=None, image_plane_max_length=None,
+                 image_n_max=16, image_d_lo=0.1, image_d_hi=0.5, z_handoff=None, sigma=None, device=0):
         self.z0 = None if z0 is None else float(z0)
         self.H = float(H)
         R = float(R)
@@ -274,6 +275,11 @@ class CylindricalWellBEMGeometry(_AxisymmetricBEMGeometry):
         self._setup(E_gun, R, profile, n_subdiv, sigma, zh, None, device)
         self.image_plane_radius = 5.0 * max(R, self.H) if image_plane_radius is None else float(image_plane_radius)
         self.image_max_length = min(R, self.H) / 14.0 if image_max_length is None else float(image_max_length)
+        self.image_bottom_max_length = image_bottom_max_length
+        self.image_fillet_max_length = image_fillet_max_length
+        self.image_wall_max_length = image_wall_max_length
+        self.image_rim_fillet_max_length = image_rim_fillet_max_length
+        self.image_plane_max_length = image_plane_max_length
         self.image_n_max = int(image_n_max)
         self.image_d_lo = float(image_d_lo)
         self.image_d_hi = float(image_d_hi)
@@ -281,7 +287,10 @@ class CylindricalWellBEMGeometry(_AxisymmetricBEMGeometry):
         if self.z0 is not None:
             self.image_profile = cylindrical_well_image_profile(
                 R, self.H, self.z0, self.image_plane_radius, self.image_max_length,
-                bottom_fillet_radius=bottom_fillet_radius, rim_fillet_radius=rim_fillet_radius)
+                bottom_fillet_radius=bottom_fillet_radius, rim_fillet_radius=rim_fillet_radius,
+                bottom_max_length=image_bottom_max_length, bottom_fillet_max_length=image_fillet_max_length,
+                wall_max_length=image_wall_max_length, rim_fillet_max_length=image_rim_fillet_max_length,
+                plane_max_length=image_plane_max_length)
             self._flat_fallback = geometry_module.FlatCathode(E_gun, z0=self.z0, kill_z_below=None, device=device)
 
     def spt_geometry(self, z_max=None):
@@ -292,16 +301,23 @@ class CylindricalWellBEMGeometry(_AxisymmetricBEMGeometry):
         return g
 
     def worker_args(self):
+        """The reference's 21-element tuple (reference bem/geometry.py:814-836),
+        field for field."""
         return ("cylindrical_well_bem", self.E_gun, self.R, self.H, self.field_plane_radius, self.field_max_length,
                 self.n_subdiv, self.bottom_fillet_radius, self.rim_fillet_radius, self.z0, self.image_plane_radius,
-                self.image_max_length, self.image_n_max, self.image_d_lo, self.image_d_hi, self.z_handoff)
+                self.image_max_length, self.image_bottom_max_length, self.image_fillet_max_length,
+                self.image_wall_max_length, self.image_rim_fillet_max_length, self.image_plane_max_length,
+                self.image_n_max, self.image_d_lo, self.image_d_hi, self.z_handoff)
 
     @classmethod
     def _from_worker_args(cls, args):
-        (E_gun, R, H, fpr, fml, n_subdiv, bf, rf, z0, ipr, iml, inm, idl, idh, z_handoff) = args
+        (E_gun, R, H, fpr, fml, n_subdiv, bf, rf, z0, ipr, iml, ibml, ifml, iwml, irml, ipml, inm, idl, idh,
+         z_handoff) = args
         return cls(E_gun, R, H, field_plane_radius=fpr, field_max_length=fml, n_subdiv=n_subdiv,
                    bottom_fillet_radius=bf, rim_fillet_radius=rf, z0=z0, image_plane_radius=ipr,
-                   image_max_length=iml, image_n_max=inm, image_d_lo=idl, image_d_hi=idh, z_handoff=z_handoff)
+                   image_max_length=iml, image_bottom_max_length=ibml, image_fillet_max_length=ifml,
+                   image_wall_max_length=iwml, image_rim_fillet_max_length=irml, image_plane_max_length=ipml,
+                   image_n_max=inm, image_d_lo=idl, image_d_hi=idh, z_handoff=z_handoff)
 
 
 geometry_module._WORKER_REGISTRY["cylindrical_well_bem"] = CylindricalWellBEMGeometry

@@ -152,6 +152,10 @@ def fit(geometry, n_terms=128, tol=1e-8, n_val=1 << 16, r_max=None, seed=13):
     """Moments of the geometry's BEM charge and the smallest r_min (from
     r_ref (1.01 .. 4)) beyond which the device expansion's field is within
     tol of |E| from the device segment sums on fresh points."""
+    if geometry.bem_table() is None or geometry.spt_geometry().kind != _lib.GEOM_BEM_POST:
+        raise ValueError("the exterior multipole zone is fitted for protrusions on the cathode (the post and the "
+                         "hemispherical tip BEM geometries): its validation points are chosen with their exterior "
+                         "test, and a well's profile runs out along the plane, leaving no exterior zone")
     prof, sig, nsub = geometry.bem_table()
     m, r_ref = moments(prof, sig, nsub, n_terms)
     r_max = 10.0 * r_ref if r_max is None else r_max

@@ -46,7 +46,7 @@ class AxisymmetricBEMSolution:
 
     def _upload(self, n_subdiv):
         ctx = _lib.Context.get(self.device)
-        key = ("sol", id(self), self.profile.shape[0], n_subdiv)
+        key = ("sol", _lib.token(self), self.profile.shape[0], n_subdiv)
         if ctx.bem_key != key:
             _, pp = _lib.host_ptr(self.profile)
             _, sp = _lib.host_ptr(self.sigma)

@@ -61,6 +61,17 @@ constexpr double kVacuumPermittivity = 8.8541878188e-12;
 constexpr double kCoulomb = 1.0 / (4.0 * kPi * kVacuumPermittivity);
 constexpr double kLight = 299792458.0;
 
+// A group stalls when an attempt is rejected and the next step is not above
+// kStallFraction |t| (zero and NaN steps included): the controller can no
+// longer advance it. Rejections shrink h by at least min_factor, so a
+// stuck group reaches this in a bounded number of attempts instead of
+// spinning to max_steps (the reference's only exit,
+// batched_rk45.py:320-321).
+constexpr double kStallFraction = 1e-14;
+__device__ __forceinline__ bool step_stalled(bool accepted, double h_next, double t) {
+  return !accepted && !(h_next > kStallFraction * fabs(t));
+}
+
 // ---------------------------------------------------------------- tables
 enum { S_P0R = 0, S_P0Z, S_ABR, S_ABZ, S_ABAB, S_L, S_UR, S_UZ, S_NR, S_NZ, SEG_STRIDE = 12 };
 
@@ -1049,6 +1060,7 @@ struct TraceParams {
   spt_event* snaps;
   int64_t snap_cap;
   unsigned long long* snap_count;
+  double* t_end;  // per group: t after the last attempt, or nullptr
 };
 
 // Output times at or before t (the ascending list's count below t's
@@ -1338,9 +1350,14 @@ __device__ void run_group(const TraceParams& P, int g, TraceShared<T, NMAX>& sh)
       }
       running = t < P.t_max * (1 - 1e-14);
     }
+    if (step_stalled(acc, h, t)) {
+      status = SPT_STATUS_STALLED;
+      break;
+    }
   }
   if (mine) P.t_death[gi] = td;
   if (tid == 0) {
+    if (P.t_end) P.t_end[g] = t;
     P.attempts[g] = attempts;
     P.accepted[g] = nacc;
     P.active_stages[g] = act_st;
@@ -1601,6 +1618,7 @@ __global__ __launch_bounds__(TG) void ls_init_kernel(LockParams L) {
   if (tid == 0) {
     L.g_t[g] = t;
     L.g_h[g] = h;
+    if (L.T.t_end) L.T.t_end[g] = t;
     L.T.attempts[g] = 0;
     L.T.accepted[g] = 0;
     L.T.active_stages[g] = 0;
@@ -1920,6 +1938,8 @@ __global__ __launch_bounds__(TG) void ls_end_kernel(LockParams L) {
     }
     running = t < L.T.t_max * (1 - 1e-14);
   }
+  const bool stalled = step_stalled(acc, h, t);
+  if (stalled) running = false;
   __syncthreads();
   if (tid == 0) {
     L.T.attempts[g] = batch + 1;
@@ -1928,6 +1948,8 @@ __global__ __launch_bounds__(TG) void ls_end_kernel(LockParams L) {
     L.T.active_stages[g] += 7 * (int64_t)L.g_nact[g];
     L.g_t[g] = t;
     L.g_h[g] = h;
+    if (L.T.t_end) L.T.t_end[g] = t;
+    if (stalled) L.T.status[g] = SPT_STATUS_STALLED;
     if (!running) {
       L.g_flags[g] = 0;
       atomicAdd(L.done, 1u);
@@ -2945,6 +2967,7 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
   P.snaps = a->snapshots;
   P.snap_cap = a->snapshot_capacity;
   P.snap_count = (unsigned long long*)a->snapshot_count;
+  P.t_end = a->t_end;
   if (a->mode == SPT_MODE_LOCKSTEP) {
     int rc2 = ensure_lockstep_scratch(c, a->n_particles, a->n_groups);
     if (rc2) return rc2;

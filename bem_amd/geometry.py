@@ -57,7 +57,7 @@ class Geometry(ABC):
         ctx = _lib.Context.get(getattr(self, "device", 0))
         tab = self.bem_table()
         if tab is not None:
-            key = (id(self), tab[0].shape[0], tab[2])
+            key = (_lib.token(self), tab[0].shape[0], tab[2])
             if ctx.bem_key != key:
                 prof, sig, nsub = tab
                 prof_c, pp = _lib.host_ptr(np.asarray(prof, dtype=np.float64))
@@ -245,6 +245,27 @@ _WORKER_REGISTRY = {
     "flat": FlatCathode,
     "hemispherical_tip": HemisphericalTip,
 }
+
+
+def lower(geometry):
+    """This package's geometry for a reference Geometry object, or for its
+    ``worker_args()`` tuple: the tuple's first element names the class and
+    the rest are its constructor arguments, field for field (reference
+    geometry.py:61-94; the reference tracker rebuilds every geometry this
+    way, tracker.py:199-202). A geometry of this package is returned as is.
+    Raises ValueError for a kind this backend does not implement."""
+    if isinstance(geometry, Geometry):
+        return geometry
+    args = geometry.worker_args() if hasattr(geometry, "worker_args") else geometry
+    if not isinstance(args, (tuple, list)) or not args or not isinstance(args[0], str):
+        raise ValueError("expected a Geometry or its worker_args() tuple (a class name first)")
+    args = tuple(args)
+    if args[0] not in _WORKER_REGISTRY:
+        from .bem import geometry as _bem_geometry  # noqa: F401  (registers BEM shapes)
+    if args[0] not in _WORKER_REGISTRY:
+        raise ValueError(f"geometry kind {args[0]!r} has no MI355X implementation; "
+                         f"known: {sorted(_WORKER_REGISTRY)}")
+    return Geometry.from_worker_args(args)
 
 
 def make_kill_fn(geometry, z_max=None):

@@ -74,7 +74,9 @@ class SpecificParticleTracer:
         self.device = device
         if backend not in ("mi355x",):
             resolve_backend(backend)  # raises ValueError for unknown names
-        self.geometry = geometry
+        from .geometry import lower
+
+        self.geometry = geometry = lower(geometry)  # a reference geometry is rebuilt from its worker_args()
         self.rtol = rtol if rtol is not None else DEFAULT_RTOL
         self.atol = atol if atol is not None else DEFAULT_ATOL
         self.block_threads = int(block_threads)
@@ -211,6 +213,7 @@ class SpecificParticleTracer:
         act = torch.zeros(G, dtype=torch.int64, device=dev)
         margin = torch.zeros(G, dtype=torch.float64, device=dev)
         status = torch.zeros(G, dtype=torch.int32, device=dev)
+        t_end = torch.zeros(G, dtype=torch.float64, device=dev)
         ns = len(self.screen_positions)
         cap = event_capacity if event_capacity is not None else max(1024, 4 * n * max(ns, 1))
         events = torch.empty((cap, _lib.EVENT_DTYPE.itemsize), dtype=torch.uint8, device=dev)
@@ -247,7 +250,7 @@ class SpecificParticleTracer:
         for i, z in enumerate(self.screen_positions):
             a.screens[i] = float(z)
         a.attempts, a.accepted, a.active_stages = _lib.ptr(attempts), _lib.ptr(accepted), _lib.ptr(act)
-        a.margin, a.status = _lib.ptr(margin), _lib.ptr(status)
+        a.margin, a.status, a.t_end = _lib.ptr(margin), _lib.ptr(status), _lib.ptr(t_end)
         a.events = _lib.ptr(events)
         a.event_capacity = cap
         a.event_count = _lib.ptr(ev_count)
@@ -285,6 +288,7 @@ class SpecificParticleTracer:
         out = dict(pos=pos.cpu().numpy(), vel=vel.cpu().numpy(), t_death=td.cpu().numpy(),
                    attempts=attempts.cpu().numpy(), accepted=accepted.cpu().numpy(),
                    active_stages=act.cpu().numpy(), margin=margin.cpu().numpy(), status=status.cpu().numpy(),
+                   t_end=t_end.cpu().numpy(),
                    events=ev, snapshots=snapshots, device_ms=t0.elapsed_time(t1))
         sizes = np.diff(self.offsets)
         out["particle_stages"] = int(N_STAGES * np.sum(sizes * out["attempts"]))
@@ -297,6 +301,10 @@ class SpecificParticleTracer:
         out = self.trace()
         if np.any(out["status"] == _lib.STATUS_STEPCAP):
             raise RuntimeError(f"batched RK45 did not converge within {self.max_steps} steps")
+        stalled = np.nonzero(out["status"] == _lib.STATUS_STALLED)[0]
+        if len(stalled):
+            raise RuntimeError(f"batched RK45 did not converge: the step size of group(s) {stalled[:8].tolist()} "
+                               "collapsed below 1e-14 |t| after a rejected attempt")
         self.last_run = out
         if verbose:
             print(f"{int(out['accepted'].max())} accepted step-batches across {self.n_groups} groups "

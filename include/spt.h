@@ -151,9 +151,17 @@ typedef struct spt_mlp_cfg {
                                       /* band and 2 band                         */
 } spt_mlp_cfg;
 
-/* Group status words written by spt_trace. */
+/* Group status words written by spt_trace.
+ *  DONE: reached t_max.
+ *  STEPCAP: made max_steps attempts first (the reference raises
+ *    RuntimeError at max_steps, batched_rk45.py:320-321).
+ *  STALLED: an attempt was rejected and the next step size was not above
+ *    1e-14 |t| (or was zero or NaN): the group can no longer advance and
+ *    stops there instead of spinning to max_steps (no counterpart in the
+ *    reference; the Python layer raises the reference's RuntimeError). */
 #define SPT_STATUS_DONE 0
 #define SPT_STATUS_STEPCAP 1
+#define SPT_STATUS_STALLED 2
 
 typedef struct spt_trace_args {
   spt_geometry geom;
@@ -200,6 +208,8 @@ typedef struct spt_trace_args {
   spt_event* snapshots;         /* device, capacity records                 */
   int64_t snapshot_capacity;
   uint64_t* snapshot_count;     /* device, total recorded                   */
+  double* t_end;                /* device, per group: t after the group's   */
+                                /* last attempt (t_max when done), or NULL  */
 } spt_trace_args;
 
 /* Context. */

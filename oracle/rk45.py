@@ -51,16 +51,18 @@ def attempt(accel, pos, vel, t, h, t_birth, t_death, gidx, rtol, atol, safety, f
 
 def integrate(accel, pos0, vel0, t_birth, t_max, rtol, atol, on_step=None, kill=None,
               safety=0.9, fmin=0.2, fmax=10.0, init_frac=1e-6, max_steps=10_000_000, output_times=None,
-              on_output=None):
+              on_output=None, raise_at_cap=True):
     """Reference ``batched_rk45.py:55-323``; output times (shared by all
     groups) clip each step's target like birth checkpoints and fire
     ``on_output(k, groups, t, pos, vel)`` when a group lands on the k-th,
     also at a group's start time (reference 181-209, 243-249, 295-312).
 
-    Returns dict(pos, vel, t_death, attempts, accepted, margin) where
+    Returns dict(pos, vel, t_death, attempts, accepted, margin, t) where
     attempts/accepted are per group and margin is the smallest |err - 1|
     seen by each group (how close any accept/reject decision came to
-    flipping)."""
+    flipping) and t each group's time after its last attempt. With raise_at_cap False, a run that reaches max_steps
+    returns its state there instead of raising (for comparisons of capped
+    traces)."""
     G, n, _ = pos0.shape
     pos = pos0.copy()
     vel = vel0.copy()
@@ -133,5 +135,6 @@ def integrate(accel, pos0, vel0, t_birth, t_max, rtol, atol, on_step=None, kill=
             running[ai] = t[ai] < t_max * (1 - 1e-14)
         idx = np.nonzero(running)[0]
     else:
-        raise RuntimeError(f"batched RK45 did not converge within {max_steps} steps")
-    return dict(pos=pos, vel=vel, t_death=t_death, attempts=attempts, accepted=accepted_n, margin=margin)
+        if raise_at_cap:
+            raise RuntimeError(f"batched RK45 did not converge within {max_steps} steps")
+    return dict(pos=pos, vel=vel, t_death=t_death, attempts=attempts, accepted=accepted_n, margin=margin, t=t)

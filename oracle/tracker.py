@@ -142,7 +142,7 @@ def default_t_max(st, E_gun, screen_z, t_out=None):
 
 
 def run(st, geom, screen_z, t_max, rtol=1e-7, atol=1e-12, plummer_radius=1e-12, z_max=None,
-        max_steps=10_000_000, count_stages=True, t_out=None):
+        max_steps=10_000_000, count_stages=True, t_out=None, raise_at_cap=True):
     """In-process tracker run (reference ``tracker.py:340-386``).
 
     Returns dict with final pos/vel/t_death, per-group attempts/accepted,
@@ -180,7 +180,8 @@ def run(st, geom, screen_z, t_max, rtol=1e-7, atol=1e-12, plummer_radius=1e-12, 
 
     out = rk45.integrate(counted if count_stages else accel, st["pos"], st["vel"], st["t_birth"], t_max,
                          rtol, atol, on_step=on_step, kill=lambda p: kill_mask(g, p), max_steps=max_steps,
-                         output_times=outs, on_output=on_output if outs is not None else None)
+                         output_times=outs, on_output=on_output if outs is not None else None,
+                         raise_at_cap=raise_at_cap)
     out["trajectories"] = traj
     # screens are recorded per screen; within one screen the reference's
     # order is (batch, interval, group, particle)

@@ -22,7 +22,11 @@ captured here:
   rounded corners): profile, sigma, perturbation and blended field at
   sample points, kill mask; the profile generator with per-region caps;
 * run_*.npz       -- full tracker runs: per-group attempts/accepted, final
-  state and screen events in the reference's own recording order.
+  state and screen events in the reference's own recording order
+  (run_post400_c2 / run_post400_c5: the first groups of the 1024- and
+  262,144-electron cap workloads on the 400-panel post);
+* worker_args.json / .npz -- the reference's worker_args() tuples of every
+  geometry kind and the image surfaces it builds from them.
 """
 import os
 import sys
@@ -318,8 +322,22 @@ def traced_run(name, src, n_emit, geometry, screens, **kw):
         return out
 
     def integrate(*a, **k):
+        # the integrator keeps t_death to itself: restate its kill rule
+        # (reference batched_rk45.py:290-293) in the on_step callback, which
+        # it calls right after, to record each particle's death time
+        t_birth = np.asarray(a[3])
+        t_death = np.full(t_birth.shape, np.inf)
+        kill_fn, inner = k.get("kill_fn"), k.get("on_step")
+        if kill_fn is not None:
+            def on_step(gidx, t_old, p_old, v_old, t_new, p_new, v_new):
+                was = (t_old[:, None] >= t_birth[gidx]) & (t_old[:, None] < t_death[gidx])
+                t_death[gidx] = np.where(was & kill_fn(p_new), t_new[:, None], t_death[gidx])
+                if inner is not None:
+                    inner(gidx, t_old, p_old, v_old, t_new, p_new, v_new)
+
+            k["on_step"] = on_step
         pos, vel = orig_integrate(*a, **k)
-        final["pos"], final["vel"] = pos, vel
+        final["pos"], final["vel"], final["t_death"] = pos, vel, t_death
         return pos, vel
 
     batched_rk45._attempt_step = attempt
@@ -337,7 +355,7 @@ def traced_run(name, src, n_emit, geometry, screens, **kw):
     G = tr.n_groups
     out = dict(attempts=np.array([counts[g][0] for g in range(G)]),
                accepted=np.array([counts[g][1] for g in range(G)]),
-               pos=final["pos"], vel=final["vel"], t_max=tr.t_max, screens=np.array(screens, dtype=float),
+               pos=final["pos"], vel=final["vel"], t_death=final["t_death"], t_max=tr.t_max, screens=np.array(screens, dtype=float),
                rtol=tr.rtol, atol=tr.atol, n_emit=n_emit)
     for k, v in src.items():
         out["in_" + k] = np.asarray(v)
@@ -352,6 +370,62 @@ def traced_run(name, src, n_emit, geometry, screens, **kw):
     np.savez(os.path.join(HERE, f"run_{name}.npz"), **out)
     print(f"run_{name}: groups {G}, attempts {out['attempts'].tolist()[:8]}..., "
           f"events {[len(s) for s in res]}, {dt:.1f} s", flush=True)
+
+
+def capture_worker_args():
+    """The reference's worker_args() tuples (reference tracker.py:199-202
+    rebuilds every geometry from them) for each geometry kind, with the
+    image surfaces the reference builds from them: worker_args.json (the
+    tuples, None as null) and worker_args.npz (profiles)."""
+    import json
+
+    from specific_particle_tracer.geometry import HemisphericalTip
+
+    t0 = time.time()
+    geos = {
+        "flat": FlatCathode(-1e7, z0=3e-9, kill_z_below=0.0),
+        "flat_noimage": FlatCathode(-1e8, z0=None, kill_z_below=None),
+        "tip": HemisphericalTip(-1e8, 1e-6, z0=3e-9),
+        "tip_bem": ref_bem_geometry.HemisphericalTipBEMGeometry(
+            -1e8, 50e-9, field_max_length=5e-9, z0=3e-9, image_max_length=3.5e-9, image_cap_max_length=3e-9,
+            image_fillet_max_length=0.8e-9, image_plane_max_length=2.5e-9, image_n_max=4, z_handoff=0.9e-6),
+        "tip_bem_mirror": ref_bem_geometry.HemisphericalTipBEMGeometry(
+            -1e8, 50e-9, z0=3e-9, image_cap_max_length=3e-9, image_n_max=4, image_d_lo=0.15, image_d_hi=0.6,
+            image_mirror_symmetric=True, kill_z_below=None),
+        "well_bem": ref_bem_geometry.CylindricalWellBEMGeometry(
+            -1e8, 50e-9, 30e-9, field_max_length=3e-9, z0=3e-9, image_max_length=4e-9, image_bottom_max_length=3e-9,
+            image_fillet_max_length=1e-9, image_wall_max_length=2e-9, image_rim_fillet_max_length=1.5e-9,
+            image_plane_max_length=3.5e-9, image_n_max=4),
+        "well_bem_noimage": ref_bem_geometry.CylindricalWellBEMGeometry(
+            -1e8, 50e-9, 30e-9, field_plane_radius=0.8e-6, field_max_length=4e-9, bottom_fillet_radius=4e-9,
+            rim_fillet_radius=6e-9),
+    }
+    tuples, arrays = {}, {}
+    for k, g in geos.items():
+        args = g.worker_args()
+        tuples[k] = [a.item() if isinstance(a, np.generic) else a for a in args]
+        if getattr(getattr(g, "image_solution", None), "profile", None) is not None:
+            arrays[f"{k}_image_profile"] = g.image_solution.profile
+        if hasattr(g, "real_profile"):
+            arrays[f"{k}_real_profile"] = g.real_profile
+    with open(os.path.join(HERE, "worker_args.json"), "w") as f:
+        json.dump(tuples, f, indent=1)
+    np.savez(os.path.join(HERE, "worker_args.npz"), **arrays)
+    print(f"worker_args: {len(tuples)} geometries, {time.time() - t0:.1f} s", flush=True)
+
+
+def post400_run(name, n_total, n_groups, rtol):
+    """The first n_groups groups of 16 of the n_total-electron cap workload
+    (emission.cap_source(n_total, seed 0): the same electrons as the
+    benchmark's and the parity configuration's first groups) on the
+    400-panel post with the reference's sigma, screen 90 um, kill at
+    100 um."""
+    d = np.load(os.path.join(HERE, "bem_post400.npz"))
+    R, H = float(d["R"]), float(d["H"])
+    full = emission.cap_source(n_total, R, H, theta_max_deg=60.0, mean_ke_ev=0.2, seed=0)
+    src = {k: np.asarray(v)[: 16 * n_groups] for k, v in full.items()}
+    geo = _PostGeometry(-1e8, R, H, d["profile"], d["sigma"], 20 * (R + H))
+    traced_run(name, src, 16, geo, [90e-6], rtol=rtol, atol=1e-12, z_max=100e-6)
 
 
 def main(which):
@@ -404,6 +478,12 @@ def main(which):
         src = emission.cap_source(8, R, 0.0, seed=4)
         geo = ref_bem_geometry.HemisphericalTipBEMGeometry(-1e8, R)
         traced_run("tipbem", src, 4, geo, [3e-6, 8e-6], rtol=1e-9, atol=1e-12)
+    if "workerargs" in which:
+        capture_worker_args()
+    if "post400c2" in which:
+        post400_run("post400_c2", 1024, 2, 1e-9)
+    if "post400c5" in which:
+        post400_run("post400_c5", 262144, 3, 1e-8)
     if "post" in which:
         d = np.load(os.path.join(HERE, "bem_post80.npz"))
         R, H = float(d["R"]), float(d["H"])
@@ -414,4 +494,4 @@ def main(which):
 
 if __name__ == "__main__":
     main(sys.argv[1:] or ["sphere", "post80", "c1", "c1small", "tipbem", "post", "post400", "well", "wellrun", "wellimage",
-                          "wellimgrun", "c1out", "tipimage", "tipimgrun"])
+                          "wellimgrun", "c1out", "tipimage", "tipimgrun", "workerargs", "post400c2", "post400c5"])

@@ -1,0 +1,62 @@
+"""The drop-in boundary on the GPU: a context that evaluated one BEM
+geometry must upload the next one's tables even when it has the same panel
+count (the cache key is a never-reused object token, not id()); a
+reference worker_args() tuple traces like the geometry it names."""
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from bem_amd import _lib
+
+    _lib.load()
+    return torch
+
+
+def test_same_size_geometries_do_not_share_tables(gpu):
+    from bem_amd.bem.geometry import PostTipBEMGeometry
+
+    d = golden("bem_post80.npz")
+    R, H = float(d["R"]), float(d["H"])
+    pts = np.array([[0.3e-6, 0.1e-6, H + 1.5e-6], [1.4e-6, 0.0, 4e-6], [0.0, 0.0, H + 3e-6]])
+    fields = []
+    for scale in (1.0, 2.0, 1.0):
+        # a fresh object each time, freed before the next: CPython may hand
+        # the next one the same id()
+        geo = PostTipBEMGeometry(-1e8, R, H, n_segments=80, sigma=scale * d["sigma"])
+        fields.append(geo.solution.field(pts))
+        del geo
+    np.testing.assert_allclose(fields[1], 2.0 * fields[0], rtol=1e-13, atol=0)
+    np.testing.assert_array_equal(fields[2], fields[0])
+
+
+def test_reference_tuple_traces_like_its_geometry(gpu):
+    from bem_amd import FlatCathode, SpecificParticleTracer
+    from bem_amd.particles import ParticleGroup
+
+    d = golden("run_c1small.npz")
+    src = {k[3:]: d[k] for k in d.files if k.startswith("in_")}
+    n = len(src["x"])
+    pg = ParticleGroup(dict(src, weight=np.full(n, 1.602176634e-19), status=np.ones(n), species="electron"))
+
+    class ReferenceFlat:  # the reference's FlatCathode as the tracker sees it (geometry.py:157-158)
+        def worker_args(self):
+            return ("flat", -1e8, 3e-9, 0.0)
+
+    runs = []
+    for geo in (ReferenceFlat(), FlatCathode(-1e8, z0=3e-9)):
+        tr = SpecificParticleTracer(pg, int(d["n_emit"]), geo, list(d["screens"]), rtol=float(d["rtol"]),
+                                    atol=float(d["atol"]), t_max=float(d["t_max"]))
+        tr.run()
+        runs.append(tr.last_run)
+    np.testing.assert_array_equal(runs[0]["attempts"], d["attempts"])
+    np.testing.assert_array_equal(runs[0]["attempts"], runs[1]["attempts"])
+    np.testing.assert_array_equal(runs[0]["pos"], runs[1]["pos"])

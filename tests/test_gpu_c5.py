@@ -1,0 +1,224 @@
+"""The 400-panel post (C2 and C5 workloads) against reference runs, and the
+full 262,144-electron workload through size-independent properties.
+
+Golden runs (tests/golden/make_golden.py, the reference tracker on the
+reference's own evaluate_axisymmetric_field with the reference's 400-panel
+sigma): run_post400_c2 is the first 2 groups of 16 of the 1024-electron
+cap workload at rtol 1e-9, run_post400_c5 the first 3 groups of the
+262,144-electron workload at rtol 1e-8, screen 90 um, kill at 100 um.
+
+* The plain FP64 method (persistent and lockstep) must reproduce them:
+  identical attempt/accept counts, kill sets, screen-event identities and
+  order, crossings within 1e-10 (times) / 1e-8 relative, final states
+  within 1e-9 at the device's kill time (a particle killed at z_max is
+  frozen at the end of the step that crossed it, whose time moves with
+  roundoff-level step-size differences: tests/trajectory.py). At full size
+  the first groups of the FP64 trace must still equal the golden runs
+  (groups never interact, reference batched_rk45.py:4-22).
+* The forms that change arithmetic keep the reference's events and cross
+  the screen within a stated bound: FP32 + Kahan pair forces ("pairs32")
+  1e-7 of the crossing time; the field-error budget (far rule (2, 16) and
+  the validated multipole zone, each <= 1e-8 of |E|) 1e-6; both also at
+  full size against the FP64 trace of the same electrons."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def post400():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from bem_amd import _lib
+    from bem_amd.bem.geometry import PostTipBEMGeometry
+
+    _lib.load()
+    d = golden("bem_post400.npz")
+    return PostTipBEMGeometry(-1e8, float(d["R"]), float(d["H"]), n_segments=400, sigma=d["sigma"])
+
+
+def _say(*a):
+    print(*a, flush=True, file=sys.stderr)
+
+
+def _pg(src):
+    from bem_amd.particles import ParticleGroup
+
+    n = len(src["x"])
+    return ParticleGroup(dict(src, weight=np.full(n, 1.602176634e-19), status=np.ones(n), species="electron"))
+
+
+def _tracer(geo, src, rtol, **kw):
+    from bem_amd import SpecificParticleTracer
+
+    return SpecificParticleTracer(_pg(src), 16, geo, [90e-6], rtol=rtol, atol=1e-12, z_max=100e-6, **kw)
+
+
+def _golden_src(d):
+    return {k[3:]: d[k] for k in d.files if k.startswith("in_")}
+
+
+def _golden_geom():
+    d = golden("bem_post400.npz")
+    R, H = float(d["R"]), float(d["H"])
+    return dict(kind="post_bem", E_gun=-1e8, R=R, H=H, profile=d["profile"], sigma=d["sigma"], z_handoff=20 * (R + H))
+
+
+def _check_exact(out, d, groups, screens=None):
+    """Groups `groups` of a device run against the golden run d: identical
+    attempt/accept counts and kill sets; the final state at the device's
+    kill time (tests/trajectory.py) within 1e-9; screen crossings (times,
+    positions, momenta) within 1e-10 / 1e-8."""
+    from oracle import tracker as otr
+    from oracle.constants import ELECTRON_MASS_EV, ELEMENTARY_CHARGE, ev_to_kg
+    from trajectory import assert_same_state
+
+    G = len(groups)
+    n_per = 16
+    np.testing.assert_array_equal(out["attempts"][groups], d["attempts"])
+    np.testing.assert_array_equal(out["accepted"][groups], d["accepted"])
+    sl = slice(groups[0] * n_per, (groups[-1] + 1) * n_per)
+    pos = out["pos"][sl].reshape(G, n_per, 3)
+    vel = out["vel"][sl].reshape(G, n_per, 3)
+    td = out["t_death"][sl].reshape(G, n_per)
+    td_ref = d["t_death"]
+    np.testing.assert_array_equal(np.isfinite(td), np.isfinite(td_ref))
+    dt = np.where(np.isfinite(td_ref), td - np.where(np.isfinite(td_ref), td_ref, 0.0), 0.0)
+    acc = -ELEMENTARY_CHARGE / ev_to_kg(ELECTRON_MASS_EV) * otr.field(_golden_geom(), d["pos"])
+    assert_same_state(pos, vel, d["pos"], d["vel"], acc, dt, np.where(np.isfinite(td_ref), td_ref, float(d["t_max"])))
+    _say(f"  kill-time shift max {np.max(np.abs(dt) / np.where(np.isfinite(td_ref), td_ref, 1.0)):.1e} relative")
+    if screens is not None:
+        (s,) = screens
+        np.testing.assert_array_equal(s.id, d["s0_id"])
+        np.testing.assert_allclose(s.t, d["s0_t"], rtol=1e-10)
+        # positions and momenta at the crossing come from the cubic Hermite
+        # interpolant of the step that contains it (reference screens.py:
+        # 118-168), whose error depends on that step's ends, which move with
+        # the step sizes (measured 2.5e-9 of the largest momentum)
+        for f in ("x", "y", "z", "px", "py", "pz"):
+            ref = d[f"s0_{f}"]
+            np.testing.assert_allclose(getattr(s, f), ref, rtol=1e-8, atol=1e-8 * np.abs(ref).max())
+
+
+def _crossings(out, groups):
+    """Screen events of `groups` as (group, particle) -> t."""
+    ev = out["events"]
+    keep = np.isin(ev["group"], groups)
+    return {(int(g), int(p)): t for g, p, t in zip(ev["group"][keep], ev["particle"][keep], ev["t"][keep])}
+
+
+def _check_bound(out, d, groups, bound):
+    """Same crossings as the golden run, times within `bound` relative."""
+    ref = {}
+    for i, (pid, t) in enumerate(zip(d["s0_id"], d["s0_t"])):
+        ref[((int(pid) - 1) // 16, (int(pid) - 1) % 16)] = t
+    got = _crossings(out, groups)
+    assert sorted(got) == sorted(ref)
+    dt = max(abs(got[k] - ref[k]) / ref[k] for k in ref)
+    return dt
+
+
+@pytest.mark.parametrize("mode", ["persistent", "lockstep"])
+def test_c2_fp64_matches_reference_run(post400, mode):
+    d = golden("run_post400_c2.npz")
+    tr = _tracer(post400, _golden_src(d), 1e-9, mode=mode, t_max=float(d["t_max"]))
+    screens, _ = tr.run()
+    out = tr.last_run
+    _say(f"C2 400 panels [{mode}]: attempts {out['attempts'].tolist()} vs {d['attempts'].tolist()}, "
+         f"smallest |err - 1| {out['margin'].min():.1e}")
+    _check_exact(out, d, [0, 1], screens)
+
+
+def test_c2_workload_first_groups_match_reference_run(post400):
+    """The whole 1024-electron parity configuration (bench.py's c2 line) in
+    lockstep; its first two groups are the golden run's."""
+    from bem_amd.emission import cap_source
+
+    d = golden("run_post400_c2.npz")
+    src = cap_source(1024, post400.R, post400.H, theta_max_deg=60.0, mean_ke_ev=0.2, seed=0)
+    for k in ("x", "y", "z", "px", "py", "pz"):
+        np.testing.assert_array_equal(src[k][:32], d["in_" + k])
+    tr = _tracer(post400, src, 1e-9, mode="lockstep", t_max=float(d["t_max"]))
+    out = tr.trace()
+    assert np.all(out["status"] == 0) and len(out["events"]) == 1024
+    _check_exact(out, d, [0, 1])
+
+
+def test_c5_forms_match_reference_run_within_bounds(post400):
+    """The C5 golden groups (rtol 1e-8) under the plain FP64 method (exact),
+    FP32 + Kahan pair forces and the field-error budget."""
+    from bem_amd.bem import multipole as mpl
+
+    d = golden("run_post400_c5.npz")
+    src = _golden_src(d)
+    t_max = float(d["t_max"])
+    tr = _tracer(post400, src, 1e-8, mode="lockstep", t_max=t_max)
+    screens, _ = tr.run()
+    _check_exact(tr.last_run, d, [0, 1, 2], screens)
+    out = _tracer(post400, src, 1e-8, mode="lockstep", precision="pairs32", t_max=t_max).trace()
+    dt = _check_bound(out, d, [0, 1, 2], 1e-7)
+    _say(f"C5 golden groups, FP32 pairs: crossing time max rel diff {dt:.2e}, attempts {out['attempts'].tolist()}")
+    assert dt <= 1e-7
+    mp = mpl.fit(post400, n_terms=128, tol=1e-8)
+    assert not mp.inert
+    out = _tracer(post400, src, 1e-8, mode="lockstep", precision="pairs32", t_max=t_max, far_rule=(2.0, 16.0),
+                  multipole=mp).trace()
+    dt = _check_bound(out, d, [0, 1, 2], 1e-6)
+    _say(f"C5 golden groups, far rule + multipole: crossing time max rel diff {dt:.2e}, "
+         f"attempts {out['attempts'].tolist()} vs {d['attempts'].tolist()}")
+    assert dt <= 1e-6
+
+
+@pytest.fixture(scope="module")
+def c5_full(post400):
+    """The full C5 workload (262,144 electrons, 16,384 groups of 16) traced
+    three ways: plain FP64, plain with FP32 pairs (bench.py's plain line),
+    and bench.py's value form."""
+    from bem_amd.bem import multipole as mpl
+    from bem_amd.emission import cap_source
+
+    src = cap_source(262144, post400.R, post400.H, theta_max_deg=60.0, mean_ke_ev=0.2, seed=0)
+    mp = mpl.fit(post400, n_terms=128, tol=1e-8)
+    runs = {}
+    for name, kw in (("fp64", dict(precision="fp64")), ("pairs32", dict(precision="pairs32")),
+                     ("value", dict(precision="pairs32", far_rule=(2.0, 16.0), multipole=mp))):
+        tr = _tracer(post400, src, 1e-8, mode="lockstep", **kw)
+        runs[name] = tr.trace()
+        o = runs[name]
+        _say(f"C5 full [{name}]: {o['device_ms']:.0f} ms, attempts mean {o['attempts'].mean():.2f}, "
+             f"{len(o['events'])} events")
+    return runs
+
+
+def test_c5_full_fp64_first_groups_match_reference_run(c5_full):
+    d = golden("run_post400_c5.npz")
+    out = c5_full["fp64"]
+    assert np.all(out["status"] == 0)
+    _check_exact(out, d, [0, 1, 2])
+
+
+@pytest.mark.parametrize("form,bound", [("pairs32", 1e-7), ("value", 1e-6)])
+def test_c5_full_every_electron_crosses_within_bound(c5_full, form, bound):
+    """Size-independent properties at full size: every electron crosses the
+    90 um screen exactly once, groups finish, and crossing times agree with
+    the FP64 trace of the same electrons within `bound`."""
+    n = 262144
+    a, b = c5_full["fp64"], c5_full[form]
+    assert np.all(b["status"] == 0)
+    for o in (a, b):
+        ev = o["events"]
+        assert len(ev) == n
+        key = ev["group"].astype(np.int64) * 16 + ev["particle"]
+        assert np.array_equal(np.sort(key), np.arange(n))
+    ka = np.argsort(a["events"]["group"].astype(np.int64) * 16 + a["events"]["particle"])
+    kb = np.argsort(b["events"]["group"].astype(np.int64) * 16 + b["events"]["particle"])
+    dt = np.abs(a["events"]["t"][ka] - b["events"]["t"][kb]) / a["events"]["t"][ka]
+    _say(f"C5 full [{form}] vs FP64: crossing time rel diff max {dt.max():.2e} median {np.median(dt):.2e}; "
+         f"attempts differ in {int(np.sum(a['attempts'] != b['attempts']))} of 16384 groups")
+    assert dt.max() <= bound

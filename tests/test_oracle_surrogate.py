@@ -124,14 +124,11 @@ def test_initial_params_layout_matches_oracle():
 
 
 def test_surrogate_requires_lockstep_and_its_geometry():
-    from bem_amd import SpecificParticleTracer
+    from bem_amd import FlatCathode, SpecificParticleTracer
     from bem_amd.surrogate import FieldSurrogate
     from bem_amd.particles import ParticleGroup
 
-    class _Geo:
-        pass
-
-    geo = _Geo()
+    geo = FlatCathode(-1e7)
     sur = FieldSurrogate(geo, {}, None)
     n = 4
     pg = ParticleGroup(dict(x=np.zeros(n), y=np.zeros(n), z=np.zeros(n), px=np.zeros(n), py=np.zeros(n),
@@ -140,4 +137,4 @@ def test_surrogate_requires_lockstep_and_its_geometry():
     with pytest.raises(ValueError, match="lockstep"):
         SpecificParticleTracer(pg, 4, geo, [1e-6], mode="persistent", surrogate=sur)
     with pytest.raises(ValueError, match="different geometry"):
-        SpecificParticleTracer(pg, 4, _Geo(), [1e-6], surrogate=sur)
+        SpecificParticleTracer(pg, 4, FlatCathode(-1e7), [1e-6], surrogate=sur)
