@@ -32,7 +32,7 @@ EXPORTS = (
     "spt_pair_forces", "spt_kill_mask", "spt_trace", "spt_bem_potential", "spt_mlp_nparams", "spt_mlp_grad",
     "spt_mlp_train", "spt_mlp_infer", "spt_surrogate_set", "spt_toroidal_q", "spt_ring_modes",
     "spt_image_upload", "spt_image_force", "spt_set_field_shape", "spt_bem_far_rule",
-    "spt_bem_multipole",
+    "spt_bem_multipole", "spt_trace_noalloc",
 )
 MLP_FOURIER, MLP_WIDTH, MLP_DEPTH = 32, 128, 4
 
@@ -143,6 +143,7 @@ def load():
                                                ctypes.c_double, _p, _p, ctypes.c_int32]),
             "spt_kill_mask": (ctypes.c_int, [_p, P(Geometry_t), _p, ctypes.c_int64, _p]),
             "spt_trace": (ctypes.c_int, [_p, P(TraceArgs_t)]),
+            "spt_trace_noalloc": (ctypes.c_int, [_p, P(TraceArgs_t)]),
             "spt_bem_potential": (ctypes.c_int, [_p, _p, ctypes.c_int64, _p]),
             "spt_mlp_nparams": (ctypes.c_int, [P(ctypes.c_int64)]),
             "spt_mlp_grad": (ctypes.c_int, [_p, P(MlpCfg_t), _p, _p, _p, ctypes.c_int32, _p, ctypes.c_int32, _p,

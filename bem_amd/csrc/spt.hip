@@ -2134,6 +2134,16 @@ __global__ void potential_kernel(const double* prof, const double* sigma, int n,
 // ================================================================ host side
 constexpr int FIELD_NMAX_PUBLIC = 64;  // largest points-per-block of the field gather
 
+// Scratch of spt_trace_noalloc (the persistent trace without allocation):
+// the work-queue counter in the first 256 bytes, then the stage buffer of
+// kSlots doubles per particle. One copy per device, held until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress
+// returns for the context's device (spt_ctx::static_scratch).
+constexpr size_t kStaticScratchBytes = size_t(64) << 20;
+constexpr size_t kStaticKbufOffset = 256 / sizeof(double);
+constexpr size_t kStaticKbufElems = kStaticScratchBytes / sizeof(double) - kStaticKbufOffset;
+static __device__ __align__(256) double scratch[kStaticScratchBytes / sizeof(double)];
+
 struct spt_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -2154,6 +2164,8 @@ struct spt_ctx {
   double* d_kbuf = nullptr;
   size_t kbuf_elems = 0;
   int* d_queue = nullptr;
+  double* static_scratch = nullptr;  // this device's `scratch` (hipGetSymbolAddress), or nullptr
+  std::string static_err;
   rocblas_handle blas = nullptr;
   // lockstep scratch
   hipStream_t own = nullptr;
@@ -2269,6 +2281,14 @@ int spt_create(int device, spt_ctx** out) {
   if (hipMalloc(&c->d_queue, 64) != hipSuccess) {
     delete c;
     return SPT_ENOMEM;
+  }
+  // the device address of this device's copy of `scratch` (the host-side
+  // name is only a stand-in): everything that passes it to a kernel uses it
+  const hipError_t se = hipGetSymbolAddress((void**)&c->static_scratch, HIP_SYMBOL(scratch));
+  if (se != hipSuccess) {
+    c->static_scratch = nullptr;
+    c->static_err = hipGetErrorString(se);
+    (void)hipGetLastError();
   }
   *out = c;
   return SPT_OK;
@@ -2866,7 +2886,7 @@ static int ensure_lockstep_scratch(spt_ctx* c, int64_t n, int n_groups) {
 
 extern "C" {
 
-int spt_trace(spt_ctx* c, const spt_trace_args* a) {
+static int trace_impl(spt_ctx* c, const spt_trace_args* a, bool no_alloc) {
   if (!c || !a) return SPT_EINVAL;
   int rc = check_geom(c, &a->geom);
   if (rc) return rc;
@@ -2919,7 +2939,14 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
   if (a->precision == SPT_PREC_MIXED && a->mode == SPT_MODE_PERSISTENT && T != 256)
     return fail(c, SPT_EINVAL, "mixed precision in persistent mode is built for 256-thread workgroups only");
   const size_t need = (size_t)(a->mode == SPT_MODE_LOCKSTEP ? kSlotsLockstep : kSlots) * (size_t)a->n_particles;
-  if (need > c->kbuf_elems) {
+  if (no_alloc) {
+    if (a->mode != SPT_MODE_PERSISTENT) return fail(c, SPT_EINVAL, "spt_trace_noalloc runs the persistent mode only");
+    if (!c->static_scratch) return fail(c, SPT_EHIP, "spt_trace_noalloc: no device scratch (" + c->static_err + ")");
+    if (need > kStaticKbufElems)
+      return fail(c, SPT_EINVAL, "spt_trace_noalloc: n_particles " + std::to_string(a->n_particles) +
+                                     " exceeds the static scratch's " +
+                                     std::to_string(kStaticKbufElems / kSlots) + " (64 MiB)");
+  } else if (need > c->kbuf_elems) {
     if (c->d_kbuf) (void)hipFree(c->d_kbuf);
     c->d_kbuf = nullptr;
     c->kbuf_elems = 0;
@@ -2942,7 +2969,7 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
   P.charge = a->charge;
   P.mass = a->mass;
   P.t_death = a->t_death;
-  P.kbuf = c->d_kbuf;
+  P.kbuf = no_alloc ? c->static_scratch + kStaticKbufOffset : c->d_kbuf;
   P.q_over_m = a->q_over_m;
   P.a2 = a->plummer_radius * a->plummer_radius;
   P.t_max = a->t_max;
@@ -2961,7 +2988,7 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
   P.events = a->events;
   P.ev_cap = a->event_capacity;
   P.ev_count = (unsigned long long*)a->event_count;
-  P.queue = c->d_queue;
+  P.queue = no_alloc ? reinterpret_cast<int*>(c->static_scratch) : c->d_queue;
   P.t_out = a->t_out;
   P.n_out = a->n_out;
   P.snaps = a->snapshots;
@@ -3021,7 +3048,7 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
     HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_out, 0));
     return rc3;
   }
-  reset_int_kernel<<<1, 1, 0, c->stream>>>(c->d_queue, 0);
+  reset_int_kernel<<<1, 1, 0, c->stream>>>(P.queue, 0);
   reset_u64_kernel<<<1, 1, 0, c->stream>>>(P.ev_count);
   if (P.n_out > 0) reset_u64_kernel<<<1, 1, 0, c->stream>>>(P.snap_count);
   switch (T) {
@@ -3034,6 +3061,10 @@ int spt_trace(spt_ctx* c, const spt_trace_args* a) {
     default: return launch_trace<1024, 64, SPT_PREC_FP64>(c, P, a->n_groups);
   }
 }
+
+int spt_trace(spt_ctx* c, const spt_trace_args* a) { return trace_impl(c, a, false); }
+
+int spt_trace_noalloc(spt_ctx* c, const spt_trace_args* a) { return trace_impl(c, a, true); }
 
 }  // extern "C"
 

@@ -188,8 +188,10 @@ class SpecificParticleTracer:
         ctx.check(ctx.lib.spt_device_cus(ctx.h, ctypes.byref(n_cu)), "spt_device_cus")
         return _lib.MODE_LOCKSTEP if self.n_groups < 2 * n_cu.value else _lib.MODE_PERSISTENT
 
-    def trace(self, event_capacity=None):
-        """Run spt_trace; returns the raw device results as NumPy arrays."""
+    def trace(self, event_capacity=None, no_alloc=False):
+        """Run spt_trace; returns the raw device results as NumPy arrays.
+        no_alloc: call spt_trace_noalloc instead (persistent mode, the
+        library's static device scratch, at most 155,344 particles)."""
         import torch
 
         ctx = self.geometry._ctx()
@@ -271,7 +273,10 @@ class SpecificParticleTracer:
         t1 = torch.cuda.Event(enable_timing=True)
         t0.record()
         try:
-            ctx.check(ctx.lib.spt_trace(ctx.h, a), "spt_trace")
+            if no_alloc:
+                ctx.check(ctx.lib.spt_trace_noalloc(ctx.h, a), "spt_trace_noalloc")
+            else:
+                ctx.check(ctx.lib.spt_trace(ctx.h, a), "spt_trace")
         finally:
             ctx.lib.spt_set_field_shape(ctx.h, 0, 0)
             ctx.lib.spt_bem_far_rule(ctx.h, 0.0, 0.0)
@@ -281,7 +286,7 @@ class SpecificParticleTracer:
         torch.cuda.synchronize(dev)
         n_ev = int(ev_count.item())
         if n_ev > cap:
-            return self.trace(event_capacity=n_ev)
+            return self.trace(event_capacity=n_ev, no_alloc=no_alloc)
         ev = np.frombuffer(events[:n_ev].cpu().numpy().tobytes(), dtype=_lib.EVENT_DTYPE)
         n_sn = min(int(snap_count.item()), snap_cap)
         snapshots = np.frombuffer(snaps[:n_sn].cpu().numpy().tobytes(), dtype=_lib.EVENT_DTYPE)

@@ -369,6 +369,20 @@ int spt_surrogate_set(spt_ctx* ctx, const spt_mlp_cfg* cfg, const float* params)
  * args->events. Asynchronous; call spt_sync before reading outputs. */
 int spt_trace(spt_ctx* ctx, const spt_trace_args* args);
 
+/* spt_trace that allocates nothing. No counterpart in the reference; it is
+ * for callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc. PERSISTENT
+ * mode only, with the same kernels and results as spt_trace in that mode;
+ * the stage buffer (54 doubles per particle) and the work-queue counter
+ * live in a 64 MiB file-scope device array of the library instead of
+ * context memory, so n_particles <= 155,344 (larger inputs: SPT_EINVAL).
+ * Calls no hipMalloc, hipFree or allocating library routine. Each device
+ * holds its own copy of the array until the process ends; every call on a
+ * device shares it and it keeps the last call's contents (the queue
+ * counter is cleared at the start of each call), so calls that overlap on
+ * one device (any context, any stream) race. */
+int spt_trace_noalloc(spt_ctx* ctx, const spt_trace_args* args);
+
 #ifdef __cplusplus
 }
 #endif
