@@ -32,7 +32,7 @@ EXPORTS = (
     "spt_pair_forces", "spt_kill_mask", "spt_trace", "spt_bem_potential", "spt_mlp_nparams", "spt_mlp_grad",
     "spt_mlp_train", "spt_mlp_infer", "spt_surrogate_set", "spt_toroidal_q", "spt_ring_modes",
     "spt_image_upload", "spt_image_force", "spt_set_field_shape", "spt_bem_far_rule",
-    "spt_bem_multipole", "spt_trace_noalloc",
+    "spt_bem_multipole", "spt_trace_noalloc", "spt_bem_clusters", "spt_bem_near_width",
 )
 MLP_FOURIER, MLP_WIDTH, MLP_DEPTH = 32, 128, 4
 
@@ -95,7 +95,7 @@ class TraceArgs_t(ctypes.Structure):
         ("events", _p), ("event_capacity", ctypes.c_int64), ("event_count", _p),
         ("t_out", _p), ("n_out", ctypes.c_int32), ("pad1", ctypes.c_int32),
         ("snapshots", _p), ("snapshot_capacity", ctypes.c_int64), ("snapshot_count", _p),
-        ("t_end", _p),
+        ("t_end", _p), ("field_order", _p),
     ]
 
 
@@ -134,6 +134,9 @@ def load():
             "spt_bem_upload": (ctypes.c_int, [_p, _p, ctypes.c_int32, _p, ctypes.c_int32]),
             "spt_bem_far_rule": (ctypes.c_int, [_p, ctypes.c_double, ctypes.c_double]),
             "spt_bem_multipole": (ctypes.c_int, [_p, _p, ctypes.c_int32, ctypes.c_double, ctypes.c_double]),
+            "spt_bem_clusters": (ctypes.c_int, [_p, _p, ctypes.c_int32, _p, _p, _p, ctypes.c_int32, ctypes.c_double,
+                                                ctypes.c_double]),
+            "spt_bem_near_width": (ctypes.c_int, [_p, ctypes.c_double]),
             "spt_bem_assemble": (ctypes.c_int, [_p, _p, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _p]),
             "spt_bem_solve": (ctypes.c_int, [_p, _p, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_int32, _p]),

@@ -1,9 +1,10 @@
 """Autotuner for the trace's launch configuration (no counterpart in the
 reference, which has one NumPy/CuPy code path).
 
-For a configured SpecificParticleTracer it times short traces capped at
-`attempts` RK45 attempts per group (every candidate does the same work:
-the same groups, the same attempts) and keeps the fastest of:
+For a configured SpecificParticleTracer it times traces capped at
+`attempts` RK45 attempts per group, or whole traces (every candidate does
+the same work: the same groups, the same attempts) and keeps the fastest
+of:
 
 * mode "persistent" with each workgroup size that holds the largest group
   (64, 128, 256 threads), one group per workgroup from a device queue;
@@ -50,8 +51,9 @@ def candidates(tracer, n_cu):
 
 
 def tune(tracer, attempts=6, log=None):
-    """Time every candidate on `attempts` attempts per group; apply and
-    return the best (dict with "best" and "table")."""
+    """Time every candidate on `attempts` attempts per group (None: whole
+    traces, for workloads whose point mix changes along the trace); apply
+    and return the best (dict with "best" and "table")."""
     import ctypes
 
     ctx = tracer.geometry._ctx()
@@ -60,7 +62,8 @@ def tune(tracer, attempts=6, log=None):
     saved = (tracer.mode, tracer.block_threads, getattr(tracer, "field_shape", None), tracer.max_steps)
     table = []
     try:
-        tracer.max_steps = int(attempts)
+        if attempts is not None:
+            tracer.max_steps = int(attempts)
         for cand in candidates(tracer, n_cu.value):
             tracer.mode = cand["mode"]
             tracer.block_threads = cand["block_threads"]

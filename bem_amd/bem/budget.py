@@ -1,0 +1,62 @@
+"""The field-error budget of the fast form (no counterpart in the
+reference, whose plain method sums every segment with the fixed rule at
+every point, bem/axisymmetric.py:501-605).
+
+A FieldBudget bundles the four exact-in-the-limit approximations of the BEM
+perturbation that the fast traces use, each converging to the plain
+method's fixed-rule sum:
+
+* the distance-adaptive far rule (k4, k2): Gauss-8 / Gauss-4 / Gauss-2 per
+  segment instead of 8 sub-panels x Gauss-4 (spt_bem_far_rule);
+* a wider near rule: sinh-substituted Gauss-8 on intervals of width
+  near_width instead of 0.5 (spt_bem_near_width);
+* cluster expansions: axis-centred exterior expansions of runs of
+  segments beyond kappa times their radius (spt_bem_clusters,
+  bem_amd.bem.clusters);
+* the exterior multipole zone of the whole charge (spt_bem_multipole,
+  bem_amd.bem.multipole).
+
+fit() validates the first three together against the device's fixed rule
+on fresh points around the cap (field error relative to |E|) and the
+multipole zone on its own, and refuses a budget that misses the
+tolerance."""
+from . import clusters as _clusters
+from . import multipole as _multipole
+
+
+class FieldBudget:
+    def __init__(self, geometry, far_rule, near_width, clusters, multipole, report):
+        self.geometry = geometry
+        self.far_rule = far_rule
+        self.near_width = near_width
+        self.clusters = clusters
+        self.multipole = multipole
+        self.report = report
+
+    def tracer_kwargs(self):
+        """Keyword arguments of SpecificParticleTracer for this budget."""
+        return dict(far_rule=self.far_rule, near_width=self.near_width, clusters=self.clusters,
+                    multipole=None if self.multipole is None or self.multipole.inert else self.multipole)
+
+
+def fit(geometry, tol=1e-6, far_rule=(2.0, 16.0), near_width=2.0, seg_per_cluster=25, kappa=1.6,
+        multipole_terms=128, multipole_tol=None, n_val=1 << 16, use_clusters=True, use_multipole=True):
+    """Build and validate a budget for a protrusion BEM geometry (the post or
+    the hemispherical tip). Raises ValueError when the validated far-rule
+    gather (far rule + near width + clusters) misses tol."""
+    from ..surrogate import far_rule_error
+
+    cl = _clusters.fit(geometry, seg_per_cluster=seg_per_cluster, kappa=kappa) if use_clusters else None
+    err = far_rule_error(geometry, *far_rule, n=n_val, near_width=near_width, clusters=cl)
+    if not err["field_err_max"] <= tol:
+        raise ValueError(f"the far-rule gather misses the field tolerance {tol:g}: {err['field_err_max']:.3e}")
+    mp = None
+    if use_multipole:
+        mp = _multipole.fit(geometry, n_terms=multipole_terms, tol=min(1e-8, tol) if multipole_tol is None
+                            else multipole_tol)
+    rep = dict(tol=tol, far_rule=list(far_rule), near_width=near_width, gather=err,
+               clusters=None if cl is None else dict(n=cl.n_clusters, kappa=cl.kappa, tol=cl.tol,
+                                                     n_terms=cl.n_terms, seg_per_cluster=seg_per_cluster),
+               multipole=None if mp is None else {k: v for k, v in mp.report.items()
+                                                 if k not in ("candidates", "worst_beyond")})
+    return FieldBudget(geometry, tuple(far_rule), near_width, cl, mp, rep)

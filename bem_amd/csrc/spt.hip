@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
+#include <algorithm>
 #include <vector>
 
 #include <math.h>
@@ -94,9 +95,25 @@ struct BemView {
   const double4* mp;
   int mp_n;
   double mp_rref, mp_rmin2;
+  // Axis-centred expansions of runs of consecutive segments (far-rule
+  // gathers only; nullptr: off): per cluster (z_c, r_c, (kappa r_c)^2, 0),
+  // its segments cl_seg[c] .. cl_seg[c + 1] - 1, moments cl_mom[c][l]; a
+  // point at (kappa r_c)^2 or more from (0, z_c) takes the cluster's field
+  // from the expansion, truncated where the tail bound drops below cl_tol.
+  const double4* cl;
+  const int* cl_seg;
+  const int* seg_cl;        // per segment: its cluster
+  const double* cl_mom;     // cl_n x cl_terms
+  const double2* cl_coef;   // per term: ((2l+1)/(l+1), l/(l+1))
+  int cl_n, cl_terms;
+  double cl_tol;
+  double near_w;            // near-rule interval width of far-rule gathers (the reference's: 0.5)
 };
 constexpr int kCoarsePer = 14;
 constexpr int kMultipoleMaxTerms = 1024;
+constexpr int kMaxClusters = 64;
+constexpr int kClusterMaxTerms = 256;
+constexpr double kRefNearWidth = 0.5;
 
 // Segment geometry (reference bem/axisymmetric.py:329-338,490-498).
 __global__ void seg_table_kernel(const double* prof, int n_seg, double* segs) {
@@ -184,14 +201,15 @@ struct NearRule {
   int count;
 };
 
-__device__ __forceinline__ NearRule near_rule(const double* g, double proj, double dist) {
+__device__ __forceinline__ NearRule near_rule(const double* g, double proj, double dist,
+                                              double width = kRefNearWidth) {
   NearRule r;
   const double L = g[S_L];
   r.d = fmax(dist, 1e-15 * L);
   r.s0 = proj;
   r.lo = asinh(-r.s0 / r.d);
   const double hi = asinh((L - r.s0) / r.d);
-  r.count = max(1, (int)ceil((hi - r.lo) / 0.5));
+  r.count = max(1, (int)ceil((hi - r.lo) / width));
   r.step = (hi - r.lo) / r.count;
   r.half = 0.5 * (hi - r.lo) / r.count;
   return r;
@@ -244,8 +262,9 @@ __device__ __forceinline__ void near_nodes(double re, double re2, double ze, con
 // Whole near segment on one thread; its sum comes back by value (the
 // caller adds it), so the caller's accumulator never goes to memory.
 __device__ __noinline__ RingAcc near_segment(double re, double re2, double ze, const double* g, double sig0,
-                                             double sig1, double proj, double dist, const double2* lt) {
-  const NearRule r = near_rule(g, proj, dist);
+                                             double sig1, double proj, double dist, const double2* lt,
+                                             double width) {
+  const NearRule r = near_rule(g, proj, dist, width);
   RingAcc acc;
   acc.zero();
   near_nodes(re, re2, ze, g, sig0, sig1, r, 0, 1, acc, lt);
@@ -380,6 +399,70 @@ __device__ __forceinline__ void coarse_segment(double re, double re2, double ze,
   }
 }
 
+// Bit c set: the point (re, ze) is at least kappa r_c from cluster c's
+// centre, so the cluster's expansion replaces its segments.
+__device__ __forceinline__ unsigned long long far_clusters(const BemView& B, double re2, double ze) {
+  unsigned long long m = 0ull;
+  const cseg_ptr t = (cseg_ptr)B.cl;
+  for (int c = 0; c < B.cl_n; ++c) {
+    const double dz = ze - t[4 * c];
+    if (fma(dz, dz, re2) >= t[4 * c + 2]) m |= 1ull << c;
+  }
+  return m;
+}
+
+// f(j) for the segments j = s, s + S, ... that are not in a far cluster
+// (all of them without clusters), in ascending order.
+template <class F>
+__device__ __forceinline__ void local_segments(const BemView& B, unsigned long long far, int s, int S, F&& f) {
+  if (!B.cl) {
+    for (int j = s; j < B.n_seg; j += S) f(j);
+    return;
+  }
+  for (int c = 0; c < B.cl_n; ++c) {
+    if ((far >> c) & 1ull) continue;
+    const int b = B.cl_seg[c], e = B.cl_seg[c + 1];
+    for (int j = b + ((s - b) % S + S) % S; j < e; j += S) f(j);
+  }
+}
+
+// Field of cluster c's exterior expansion at (re, ze), added to the
+// accumulator's final-unit terms: phi = (1/4 pi) sum_l m_l q^l P_l(cos t)/r
+// about (0, z_c), q = r_c / r, E_r = (1/4 pi r^2) sum m_l (l+1) q^l P_l,
+// E_t = (sin t / 4 pi r^2) sum m_l q^l P_l'; terms up to the first l whose
+// tail bound sum_{k > l} (k+1) q^k = q^(l+1) (l + 2 - (l+1) q) / (1-q)^2
+// is <= cl_tol.
+__device__ __forceinline__ void cluster_expansion(const BemView& B, int c, double re, double re2, double ze,
+                                                  RingAcc& acc) {
+  const cseg_ptr t = (cseg_ptr)B.cl;
+  const double dz = ze - t[4 * c];
+  const double r2 = fma(dz, dz, re2);
+  const double ir = rsqrt64(r2);
+  const double cs = dz * ir, sn = re * ir, q = t[4 * c + 1] * ir;
+  const double omq = 1.0 - q;
+  const double tolw = B.cl_tol * omq * omq;
+  const cseg_ptr mom = (cseg_ptr)B.cl_mom + (size_t)c * B.cl_terms;
+  const cseg_ptr cf = (cseg_ptr)B.cl_coef;
+  double pl = 1.0, plm = 0.0, dl = 0.0, dlm = 0.0, qp = 1.0, sa = 0.0, sb = 0.0;
+  for (int l = 0; l < B.cl_terms; ++l) {
+    const double mq = mom[l] * qp;
+    sa = fma(mq * (double)(l + 1), pl, sa);
+    sb = fma(mq, dl, sb);
+    qp *= q;
+    if (qp * fma(-(double)(l + 1), q, (double)(l + 2)) <= tolw) break;
+    const double pn = fma(cf[2 * l] * cs, pl, -cf[2 * l + 1] * plm);
+    const double dn = fma((double)(2 * l + 1), pl, dlm);
+    plm = pl;
+    pl = pn;
+    dlm = dl;
+    dl = dn;
+  }
+  const double k = (ir * ir) * (1.0 / (4.0 * kPi));
+  const double er = k * sa, et = k * sn * sb;
+  acc.er_ax += er * sn + et * cs;
+  acc.ez_ax += er * cs - et * sn;
+}
+
 // Perturbation field -grad S[sigma] at m <= NMAX points held in LDS, by all
 // T threads of the block: S threads ("slices") per point, slice s taking
 // segments s, s+S, ... Two lane layouts: slice-major (S = T/m, point p's
@@ -409,12 +492,13 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
   // thread holding slice k of this thread's point: slot0 + k * sstep
   const int slot0 = wave_major ? p : p * S, sstep = wave_major ? 64 : 1;
   auto slot = [=](int k) { return slot0 + k * sstep; };
+  const double nw = FAR ? B.near_w : kRefNearWidth;  // near-rule interval width
   if (on) {
     const double x = px[p], y = py[p], z = pz[p];
     const double rho = hypot(x, y);
     double bd2 = INFINITY;
     int bseg = 0;
-    for (int j = s; j < B.n_seg; j += S) {
+    auto closest = [&](int j) {
       const double* g = B.segs + (size_t)j * SEG_STRIDE;
       const double ax = g[S_ABR], ay = g[S_ABZ];
       double t = ((rho - g[S_P0R]) * ax + (z - g[S_P0Z]) * ay) / g[S_ABAB];
@@ -426,6 +510,14 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
         bd2 = d2;
         bseg = j;
       }
+    };
+    if (FAR) {
+      // the closest segment of a point on the surface lies in a cluster
+      // whose sphere holds the point, never in a far one: only the
+      // offset of such points depends on this search
+      local_segments(B, B.cl ? far_clusters(B, rho * rho, z) : 0ull, s, S, closest);
+    } else {
+      for (int j = s; j < B.n_seg; j += S) closest(j);
     }
     sh.u.a.bd2[tid] = bd2;
     sh.u.a.bseg[tid] = bseg;
@@ -470,12 +562,14 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
   acc32.zero();
   const double re = on ? sh.re[p] : 0.0, ze = on ? sh.ze[p] : 0.0, re2 = re * re;
   int n_near = 0;  // near segments handed to the point's cooperative pass
+  const unsigned long long farm = (FAR && B.cl && on) ? far_clusters(B, re2, ze) : 0ull;
   if (on) {
     // Many segments per slice: stride over whole segments from j0. In the
     // wave-major layout j0 is made wavefront-uniform (readfirstlane), so
-    // the table reads below compile to scalar loads.
+    // the table reads below compile to scalar loads. Far-rule gathers skip
+    // the segments of far clusters (their expansions are added below).
     auto whole_segments = [&](int j0) {
-      for (int j = j0; j < B.n_seg; j += S) {
+      auto seg = [&](int j) {
         const cseg_ptr g = (cseg_ptr)B.segs + (size_t)j * SEG_STRIDE;
         const double L = g[S_L];
         double proj, dist;
@@ -490,7 +584,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
               sh.u.n.seg[n_near++][tid] = j;
             else
               acc.add(near_segment(re, re2, ze, B.segs + (size_t)j * SEG_STRIDE, B.sigma[j], B.sigma[j + 1], proj,
-                                   dist, sh.logt));
+                                   dist, sh.logt, nw));
           }
           const int rl = near ? 3 : (dist >= B.k2 * L ? 2 : (dist >= B.k4 * L ? 1 : 0));
           const unsigned long long b0 = __ballot(rl == 0), b1 = __ballot(rl == 1), b2 = __ballot(rl == 2);
@@ -504,7 +598,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
             sh.u.n.seg[n_near++][tid] = j;
           else
             acc.add(near_segment(re, re2, ze, B.segs + (size_t)j * SEG_STRIDE, B.sigma[j], B.sigma[j + 1], proj,
-                                 dist, sh.logt));
+                                 dist, sh.logt, nw));
         } else if (PR == SPT_PREC_MIXED && dist >= 2.0 * L) {
           const cnode_ptr nd = (cnode_ptr)B.nodes + (size_t)j * B.nq;
 #pragma unroll 1
@@ -530,7 +624,11 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
             }
           }
         }
-      }
+      };
+      if (FAR)
+        local_segments(B, farm, j0, S, seg);
+      else
+        for (int j = j0; j < B.n_seg; j += S) seg(j);
     };
     if (B.n_seg >= 4 * S) {
       if (wave_major)
@@ -547,6 +645,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
       for (int u = s; u < n_units; u += S) {
         const int j = u / n_sub;
         const int sub = u - j * n_sub;
+        if (FAR && B.cl && ((farm >> B.seg_cl[j]) & 1ull)) continue;
         const double* g = B.segs + (size_t)j * SEG_STRIDE;
         const double L = g[S_L];
         double proj, dist;
@@ -556,7 +655,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
             if (n_near < kNearSlots)
               sh.u.n.seg[n_near++][tid] = j;
             else
-              acc.add(near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, sh.logt));
+              acc.add(near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, sh.logt, nw));
           }
         } else if (FAR) {
           if (sub == 0)
@@ -589,6 +688,10 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
       }
     }
   }
+  if (FAR && B.cl && on) {
+    for (int c = s; c < B.cl_n; c += S)
+      if ((farm >> c) & 1ull) cluster_expansion(B, c, re, re2, ze, acc);
+  }
   sh.u.n.cnt[tid] = n_near;
   {
     const unsigned long long lanes = __ballot(n_near > 0);
@@ -609,7 +712,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
         const double* g = B.segs + (size_t)j * SEG_STRIDE;
         double proj, dist;
         segment_proj(re, ze, g, proj, dist);
-        const NearRule r = near_rule(g, proj, dist);
+        const NearRule r = near_rule(g, proj, dist, nw);
         near_nodes(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], r, (s - base % S + S) % S, S, acc, sh.logt);
         base += 8 * r.count;
       }
@@ -1547,7 +1650,13 @@ struct LockParams {
   MlpView V;               // the surrogate (when sur != nullptr): split BF16
   Mlp64View V64;           // and FP64 forms
   const double* imgF;      // image-charge BEM force per particle (n x 3), or nullptr
+  const int32_t* order;    // ls_field_kernel's particle order (a permutation of 0..n-1), or nullptr
 };
+
+// Particle taken by position k of ls_field_kernel's sweep.
+__device__ __forceinline__ int64_t ls_field_particle(const LockParams& L, int64_t k) {
+  return L.order ? (int64_t)L.order[k] : k;
+}
 
 __global__ void ls_pgroup_kernel(const int64_t* offsets, int n_groups, int32_t* pgroup) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1652,7 +1761,7 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_field_kernel(LockPa
   const int tid = threadIdx.x;
   const spt_geometry& G = L.T.G;
   if (tid < m_all) {
-    const int64_t gi = base + tid;
+    const int64_t gi = ls_field_particle(L, base + tid);
     const int g = L.pgroup[gi];
     act[tid] = (L.g_flags[g] & LS_RUNNING) && L.active[gi];
     sx[tid] = LKB(48);
@@ -1704,7 +1813,7 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_field_kernel(LockPa
   bem_pert_any<T, NMAX, PR>(L.T.bem, m, sh.px, sh.py, sh.pz, sh.ox, sh.oy, sh.oz, sh.bem);
   if (tid < m) {
     const int i = sh.cidx[tid];
-    const int64_t gi = base + i;
+    const int64_t gi = ls_field_particle(L, base + i);
     double qx = sh.ox[tid], qy = sh.oy[tid], qz = sh.oz[tid];
     if (L.sur && L.sur[gi] == 2) {  // blend with the parked surrogate perturbation
       const double w = surrogate_weight(L.V.cfg, G, sx[i], sy[i], sz[i]);
@@ -1720,7 +1829,7 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_field_kernel(LockPa
     ls_keep_field(L, gi, st);
   }
   if (tid < m_all && act[tid] && !sh.need[tid]) {
-    const int64_t gi = base + tid;
+    const int64_t gi = ls_field_particle(L, base + tid);
     double ex, ey, ez;
     geometry_field_point(G, sx[tid], sy[tid], sz[tid], 0, 0, 0, ex, ey, ez);
     LKB(51) = ex;
@@ -2160,6 +2269,14 @@ struct spt_ctx {
   double4* d_mp = nullptr;            // spt_bem_multipole table (kMultipoleMaxTerms entries)
   int mp_n = 0;
   double mp_rref = 0.0, mp_rmin = 0.0;
+  // spt_bem_clusters: one allocation holding the cluster table, the
+  // moments and the recurrence coefficients; the segment ranges and the
+  // per-segment cluster index in another
+  double* d_cl = nullptr;
+  int* d_cl_idx = nullptr;
+  int cl_n = 0, cl_terms = 0;
+  double cl_tol = 0.0;
+  double near_w = kRefNearWidth;      // spt_bem_near_width
   // scratch
   double* d_kbuf = nullptr;
   size_t kbuf_elems = 0;
@@ -2244,6 +2361,17 @@ static BemView bem_view(const spt_ctx* c) {
   B.mp_n = c->mp_n;
   B.mp_rref = c->mp_rref;
   B.mp_rmin2 = c->mp_rmin * c->mp_rmin;
+  const bool cl = c->far_k4 > 0.0 && c->cl_n > 0;
+  B.cl = cl ? reinterpret_cast<const double4*>(c->d_cl) : nullptr;
+  B.cl_mom = cl ? c->d_cl + 4 * (size_t)c->cl_n : nullptr;
+  B.cl_coef = cl ? reinterpret_cast<const double2*>(c->d_cl + 4 * (size_t)c->cl_n + (size_t)c->cl_n * c->cl_terms)
+                 : nullptr;
+  B.cl_seg = cl ? c->d_cl_idx : nullptr;
+  B.seg_cl = cl ? c->d_cl_idx + c->cl_n + 1 : nullptr;
+  B.cl_n = cl ? c->cl_n : 0;
+  B.cl_terms = c->cl_terms;
+  B.cl_tol = c->cl_tol;
+  B.near_w = c->far_k4 > 0.0 ? c->near_w : kRefNearWidth;
   return B;
 }
 
@@ -2255,7 +2383,8 @@ static void free_bem(spt_ctx* c) {
   if (c->d_coarse) (void)hipFree(c->d_coarse);
   c->d_prof = c->d_sigma = c->d_segs = nullptr;
   c->d_nodes = c->d_coarse = nullptr;
-  c->mp_n = 0;  // the expansion belongs to the uploaded charge
+  c->mp_n = 0;  // the expansions belong to the uploaded charge
+  c->cl_n = 0;
   c->bem_nodes = 0;
 }
 
@@ -2299,6 +2428,8 @@ void spt_destroy(spt_ctx* c) {
   (void)hipSetDevice(c->device);
   free_bem(c);
   if (c->d_mp) (void)hipFree(c->d_mp);
+  if (c->d_cl) (void)hipFree(c->d_cl);
+  if (c->d_cl_idx) (void)hipFree(c->d_cl_idx);
   if (c->d_kbuf) (void)hipFree(c->d_kbuf);
   if (c->d_queue) (void)hipFree(c->d_queue);
   if (c->d_pgroup) (void)hipFree(c->d_pgroup);
@@ -2416,6 +2547,65 @@ int spt_bem_multipole(spt_ctx* c, const double* moments, int32_t n_terms, double
   c->mp_n = n_terms;
   c->mp_rref = r_ref;
   c->mp_rmin = r_min;
+  return SPT_OK;
+}
+
+int spt_bem_clusters(spt_ctx* c, const int32_t* seg_begin, int32_t n_clusters, const double* z_c, const double* r_c,
+                     const double* moments, int32_t n_terms, double kappa, double tol) {
+  if (!c) return SPT_EINVAL;
+  if (n_clusters == 0) {
+    c->cl_n = 0;
+    return SPT_OK;
+  }
+  if (c->bem_nodes == 0) return fail(c, SPT_ENOBEM, "spt_bem_clusters: no BEM table uploaded");
+  const int n_seg = c->bem_nodes - 1;
+  if (!seg_begin || !z_c || !r_c || !moments || n_clusters < 1 || n_clusters > kMaxClusters || n_terms < 1 ||
+      n_terms > kClusterMaxTerms || !(kappa > 1.0) || !(tol > 0.0))
+    return fail(c, SPT_EINVAL, "spt_bem_clusters: need 1 <= n_clusters <= 64, 1 <= n_terms <= 256, kappa > 1, tol > 0");
+  if (seg_begin[0] != 0 || seg_begin[n_clusters] != n_seg)
+    return fail(c, SPT_EINVAL, "spt_bem_clusters: seg_begin must run from 0 to the segment count");
+  for (int k = 0; k < n_clusters; ++k)
+    if (seg_begin[k + 1] <= seg_begin[k] || !(r_c[k] > 0.0))
+      return fail(c, SPT_EINVAL, "spt_bem_clusters: clusters need increasing segment ranges and r_c > 0");
+  HIPCHK(c, spt_begin(c));
+  const size_t nd = 4 * (size_t)n_clusters + (size_t)n_clusters * n_terms + 2 * (size_t)n_terms;
+  std::vector<double> tab(nd);
+  for (int k = 0; k < n_clusters; ++k) {
+    tab[4 * k] = z_c[k];
+    tab[4 * k + 1] = r_c[k];
+    tab[4 * k + 2] = (kappa * r_c[k]) * (kappa * r_c[k]);
+    tab[4 * k + 3] = 0.0;
+  }
+  std::copy(moments, moments + (size_t)n_clusters * n_terms, tab.begin() + 4 * n_clusters);
+  for (int l = 0; l < n_terms; ++l) {
+    tab[4 * n_clusters + (size_t)n_clusters * n_terms + 2 * l] = (2.0 * l + 1.0) / (l + 1.0);
+    tab[4 * n_clusters + (size_t)n_clusters * n_terms + 2 * l + 1] = (double)l / (l + 1.0);
+  }
+  std::vector<int> idx(n_clusters + 1 + n_seg);
+  for (int k = 0; k <= n_clusters; ++k) idx[k] = seg_begin[k];
+  for (int k = 0; k < n_clusters; ++k)
+    for (int j = seg_begin[k]; j < seg_begin[k + 1]; ++j) idx[n_clusters + 1 + j] = k;
+  if (c->d_cl) (void)hipFree(c->d_cl);
+  if (c->d_cl_idx) (void)hipFree(c->d_cl_idx);
+  c->d_cl = nullptr;
+  c->d_cl_idx = nullptr;
+  c->cl_n = 0;
+  HIPCHK(c, hipMalloc(&c->d_cl, sizeof(double) * nd));
+  HIPCHK(c, hipMalloc(&c->d_cl_idx, sizeof(int) * idx.size()));
+  HIPCHK(c, hipMemcpyAsync(c->d_cl, tab.data(), sizeof(double) * nd, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_cl_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->cl_n = n_clusters;
+  c->cl_terms = n_terms;
+  c->cl_tol = tol;
+  return SPT_OK;
+}
+
+int spt_bem_near_width(spt_ctx* c, double width) {
+  if (!c) return SPT_EINVAL;
+  if (width == 0.0) width = kRefNearWidth;
+  if (!(width > 0.0 && width <= 4.0)) return fail(c, SPT_EINVAL, "spt_bem_near_width: need 0 < width <= 4 (0: 0.5)");
+  c->near_w = width;
   return SPT_OK;
 }
 
@@ -2938,6 +3128,17 @@ static int trace_impl(spt_ctx* c, const spt_trace_args* a, bool no_alloc) {
   }
   if (a->precision == SPT_PREC_MIXED && a->mode == SPT_MODE_PERSISTENT && T != 256)
     return fail(c, SPT_EINVAL, "mixed precision in persistent mode is built for 256-thread workgroups only");
+  if (a->field_order && a->mode == SPT_MODE_LOCKSTEP) {
+    // every index the field gather takes must be a particle, each once
+    std::vector<int32_t> ord((size_t)a->n_particles);
+    HIPCHK(c, hipMemcpy(ord.data(), a->field_order, sizeof(int32_t) * ord.size(), hipMemcpyDeviceToHost));
+    std::vector<unsigned char> seen(ord.size(), 0);
+    for (int32_t k : ord) {
+      if (k < 0 || (int64_t)k >= a->n_particles || seen[k])
+        return fail(c, SPT_EINVAL, "spt_trace: field_order must be a permutation of 0 .. n_particles - 1");
+      seen[k] = 1;
+    }
+  }
   const size_t need = (size_t)(a->mode == SPT_MODE_LOCKSTEP ? kSlotsLockstep : kSlots) * (size_t)a->n_particles;
   if (no_alloc) {
     if (a->mode != SPT_MODE_PERSISTENT) return fail(c, SPT_EINVAL, "spt_trace_noalloc runs the persistent mode only");
@@ -3011,6 +3212,7 @@ static int trace_impl(spt_ctx* c, const spt_trace_args* a, bool no_alloc) {
     L.done = c->d_done;
     L.sur = a->use_surrogate ? c->d_sur : nullptr;
     L.fvalid = c->d_fvalid;
+    L.order = a->field_order;
     memset(&L.V, 0, sizeof(L.V));
     memset(&L.V64, 0, sizeof(L.V64));
     L.imgF = nullptr;

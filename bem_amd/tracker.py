@@ -50,7 +50,8 @@ class SpecificParticleTracer:
     def __init__(self, initial_particles, n_emit, geometry, screens, t_out=None,
                  plummer_radius=DEFAULT_PLUMMER_RADIUS, backend="mi355x", n_workers=1, rtol=None, atol=None,
                  t_max=None, z_max=None, group_sizes=None, block_threads=0, device=0, max_steps=10_000_000,
-                 mode="auto", precision="fp64", surrogate=None, far_rule=None, multipole=None):
+                 mode="auto", precision="fp64", surrogate=None, far_rule=None, multipole=None, near_width=None,
+                 clusters=None, spatial_order=False):
         if n_workers != 1:
             raise ValueError(f"n_workers is only supported for backend='cpu' (got backend={backend!r}, "
                              f"n_workers={n_workers!r})")
@@ -102,10 +103,18 @@ class SpecificParticleTracer:
             if geometry.bem_table() is None:
                 raise ValueError("far_rule applies to BEM geometries only")
         self.far_rule = far_rule
+        if (near_width is not None or clusters is not None) and far_rule is None:
+            raise ValueError("near_width and clusters refine the far rule: set far_rule too")
+        if clusters is not None and clusters.geometry is not geometry:
+            raise ValueError("the cluster expansions were built for a different geometry object")
+        self.near_width = near_width
+        self.clusters = clusters
         if multipole is not None and multipole.geometry is not geometry:
             raise ValueError("the multipole expansion was fitted for a different geometry object")
         self.multipole = multipole
         self.field_shape = None  # (threads, points per block) of the field gather; None = automatic
+        self.spatial_order = bool(spatial_order)
+        self._field_order = None
         self.max_steps = int(max_steps)
         self._setup_state(group_sizes)
         self.t_max = t_max if t_max is not None else self._default_t_max()
@@ -188,6 +197,34 @@ class SpecificParticleTracer:
         ctx.check(ctx.lib.spt_device_cus(ctx.h, ctypes.byref(n_cu)), "spt_device_cus")
         return _lib.MODE_LOCKSTEP if self.n_groups < 2 * n_cu.value else _lib.MODE_PERSISTENT
 
+    def field_order(self, ctx):
+        """The lockstep field gather's particle order: particles sorted by the
+        profile node closest to their initial (rho, z) (stable, so ties keep
+        index order), computed once on the device. Neighbouring lanes then
+        hold nearby points, so a wavefront's segment rules and cluster
+        decisions agree; the field is axisymmetric, so the azimuth is
+        irrelevant. None for non-BEM geometries or when switched off (the
+        default: on the 262k-electron post trace it measured neutral, since
+        the far-rule nodes it saves are no longer what the gather's time
+        goes to)."""
+        tab = self.geometry.bem_table()
+        if not self.spatial_order or tab is None:
+            return None
+        if self._field_order is None:
+            import torch
+
+            dev = ctx.torch_device
+            prof = torch.as_tensor(np.asarray(tab[0], dtype=np.float64), device=dev)
+            p = torch.as_tensor(self.pos0, device=dev)
+            rho, z = torch.hypot(p[:, 0], p[:, 1]), p[:, 2]
+            best = torch.empty(p.shape[0], dtype=torch.int64, device=dev)
+            step = max(1, (1 << 24) // prof.shape[0])
+            for a in range(0, p.shape[0], step):
+                d2 = (rho[a:a + step, None] - prof[None, :, 0]) ** 2 + (z[a:a + step, None] - prof[None, :, 1]) ** 2
+                best[a:a + step] = torch.argmin(d2, dim=1)
+            self._field_order = torch.argsort(best, stable=True).to(torch.int32).contiguous()
+        return self._field_order
+
     def trace(self, event_capacity=None, no_alloc=False):
         """Run spt_trace; returns the raw device results as NumPy arrays.
         no_alloc: call spt_trace_noalloc instead (persistent mode, the
@@ -261,12 +298,17 @@ class SpecificParticleTracer:
         a.snapshots = _lib.ptr(snaps)
         a.snapshot_capacity = snap_cap
         a.snapshot_count = _lib.ptr(snap_count)
+        a.field_order = _lib.ptr(self.field_order(ctx)) if a.mode == _lib.MODE_LOCKSTEP else None
 
         ctx.bind_stream()
         shape = getattr(self, "field_shape", None) or (0, 0)
         ctx.check(ctx.lib.spt_set_field_shape(ctx.h, int(shape[0]), int(shape[1])), "spt_set_field_shape")
         k4, k2 = self.far_rule or (0.0, 0.0)
         ctx.check(ctx.lib.spt_bem_far_rule(ctx.h, k4, k2), "spt_bem_far_rule")
+        if self.near_width is not None:
+            ctx.check(ctx.lib.spt_bem_near_width(ctx.h, float(self.near_width)), "spt_bem_near_width")
+        if self.clusters is not None:
+            self.clusters.install(ctx)
         if self.multipole is not None and not self.multipole.inert:
             self.multipole.install(ctx)
         t0 = torch.cuda.Event(enable_timing=True)
@@ -280,6 +322,8 @@ class SpecificParticleTracer:
         finally:
             ctx.lib.spt_set_field_shape(ctx.h, 0, 0)
             ctx.lib.spt_bem_far_rule(ctx.h, 0.0, 0.0)
+            ctx.lib.spt_bem_near_width(ctx.h, 0.0)
+            ctx.lib.spt_bem_clusters(ctx.h, None, 0, None, None, None, 0, 0.0, 0.0)
             ctx.lib.spt_bem_multipole(ctx.h, None, 0, 0.0, 0.0)
         t1.record()
         ctx.sync()

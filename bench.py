@@ -48,8 +48,14 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-RK-stages/sec, 1 GPU, surrogate+batched vs plain at field err ≤1e-6"
 PEAK_FP64_TFLOPS = 78.6
 C_RING = 120          # FP64 flops per ring-kernel node (K, E, log, kernel algebra, accumulation)
-TRAFFIC_BYTES = 19.8e6  # HBM bytes per 262,144-point field-gather launch at (256, 64), from the rocprofv3 counter runs
-FAR_TRAFFIC_BYTES = 13.9e6  # the same with the far rule (2, 16)
+C_TERM = 16    # FP64 flops per term of an axis-centred cluster expansion
+# HBM bytes per 262,144-point field-gather launch from the rocprofv3 counter runs (FETCH_SIZE and WRITE_SIZE at
+# the timed launch shape, scaled by the same counters' reading for a copy of known size)
+TRAFFIC_BYTES = 19.8e6
+TRAFFIC_SOURCE = ("FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB copy), fixed rule at (256, 64), "
+                  "profiles/bem_field_gather_post400_262k.md")
+FAST_TRAFFIC_BYTES = None
+FAST_TRAFFIC_SOURCE = None
 
 
 def workload(n_el=262144, group=16, n_segments=400, seed=0, rtol=1e-8):
@@ -61,7 +67,8 @@ def workload(n_el=262144, group=16, n_segments=400, seed=0, rtol=1e-8):
                 rtol=rtol, atol=1e-12, n_el=n_el)
 
 
-def make_tracer(wl, geo, surrogate=None, precision="pairs32", mode="lockstep", far_rule=None, multipole=None):
+def make_tracer(wl, geo, surrogate=None, precision="pairs32", mode="lockstep", far_rule=None, multipole=None,
+                near_width=None, clusters=None):
     from bem_amd import SpecificParticleTracer
     from bem_amd.particles import ParticleGroup
 
@@ -70,7 +77,7 @@ def make_tracer(wl, geo, surrogate=None, precision="pairs32", mode="lockstep", f
     pg = ParticleGroup(dict(src, weight=np.full(n, 1.602176634e-19), status=np.ones(n), species="electron"))
     return SpecificParticleTracer(pg, wl["group"], geo, wl["screens"], rtol=wl["rtol"], atol=wl["atol"],
                                   z_max=wl["z_max"], mode=mode, precision=precision, surrogate=surrogate,
-                                  far_rule=far_rule, multipole=multipole)
+                                  far_rule=far_rule, multipole=multipole, near_width=near_width, clusters=clusters)
 
 
 def timed_traces(tr, steps, warmup):
@@ -109,34 +116,56 @@ def gather_points(wl, n_pts=262144):
     return pts
 
 
-def far_rule_nodes_per_point(geo, pts, k4, k2, sample=8192):
-    """Mean ring-kernel nodes the far rule sums per point (8 per segment at
-    L <= D < k4 L, 4 at k4 L <= D < k2 L, 2 beyond; near segments, D < L,
-    take the sinh rule and are not counted), over the first `sample`
-    points."""
+def gather_work_per_point(geo, pts, k4, k2, near_width=0.5, clusters=None, sample=8192):
+    """Mean algorithmic work per point of a far-rule gather over the first
+    `sample` points, counted on the host from the same rules the device
+    applies per point: ring-kernel nodes of the far rule on segments outside
+    far clusters (8 per segment at L <= D < k4 L, 4 below k2 L, 2 beyond),
+    sinh-rule nodes of near segments (D < L: 8 per interval of width
+    near_width), and expansion terms of far clusters (the truncation rule of
+    bem_amd.bem.clusters). Returns (ring nodes, near nodes, expansion terms)
+    per point."""
+    from bem_amd.bem import clusters as bcl
+
     prof = np.asarray(geo.solution.profile)
     p0, t = prof[:-1], np.diff(prof, axis=0)
     L = np.linalg.norm(t, axis=1)
+    u = t / L[:, None]
     q = pts[:sample]
     rho, z = np.hypot(q[:, 0], q[:, 1]), q[:, 2]
-    tt = np.clip(((rho[:, None] - p0[None, :, 0]) * t[None, :, 0] + (z[:, None] - p0[None, :, 1]) * t[None, :, 1])
-                 / (t ** 2).sum(1)[None], 0.0, 1.0)
-    D = np.hypot(rho[:, None] - (p0[None, :, 0] + tt * t[None, :, 0]), z[:, None] - (p0[None, :, 1] + tt * t[None, :, 1]))
+    dr, dz = rho[:, None] - p0[None, :, 0], z[:, None] - p0[None, :, 1]
+    s0 = np.clip(dr * u[None, :, 0] + dz * u[None, :, 1], 0.0, L[None])
+    D = np.hypot(dr - s0 * u[None, :, 0], dz - s0 * u[None, :, 1])
     Lr = L[None]
-    nodes = np.where(D < Lr, 0, np.where(D < k4 * Lr, 8, np.where(D < k2 * Lr, 4, 2)))
-    return float(nodes.sum(1).mean())
+    local = np.ones_like(D, dtype=bool)
+    terms = np.zeros(len(q))
+    if clusters is not None:
+        for c in range(clusters.n_clusters):
+            r = np.hypot(rho, z - clusters.zc[c])
+            far = r >= clusters.kappa * clusters.rc[c]
+            local[:, clusters.seg_begin[c]:clusters.seg_begin[c + 1]] &= ~far[:, None]
+            qq = np.where(far, clusters.rc[c] / np.maximum(r, 1e-300), 0.0)
+            terms += np.where(far, bcl.terms_needed(qq, clusters.tol, clusters.n_terms), 0)
+    ring = np.where(D < Lr, 0, np.where(D < k4 * Lr, 8, np.where(D < k2 * Lr, 4, 2)))
+    d = np.maximum(D, 1e-15 * Lr)
+    span = np.arcsinh((Lr - s0) / d) - np.arcsinh(-s0 / d)
+    near = np.where(D < Lr, 8 * np.maximum(1, np.ceil(span / near_width)), 0)
+    return (float(np.where(local, ring, 0).sum(1).mean()), float(np.where(local, near, 0).sum(1).mean()),
+            float(terms.mean()))
 
 
-def field_gather_timing(wl, geo, n_pts=262144, reps=10, shape=None, far_rule=None):
+def field_gather_timing(wl, geo, n_pts=262144, reps=10, shape=None, budget=None):
     """Average launch duration of the BEM field gather over gather_points(),
     timed with HIP events on the stream the kernel is launched on (the
-    library is bound to torch's current stream); far_rule = (k4, k2) times
-    the distance-adaptive far rule instead of the fixed one."""
+    library is bound to torch's current stream); with a FieldBudget, the
+    far-rule gather of the fast form (far rule, near width, clusters)."""
     import contextlib
 
     pts = gather_points(wl, n_pts)
     ctx = geo._ctx()
-    with (geo.far_rule(*far_rule) if far_rule else contextlib.nullcontext()):
+    cm = (geo.far_rule(*budget.far_rule, near_width=budget.near_width, clusters=budget.clusters) if budget
+          else contextlib.nullcontext())
+    with cm:
         return _gather_timing(ctx, pts, n_pts, reps, shape)
 
 
@@ -203,11 +232,30 @@ def _cpu_group_rate(gi):
     return count[0], time.perf_counter() - t0[0]
 
 
-def cpu_baseline(wl, seconds=15.0, workers=None):
-    """The oracle on the first `workers` groups of the workload, one forked
-    process per group and host core (the reference's own process-level
-    group partitioning), each for `seconds` of work. Must run before
-    anything touches the GPU (the workers are forked)."""
+def host_cpu():
+    """CPU model, sockets, cores per socket and threads per core (lscpu)."""
+    import subprocess
+
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_baseline(wl, seconds=15.0, n_groups=64, workers=None):
+    """The oracle on the first n_groups groups of the workload, spread over
+    `workers` forked processes (the reference's own process-level group
+    partitioning; default: every host core this process may use, at most
+    16, the GPU box's share), each group traced for seconds * workers /
+    n_groups of work, so the whole sample takes ~`seconds`. Rate = all
+    particle-stages / wall time of the pool. Must run before anything
+    touches the GPU (the workers are forked)."""
     import multiprocessing as mp
 
     from oracle import bem as ob
@@ -216,18 +264,28 @@ def cpu_baseline(wl, seconds=15.0, workers=None):
     R, H = wl["R"], wl["H"]
     prof = post_profile(R, H, wl["n_segments"])
     sigma = ob.solve(prof, lambda r, z: wl["E_gun"] * z)
-    _CPU.update(wl=wl, seconds=seconds,
+    avail = len(os.sched_getaffinity(0))
+    if workers is None:
+        workers = max(1, min(16, avail))
+    _CPU.update(wl=wl, seconds=seconds * workers / n_groups,
                 geom=dict(kind="post_bem", E_gun=wl["E_gun"], R=R, H=H, profile=prof, sigma=sigma,
                           z_handoff=20 * (R + H), kill_z_below=0.0, z_max=wl["z_max"]))
-    if workers is None:
-        workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    t0 = time.perf_counter()
     with mp.get_context("fork").Pool(workers) as pool:
-        parts = pool.map(_cpu_group_rate, range(workers))
+        parts = pool.map(_cpu_group_rate, range(n_groups), chunksize=1)
+    wall = time.perf_counter() - t0
     count = sum(c for c, _ in parts)
-    el = max(e for _, e in parts)
-    return dict(value=count / el, unit="particle-RK-stages/s", cores=workers, kind="port",
-                sample=f"oracle (NumPy FP64) on groups 0..{workers - 1} ({wl['group']} electrons each) of the "
-                       f"workload, one process per group and core, {el:.1f} s each ({count} particle-stages)")
+    cpu = host_cpu()
+    return dict(value=count / wall, unit="particle-RK-stages/s", cores=workers, kind="port",
+                cpu_model=cpu.get("Model name"), sockets=cpu.get("Socket(s)"),
+                cores_per_socket=cpu.get("Core(s) per socket"), threads_per_core=cpu.get("Thread(s) per core"),
+                host_cpus_online=cpu.get("CPU(s)"), cpus_available_to_process=avail,
+                cap=f"{workers} worker processes: every CPU this process may use, capped at 16 (the GPU box's "
+                    "share of its host)",
+                sample=f"oracle (NumPy FP64, one BLAS thread per process) on groups 0..{n_groups - 1} "
+                       f"({wl['group']} electrons each) of the workload, each traced from birth for "
+                       f"{_CPU['seconds']:.2f} s of work (the first, near-surface attempts); {wall:.1f} s wall, "
+                       f"{count} particle-stages")
 
 
 def c2_plain_rate(steps=3):
@@ -283,7 +341,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--electrons", type=int, default=262144)
-    ap.add_argument("--tol", type=float, default=1e-6, help="surrogate field-error tolerance for the headline value")
+    ap.add_argument("--tol", type=float, default=1e-6, help="field-error tolerance of the headline value")
     ap.add_argument("--train-steps", type=int, default=60000)
     ap.add_argument("--no-sweep", action="store_true", help="skip the looser-tolerance runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -291,33 +349,30 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--field-only", action="store_true",
                     help="only the field-gather timing and a 256 MiB device copy (for hardware-counter runs)")
-    ap.add_argument("--field-shape", default="0,0",
-                    help="threads,points-per-block of the field gather for --field-only (0,0 = automatic)")
+    ap.add_argument("--plain-gather", action="store_true",
+                    help="with --field-only: the fixed-rule gather instead of the fast form's")
+    ap.add_argument("--field-shape", default="256,64",
+                    help="threads,points-per-block of the field gather for --field-only and --value-only")
     ap.add_argument("--value-only", action="store_true",
-                    help="only the headline trace (far rule and multipole zone, no network: the network is inert "
-                         "at 1e-6) after one warm-up trace, for profiler runs")
-    ap.add_argument("--no-multipole", action="store_true",
-                    help="no exterior multipole zone in the surrogate + batched traces")
-    ap.add_argument("--far-rule", default="2,16",
-                    help="k4,k2 of the distance-adaptive far rule in the surrogate + batched traces "
-                         "(and in --field-only); 0 = the fixed rule")
+                    help="only the headline trace (the field-error budget; the network is inert at 1e-6) after "
+                         "one warm-up trace, for profiler runs")
     args = ap.parse_args()
-    far_rule = tuple(float(v) for v in args.far_rule.split(","))
-    far_rule = None if far_rule[0] == 0.0 else far_rule
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the tracing path is benchmarked per GPU (groups shard across GPUs "
               "with no exchange during integration); --gpus above 1 is not supported.", file=sys.stderr)
         sys.exit(2)
     from bem_amd import surrogate
+    from bem_amd.bem import budget as bbudget
     from bem_amd.bem.geometry import PostTipBEMGeometry
 
     def say(*a):
         print(*a, file=sys.stderr, flush=True)
 
     wl = workload(n_el=args.electrons)
+    shape_arg = tuple(int(v) for v in args.field_shape.split(","))
     cpu = None
     if not args.no_cpu_baseline and not args.field_only and not args.value_only:
-        say(f"bench: CPU baseline (oracle, {args.cpu_seconds:.0f} s per process)")
+        say(f"bench: CPU baseline (oracle, 64 groups, ~{args.cpu_seconds:.0f} s)")
         cpu = cpu_baseline(wl, args.cpu_seconds)  # before the GPU is touched: the workers are forked
         say(f"bench: CPU baseline {cpu['value']:.0f} particle-stages/s on {cpu['cores']} cores")
     t0 = time.perf_counter()
@@ -326,25 +381,23 @@ def main():
     if args.field_only:
         import torch
 
-        shape = tuple(int(v) for v in args.field_shape.split(","))
-        field_ms, n_pts = field_gather_timing(wl, geo, reps=3, shape=shape, far_rule=far_rule)
+        bud = None if args.plain_gather else bbudget.fit(geo, tol=args.tol, use_multipole=False, n_val=1 << 12)
+        field_ms, n_pts = field_gather_timing(wl, geo, reps=3, shape=shape_arg, budget=bud)
         src = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-        print(json.dumps(dict(field_ms=field_ms, points=n_pts, field_shape=shape, far_rule=far_rule,
-                              copy_bytes_read=src.numel() * 4,
+        print(json.dumps(dict(field_ms=field_ms, points=n_pts, field_shape=shape_arg,
+                              gather="fixed rule" if bud is None else "fast form", copy_bytes_read=src.numel() * 4,
                               copy_bytes_written=dst.numel() * 4)))
         return
     if args.value_only:
-        from bem_amd.bem import multipole as mpl
-
-        mp = None if args.no_multipole else mpl.fit(geo, n_terms=128, tol=min(1e-8, args.tol))
-        vt = make_tracer(wl, geo, far_rule=far_rule, multipole=mp)
-        vt.field_shape = (256, 64)
+        bud = bbudget.fit(geo, tol=args.tol)
+        vt = make_tracer(wl, geo, **bud.tracer_kwargs())
+        vt.field_shape = shape_arg
         o, w = timed_traces(vt, args.steps, args.warmup)
         print(json.dumps(dict(value=sum(x["particle_stages"] for x in o) / w, wall_s=w, steps=args.steps,
-                              far_rule=far_rule, multipole_r_min=None if mp is None else mp.r_min)))
+                              field_shape=shape_arg, budget=bud.report)))
         return
     say(f"bench: BEM solve {t_solve:.2f} s; training the surrogate ({args.train_steps} steps)")
     sur = surrogate.train(geo, n_samples=1 << 20, steps=args.train_steps, batch=8192, tol=args.tol, chunk=10000)
@@ -353,46 +406,38 @@ def main():
     from bem_amd import autotune
 
     plain_tr = make_tracer(wl, geo)
-    tuning = autotune.tune(plain_tr, attempts=4)
-    say(f"bench: autotuned field shape {tuning['best']}")
-    far_err = None
-    if far_rule:
-        far_err = surrogate.far_rule_error(geo, *far_rule, n=1 << 16)
-        say(f"bench: far rule {far_rule}: field err max {far_err['field_err_max']:.2e}")
-        if far_err["field_err_max"] > args.tol:
-            say("bench: the far rule misses the tolerance; the surrogate + batched traces use the fixed rule")
-            far_rule = None
-    mp = None
-    if not args.no_multipole:
-        from bem_amd.bem import multipole as mpl
-
-        mp = mpl.fit(geo, n_terms=128, tol=min(1e-8, args.tol))
-        say(f"bench: multipole r_min {mp.r_min:.3e} m ({mp.report['r_min_over_r_ref']:.3f} r_ref)")
-        if mp.inert:
-            mp = None
-    tr = make_tracer(wl, geo, surrogate=sur, far_rule=far_rule, multipole=mp)
-    tuning_value = autotune.tune(tr, attempts=4) if (far_rule or mp) else None  # its own launch shape
-    if tuning_value is None:
-        tr.field_shape = plain_tr.field_shape
-    else:
-        say(f"bench: autotuned field shape of the fast trace {tuning_value['best']}")
+    tuning = autotune.tune(plain_tr, attempts=8)
+    say(f"bench: autotuned plain field shape {tuning['best']}")
+    bud = bbudget.fit(geo, tol=args.tol)
+    say(f"bench: field budget validated: gather err max {bud.report['gather']['field_err_max']:.2e}, multipole "
+        f"r_min {bud.multipole.r_min:.3e} m")
+    tr = make_tracer(wl, geo, surrogate=sur, **bud.tracer_kwargs())
+    tuning_value = autotune.tune(tr, attempts=None)  # whole traces: the point mix changes along the trace
+    say(f"bench: autotuned field shape of the fast trace {tuning_value['best']}")
     outs, wall = timed_traces(tr, args.steps, args.warmup)
-    say(f"bench: {args.steps} surrogate traces in {wall:.2f} s")
+    say(f"bench: {args.steps} fast traces in {wall:.2f} s")
     stages = sum(o["particle_stages"] for o in outs)
     act = sum(int(o["active_stages"].sum()) for o in outs)
     plain_outs, plain_wall = timed_traces(plain_tr, 1, 0)
     plain_value = plain_outs[0]["particle_stages"] / plain_wall
     say(f"bench: plain trace {plain_wall:.2f} s")
-    net_off = None  # the same batched trace without the network: what the network itself gains
-    if far_rule or mp:
-        t3 = make_tracer(wl, geo, far_rule=far_rule, multipole=mp)
-        t3.field_shape = tr.field_shape
-        o3, w3 = timed_traces(t3, 1, 0)
-        net_off = dict(value=o3[0]["particle_stages"] / w3, wall_s=w3,
-                       attempts_mean=float(o3[0]["attempts"].mean()),
-                       screen_t_rel_diff_vs_plain=screen_time_diff(plain_outs[0], o3[0]))
-        say(f"bench: far-rule trace without the network {w3:.2f} s")
-    base_wall = net_off["wall_s"] if net_off else plain_wall
+    # the value counts only if the fast trace reproduces the plain one: every
+    # electron's screen crossing, at times within the tolerance
+    n_el = wl["n_el"]
+    st_diff = screen_time_diff(plain_outs[0], outs[-1])
+    value_check = dict(events=int(len(outs[-1]["events"])), plain_events=int(len(plain_outs[0]["events"])),
+                       screen_t_rel_diff=st_diff, bound=args.tol,
+                       groups_done=bool(np.all(outs[-1]["status"] == 0)))
+    value_check["pass"] = bool(st_diff is not None and value_check["events"] == value_check["plain_events"] == n_el
+                               and st_diff["max"] <= args.tol and value_check["groups_done"])
+    if not value_check["pass"]:
+        say(f"bench: WARNING the fast trace does not reproduce the plain one: {value_check}")
+    t3 = make_tracer(wl, geo, **bud.tracer_kwargs())  # the same fast trace without the network
+    t3.field_shape = tr.field_shape
+    o3, w3 = timed_traces(t3, 1, 0)
+    net_off = dict(value=o3[0]["particle_stages"] / w3, wall_s=w3, attempts_mean=float(o3[0]["attempts"].mean()),
+                   screen_t_rel_diff_vs_plain=screen_time_diff(plain_outs[0], o3[0]))
+    say(f"bench: fast trace without the network {w3:.3f} s")
     sweep = []
     if not args.no_sweep:
         for tol in (1e-4, 1e-3):
@@ -400,47 +445,47 @@ def main():
             if not np.isfinite(s.cfg["band"]):
                 sweep.append(dict(tol=tol, band_m=None))
                 continue
-            t2 = make_tracer(wl, geo, surrogate=s, far_rule=far_rule, multipole=mp)
+            t2 = make_tracer(wl, geo, surrogate=s, **bud.tracer_kwargs())
             t2.field_shape = tr.field_shape
             o, w = timed_traces(t2, 1, 0)
             sweep.append(dict(tol=tol, band_m=s.cfg["band"], value=o[0]["particle_stages"] / w,
-                              wall_s=w, gain_wall_vs_plain=plain_wall / w, network_gain_wall=base_wall / w,
+                              wall_s=w, gain_wall_vs_plain=plain_wall / w, network_gain_wall=w3 / w,
                               attempts_mean=float(o[0]["attempts"].mean()),
                               screen_t_rel_diff_vs_plain=screen_time_diff(plain_outs[0], o[0])))
-            say(f"bench: tol {tol:g} trace {w:.2f} s")
-    field_ms, n_pts = field_gather_timing(wl, geo, shape=plain_tr.field_shape)
+            say(f"bench: tol {tol:g} trace {w:.3f} s")
+    # dominant kernel of the value trace: the fast form's field gather, at the
+    # value trace's launch shape, timed with HIP events on its stream
+    pts = gather_points(wl)
+    n_pts = len(pts)
+    fast_shape = tuple(tr.field_shape or (0, 0))
+    fast_ms, _ = field_gather_timing(wl, geo, shape=fast_shape, budget=bud)
+    ring, near, terms = gather_work_per_point(geo, pts, *bud.far_rule, near_width=bud.near_width,
+                                              clusters=bud.clusters)
+    fast_flops = n_pts * ((ring + near) * C_RING + terms * C_TERM)
+    fa = fast_flops / (fast_ms / 1e3) / 1e12
+    fast_roof = {"bound": "fp64_valu", "achieved": fa, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                 "frac": fa / PEAK_FP64_TFLOPS, "traffic": FAST_TRAFFIC_BYTES,
+                 "traffic_source": FAST_TRAFFIC_SOURCE,
+                 "kernel": "bem_pert_kernel, far-rule instantiation with the field budget (far rule "
+                           f"{bud.far_rule}, near width {bud.near_width}, {bud.clusters.n_clusters} cluster "
+                           "expansions): the same device code as the value trace's ls_field_kernel",
+                 "kernel_ms_mean": fast_ms, "points_per_launch": n_pts, "field_shape": list(fast_shape),
+                 "ring_nodes_per_point": ring, "near_nodes_per_point": near, "expansion_terms_per_point": terms,
+                 "flops_per_launch": fast_flops,
+                 "note": "FP64 VALU-bound (MI355X FP64 vector peak = FP64 MFMA peak = 78.6 TFLOP/s); algorithmic "
+                         "flops = (far-rule ring nodes on segments outside far clusters + near-rule nodes) x 120 "
+                         "+ cluster-expansion terms x 16, counted on the host for the same points"}
+    plain_shape = tuple(plain_tr.field_shape or (0, 0))
+    field_ms, _ = field_gather_timing(wl, geo, shape=plain_shape)
     n_quad = wl["n_segments"] * 8 * 4
     plain_achieved = n_pts * n_quad * C_RING / (field_ms / 1e3) / 1e12
-    far_gather = None
-    if far_rule:
-        far_ms, _ = field_gather_timing(wl, geo, shape=tr.field_shape, far_rule=far_rule)
-        npp = far_rule_nodes_per_point(geo, gather_points(wl, n_pts), *far_rule)
-        far_gather = dict(kernel_ms_mean=far_ms, nodes_per_point_mean=npp, points_per_launch=n_pts,
-                          flops_per_launch=n_pts * npp * C_RING)
-    value = stages / wall
-    plain_roof = {"bound": "mfma", "achieved": plain_achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+    plain_roof = {"bound": "fp64_valu", "achieved": plain_achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                   "frac": plain_achieved / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_BYTES,
-                  "traffic_source": "FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB copy), "
-                                    "profiles/bem_field_gather_post400_262k.md",
-                  "kernel": "bem_pert_kernel, fixed rule (BEM field gather; the same device code as the plain "
-                            "trace's ls_field_kernel)",
-                  "kernel_ms_mean": field_ms, "points_per_launch": n_pts, "flops_per_launch": n_pts * n_quad * C_RING,
-                  "note": "FP64 VALU-bound; MI355X FP64 vector peak = FP64 MFMA peak = 78.6 TFLOP/s"}
-    far_roof = None
-    if far_gather:
-        fa = far_gather["flops_per_launch"] / (far_gather["kernel_ms_mean"] / 1e3) / 1e12
-        far_roof = {"bound": "mfma", "achieved": fa, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                    "frac": fa / PEAK_FP64_TFLOPS, "traffic": FAR_TRAFFIC_BYTES,
-                    "traffic_source": "FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB copy), "
-                                      "profiles/bem_field_gather_far_rule_post400_262k.md",
-                    "kernel": f"bem_pert_kernel, far rule (k4, k2) = {far_rule} (the same device code as the "
-                              "value trace's ls_field_kernel)",
-                    "kernel_ms_mean": far_gather["kernel_ms_mean"], "points_per_launch": n_pts,
-                    "field_shape": list(tr.field_shape or (0, 0)), "traffic_field_shape": [256, 64],
-                    "nodes_per_point_mean": far_gather["nodes_per_point_mean"],
-                    "flops_per_launch": far_gather["flops_per_launch"],
-                    "note": "algorithmic flops = ring-kernel nodes the far rule sums (counted on the host for the "
-                            "same points) x 120; near-rule nodes not counted"}
+                  "traffic_source": TRAFFIC_SOURCE,
+                  "kernel": "bem_pert_kernel, fixed rule (the plain trace's field gather)",
+                  "kernel_ms_mean": field_ms, "points_per_launch": n_pts, "field_shape": list(plain_shape),
+                  "flops_per_launch": n_pts * n_quad * C_RING}
+    value = stages / wall
     res = {
         "metric": METRIC,
         "value": value,
@@ -456,34 +501,29 @@ def main():
         "data": "synthetic (seeded cap emission, numpy default_rng(0)); surrogate trained on BEM samples",
         "config": {"workload": f"post-tip 1 um on 10 um post, 400-panel BEM, {wl['n_el']} e- in "
                                f"{wl['n_el'] // wl['group']} groups of 16, rtol 1e-8, screen 90 um, anode kill 100 um",
-                   "form": f"surrogate + batched lockstep at field err <= {args.tol:g}: validated exterior "
-                           "multipole expansion of the BEM charge beyond its radius, learned network beyond its "
-                           f"band (inert if none), BEM gathers with the far rule {far_rule} (fixed rule if None) "
-                           "elsewhere; FP64 field and step control, FP32 + Kahan pair forces",
-                   "n_quad": n_quad, "field_shape": list(tr.field_shape or (0, 0)), "plain_field_shape": list(plain_tr.field_shape or (0, 0))},
+                   "form": f"surrogate + batched lockstep at field err <= {args.tol:g}: exterior multipole zone, "
+                           "learned network beyond its band (inert if none), BEM gathers with the far rule, a "
+                           "wider near rule and cluster expansions, all validated against the plain fixed rule; "
+                           "FP64 field and step control, FP32 + Kahan pair forces",
+                   "n_quad": n_quad, "field_shape": list(fast_shape), "plain_field_shape": list(plain_shape)},
+        "value_check": value_check,
         "plain_value": plain_value,
         "gain_vs_plain": value / plain_value,
-        "far_rule": None if not far_rule else dict(far_err, screen_t_rel_diff_vs_plain=screen_time_diff(
-            plain_outs[0], outs[-1])),
-        "multipole": None if mp is None else {k: v for k, v in mp.report.items()
-                                              if k not in ("candidates", "worst_beyond")},
+        "budget": bud.report,
         "without_network": net_off,
         "surrogate": {"tol": args.tol, "band_m": rep["band"] if np.isfinite(rep["band"]) else None,
-                      "band_panels": rep["band_panels"],
-                      "inert": bool(sur.inert),
+                      "band_panels": rep["band_panels"], "inert": bool(sur.inert),
                       "field_err_max": rep["field_err_max"], "field_err_median": rep["field_err_median"],
                       "field_err_p99": rep["field_err_p99"], "phi_err_max": rep["phi_err_max"],
                       "train_s": rep["t_train_s"], "labels_s": rep["t_labels_s"], "train_steps": rep["steps"],
-                      "batch": rep["batch"], "n_samples": rep["n_samples"],
-                      "screen_t_rel_diff_vs_plain": screen_time_diff(plain_outs[0], outs[-1])},
+                      "batch": rep["batch"], "n_samples": rep["n_samples"]},
         "tolerance_sweep": sweep,
         "autotune": {"best": tuning["best"], "table": tuning["table"], "sample_attempts": tuning["attempts"]},
-        "autotune_value": None if tuning_value is None else {"best": tuning_value["best"],
-                                                             "table": tuning_value["table"]},
+        "autotune_value": {"best": tuning_value["best"], "table": tuning_value["table"], "sample": "whole traces"},
         "attempts_per_group_mean": float(np.mean(outs[-1]["attempts"])),
         "plain_attempts_per_group_mean": float(np.mean(plain_outs[0]["attempts"])),
         "active_particle_stages_per_s": act / wall,
-        "roofline": far_roof if far_gather else plain_roof,
+        "roofline": fast_roof,
         "roofline_plain_gather": plain_roof,
         "c2_plain_fp64": c2_plain_rate(),
     }

@@ -210,6 +210,13 @@ typedef struct spt_trace_args {
   uint64_t* snapshot_count;     /* device, total recorded                   */
   double* t_end;                /* device, per group: t after the group's   */
                                 /* last attempt (t_max when done), or NULL  */
+  const int32_t* field_order;   /* device, n_particles, LOCKSTEP: the order */
+                                /* in which the field gather takes the      */
+                                /* particles (a permutation; a spatial sort */
+                                /* keeps each wavefront's points together), */
+                                /* or NULL for index order. It changes a    */
+                                /* fixed-rule result in the last bits only  */
+                                /* (through the slice count of its block)   */
 } spt_trace_args;
 
 /* Context. */
@@ -252,6 +259,31 @@ int spt_bem_far_rule(spt_ctx* ctx, double k4, double k2);
  * (r_min and n_terms come from a validation against the segment sums), not
  * a parity mode. */
 int spt_bem_multipole(spt_ctx* ctx, const double* moments, int32_t n_terms, double r_ref, double r_min);
+
+/* Axis-centred expansions of runs of consecutive segments for every later
+ * far-rule gather on this context (no counterpart in the reference, which
+ * sums every segment at every point, reference bem/axisymmetric.py:560-605;
+ * only active while spt_bem_far_rule is). Cluster c holds segments
+ * seg_begin[c] .. seg_begin[c+1] - 1 (seg_begin[0] = 0, seg_begin[n] = the
+ * segment count); moments[c * n_terms + l] = sum over its fixed-rule
+ * quadrature nodes of (area * sigma)_q (d_q / r_c)^l P_l((z_q - z_c) / d_q),
+ * d_q = |(rho_q, z_q - z_c)| <= r_c. A point at distance r >= kappa r_c
+ * from (0, z_c) takes the cluster's perturbation from
+ *   phi = (1/4 pi) sum_l moments[l] (r_c / r)^l P_l(cos theta) / r
+ * about (0, z_c), truncated after the first l whose tail bound
+ * sum_{k > l} (k+1) q^k (q = r_c / r) is <= tol, instead of the cluster's
+ * segments. Host arrays; 1 <= n_clusters <= 64, 1 <= n_terms <= 256,
+ * kappa > 1, tol > 0; n_clusters = 0 turns it off (spt_bem_upload does
+ * too). A field-error budget (bem_amd.bem.clusters validates it), not a
+ * parity mode. */
+int spt_bem_clusters(spt_ctx* ctx, const int32_t* seg_begin, int32_t n_clusters, const double* z_c,
+                     const double* r_c, const double* moments, int32_t n_terms, double kappa, double tol);
+
+/* Interval width (in the sinh-substituted variable) of the near rule in
+ * far-rule gathers on this context; the reference's rule, used by every
+ * fixed-rule gather, is 0.5 (reference bem/axisymmetric.py:583). 0 restores
+ * 0.5; at most 4. A field-error budget, not a parity mode. */
+int spt_bem_near_width(spt_ctx* ctx, double width);
 
 /* Collocation matrix of the P1 hat basis (reference
  * bem/axisymmetric.py:275-302,430-441), row-major into host A_out

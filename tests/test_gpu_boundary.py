@@ -60,3 +60,50 @@ def test_reference_tuple_traces_like_its_geometry(gpu):
     np.testing.assert_array_equal(runs[0]["attempts"], d["attempts"])
     np.testing.assert_array_equal(runs[0]["attempts"], runs[1]["attempts"])
     np.testing.assert_array_equal(runs[0]["pos"], runs[1]["pos"])
+
+
+def _post80_tracer(**kw):
+    from bem_amd import SpecificParticleTracer
+    from bem_amd.bem.geometry import PostTipBEMGeometry
+    from bem_amd.particles import ParticleGroup
+
+    d = golden("run_post80.npz")
+    p = golden("bem_post80.npz")
+    src = {k[3:]: d[k] for k in d.files if k.startswith("in_")}
+    n = len(src["x"])
+    pg = ParticleGroup(dict(src, weight=np.full(n, 1.602176634e-19), status=np.ones(n), species="electron"))
+    geo = PostTipBEMGeometry(-1e8, float(p["R"]), float(p["H"]), n_segments=80, sigma=p["sigma"])
+    return d, SpecificParticleTracer(pg, int(d["n_emit"]), geo, list(d["screens"]), rtol=float(d["rtol"]),
+                                     atol=float(d["atol"]), t_max=float(d["t_max"]), mode="lockstep", **kw)
+
+
+def test_field_order_leaves_the_plain_method_unchanged(gpu):
+    """The lockstep field gather's particle order (the optional spatial sort)
+    does not change the fixed-rule results here (one point per block, so no
+    slice count changes): bit-identical runs, equal to the reference run's
+    counts."""
+    runs = []
+    for so in (True, False):
+        d, tr = _post80_tracer(spatial_order=so)
+        out = tr.trace()
+        runs.append(out)
+    assert runs[0]["attempts"].tolist() == d["attempts"].tolist()
+    for k in ("attempts", "accepted", "pos", "vel", "t_death"):
+        np.testing.assert_array_equal(runs[0][k], runs[1][k], err_msg=k)
+    order = tr.field_order(tr.geometry._ctx())
+    assert order is None  # spatial_order=False
+    d, tr = _post80_tracer(spatial_order=True)
+    order = tr.field_order(tr.geometry._ctx()).cpu().numpy()
+    assert np.array_equal(np.sort(order), np.arange(len(order))) and not np.array_equal(order, np.arange(len(order)))
+
+
+def test_a_bad_field_order_is_rejected(gpu):
+    """The library checks the permutation before launching anything."""
+    import torch
+
+    d, tr = _post80_tracer(spatial_order=True)
+    bad = np.arange(tr.pos0.shape[0], dtype=np.int32)
+    bad[3] = bad[5]  # a duplicate, one particle missing
+    tr._field_order = torch.as_tensor(bad, device=tr.geometry._ctx().torch_device)
+    with pytest.raises(ValueError, match="permutation"):
+        tr.trace()

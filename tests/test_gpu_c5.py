@@ -17,8 +17,9 @@ cap workload at rtol 1e-9, run_post400_c5 the first 3 groups of the
   (groups never interact, reference batched_rk45.py:4-22).
 * The forms that change arithmetic keep the reference's events and cross
   the screen within a stated bound: FP32 + Kahan pair forces ("pairs32")
-  1e-7 of the crossing time; the field-error budget (far rule (2, 16) and
-  the validated multipole zone, each <= 1e-8 of |E|) 1e-6; both also at
+  1e-7 of the crossing time; the field-error budget (bem_amd.bem.budget:
+  far rule (2, 16), near rule of width 2, cluster expansions and the
+  multipole zone, validated together within 1e-7 of |E|) 1e-6; both also at
   full size against the FP64 trace of the same electrons."""
 import sys
 
@@ -152,7 +153,10 @@ def test_c2_workload_first_groups_match_reference_run(post400):
 
 def test_c5_forms_match_reference_run_within_bounds(post400):
     """The C5 golden groups (rtol 1e-8) under the plain FP64 method (exact),
-    FP32 + Kahan pair forces and the field-error budget."""
+    FP32 + Kahan pair forces and the field-error budget (far rule and
+    multipole zone; then all of it, with the wide near rule and cluster
+    expansions)."""
+    from bem_amd.bem import budget
     from bem_amd.bem import multipole as mpl
 
     d = golden("run_post400_c5.npz")
@@ -173,6 +177,13 @@ def test_c5_forms_match_reference_run_within_bounds(post400):
     _say(f"C5 golden groups, far rule + multipole: crossing time max rel diff {dt:.2e}, "
          f"attempts {out['attempts'].tolist()} vs {d['attempts'].tolist()}")
     assert dt <= 1e-6
+    bud = budget.fit(post400, tol=1e-6)
+    out = _tracer(post400, src, 1e-8, mode="lockstep", precision="pairs32", t_max=t_max,
+                  **bud.tracer_kwargs()).trace()
+    dt = _check_bound(out, d, [0, 1, 2], 1e-6)
+    _say(f"C5 golden groups, the whole field budget (bench.py's value form): crossing time max rel diff {dt:.2e}, "
+         f"attempts {out['attempts'].tolist()} vs {d['attempts'].tolist()}")
+    assert dt <= 1e-6
 
 
 @pytest.fixture(scope="module")
@@ -180,14 +191,14 @@ def c5_full(post400):
     """The full C5 workload (262,144 electrons, 16,384 groups of 16) traced
     three ways: plain FP64, plain with FP32 pairs (bench.py's plain line),
     and bench.py's value form."""
-    from bem_amd.bem import multipole as mpl
+    from bem_amd.bem import budget
     from bem_amd.emission import cap_source
 
     src = cap_source(262144, post400.R, post400.H, theta_max_deg=60.0, mean_ke_ev=0.2, seed=0)
-    mp = mpl.fit(post400, n_terms=128, tol=1e-8)
+    bud = budget.fit(post400, tol=1e-6)
     runs = {}
     for name, kw in (("fp64", dict(precision="fp64")), ("pairs32", dict(precision="pairs32")),
-                     ("value", dict(precision="pairs32", far_rule=(2.0, 16.0), multipole=mp))):
+                     ("value", dict(precision="pairs32", **bud.tracer_kwargs()))):
         tr = _tracer(post400, src, 1e-8, mode="lockstep", **kw)
         runs[name] = tr.trace()
         o = runs[name]
