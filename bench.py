@@ -54,8 +54,9 @@ C_TERM = 16    # FP64 flops per term of an axis-centred cluster expansion
 TRAFFIC_BYTES = 19.8e6
 TRAFFIC_SOURCE = ("FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB copy), fixed rule at (256, 64), "
                   "profiles/bem_field_gather_post400_262k.md")
-FAST_TRAFFIC_BYTES = None
-FAST_TRAFFIC_SOURCE = None
+FAST_TRAFFIC_BYTES = 13.4e6
+FAST_TRAFFIC_SOURCE = ("FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB copy), fast-form gather at "
+                       "(192, 64), profiles/bem_field_gather_fast_form_post400_262k.md")
 
 
 def workload(n_el=262144, group=16, n_segments=400, seed=0, rtol=1e-8):
@@ -470,6 +471,7 @@ def main():
                            f"{bud.far_rule}, near width {bud.near_width}, {bud.clusters.n_clusters} cluster "
                            "expansions): the same device code as the value trace's ls_field_kernel",
                  "kernel_ms_mean": fast_ms, "points_per_launch": n_pts, "field_shape": list(fast_shape),
+                 "traffic_field_shape": [192, 64],
                  "ring_nodes_per_point": ring, "near_nodes_per_point": near, "expansion_terms_per_point": terms,
                  "flops_per_launch": fast_flops,
                  "note": "FP64 VALU-bound (MI355X FP64 vector peak = FP64 MFMA peak = 78.6 TFLOP/s); algorithmic "
