@@ -266,7 +266,13 @@ def validate(sur, n_val, d_min, d_max, tol, panel, seed, theta_max=0.5 * np.pi):
     cands = cands[np.array([np.any(d >= b) for b in cands])]
     worst = np.array([errE[d >= b].max() for b in cands])
     band = band_for(cands, worst, tol)
-    return dict(tol=tol, band=band, band_panels=band / panel if np.isfinite(band) else None,
+    edges = np.geomspace(d_min, d_max, 9)
+    by_d = []
+    for a, b in zip(edges[:-1], edges[1:]):
+        sel = (d >= a) & (d < b)
+        if sel.any():
+            by_d.append(dict(d_lo=float(a), d_hi=float(b), max=float(errE[sel].max()), median=float(np.median(errE[sel]))))
+    return dict(tol=tol, band=band, band_panels=band / panel if np.isfinite(band) else None, error_by_distance=by_d,
                 field_err_max=float(errE.max()), field_err_median=float(np.median(errE)),
                 field_err_p99=float(np.quantile(errE, 0.99)), phi_err_max=float(errP.max()),
                 phi_err_median=float(np.median(errP)), band_candidates=cands.tolist(), worst_beyond=worst.tolist(),

@@ -289,6 +289,34 @@ def cpu_baseline(wl, seconds=15.0, n_groups=64, workers=None):
                        f"{count} particle-stages")
 
 
+def surrogate_cost(wl, geo, sur, bud, n=262144):
+    """Per-point device time in the network's own domain (3 panel lengths to
+    1 um off the cap): the fast form's gather against the network's FP64
+    inference (HIP events around 5 launches each, after one warm-up)."""
+    import torch
+
+    from bem_amd import surrogate
+
+    ctx = geo._ctx()
+    panel = float(np.max(np.linalg.norm(np.diff(geo.solution.profile, axis=0), axis=1)))
+    rho, z, _ = surrogate.cap_samples(geo.R, geo.H, n, 3 * panel, 1e-6, np.random.default_rng(3), np.deg2rad(80))
+    pts = np.stack([rho, np.zeros_like(rho), z], -1)
+    with geo.far_rule(*bud.far_rule, near_width=bud.near_width, clusters=bud.clusters):
+        fast_ms, _ = _gather_timing(ctx, pts, n, 5, (192, 64))
+    P = torch.as_tensor(pts, device=ctx.torch_device).contiguous()
+    sur.predict(P)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        sur.predict(P)
+    e1.record()
+    torch.cuda.synchronize()
+    net_ms = e0.elapsed_time(e1) / 5
+    return dict(points=n, domain="3 panel lengths .. 1 um off the cap, polar angle <= 80 deg",
+                fast_gather_ns_per_point=fast_ms * 1e6 / n, network_fp64_ns_per_point=net_ms * 1e6 / n)
+
+
 def c2_plain_rate(steps=3):
     """Plain FP64 method on 1024 electrons (64 groups of 16, rtol 1e-9), the
     configuration the parity tests pin against the oracle."""
@@ -439,9 +467,12 @@ def main():
     net_off = dict(value=o3[0]["particle_stages"] / w3, wall_s=w3, attempts_mean=float(o3[0]["attempts"].mean()),
                    screen_t_rel_diff_vs_plain=screen_time_diff(plain_outs[0], o3[0]))
     say(f"bench: fast trace without the network {w3:.3f} s")
+    # what the network gains at the tolerances where validation gives it a
+    # band: over the plain method (network + the reference's fixed rule vs the
+    # plain trace) and over the fast form without it
     sweep = []
     if not args.no_sweep:
-        for tol in (1e-4, 1e-3):
+        for tol in (1e-2, 1e-3, 1e-4, 3e-5, 1e-5):
             s = surrogate.with_tolerance(sur, tol)
             if not np.isfinite(s.cfg["band"]):
                 sweep.append(dict(tol=tol, band_m=None))
@@ -449,11 +480,21 @@ def main():
             t2 = make_tracer(wl, geo, surrogate=s, **bud.tracer_kwargs())
             t2.field_shape = tr.field_shape
             o, w = timed_traces(t2, 1, 0)
-            sweep.append(dict(tol=tol, band_m=s.cfg["band"], value=o[0]["particle_stages"] / w,
-                              wall_s=w, gain_wall_vs_plain=plain_wall / w, network_gain_wall=w3 / w,
-                              attempts_mean=float(o[0]["attempts"].mean()),
-                              screen_t_rel_diff_vs_plain=screen_time_diff(plain_outs[0], o[0])))
-            say(f"bench: tol {tol:g} trace {w:.3f} s")
+            t4 = make_tracer(wl, geo, surrogate=s)
+            t4.field_shape = plain_tr.field_shape
+            o4, w4 = timed_traces(t4, 1, 0)
+            sweep.append(dict(tol=tol, band_m=s.cfg["band"],
+                              with_plain_gather=dict(wall_s=w4, network_gain_wall_vs_plain=plain_wall / w4,
+                                                     screen_t_rel_diff_vs_plain=screen_time_diff(plain_outs[0],
+                                                                                                 o4[0])),
+                              with_fast_form=dict(wall_s=w, value=o[0]["particle_stages"] / w,
+                                                  network_gain_wall=w3 / w,
+                                                  screen_t_rel_diff_vs_plain=screen_time_diff(plain_outs[0], o[0]))))
+            say(f"bench: network at tol {tol:g}: band {s.cfg['band']:.2e} m, with the plain gather {w4:.2f} s "
+                f"(plain {plain_wall:.2f} s), with the fast form {w:.3f} s (without the network {w3:.3f} s)")
+        cost = surrogate_cost(wl, geo, sur, bud)
+        say(f"bench: per point in the network's domain: fast gather {cost['fast_gather_ns_per_point']:.2f} ns, "
+            f"network {cost['network_fp64_ns_per_point']:.2f} ns")
     # dominant kernel of the value trace: the fast form's field gather, at the
     # value trace's launch shape, timed with HIP events on its stream
     pts = gather_points(wl)
@@ -520,6 +561,8 @@ def main():
                       "train_s": rep["t_train_s"], "labels_s": rep["t_labels_s"], "train_steps": rep["steps"],
                       "batch": rep["batch"], "n_samples": rep["n_samples"]},
         "tolerance_sweep": sweep,
+        "network_cost": cost if not args.no_sweep else None,
+        "network_error_by_distance": sur.report.get("error_by_distance"),
         "autotune": {"best": tuning["best"], "table": tuning["table"], "sample_attempts": tuning["attempts"]},
         "autotune_value": {"best": tuning_value["best"], "table": tuning_value["table"], "sample": "whole traces"},
         "attempts_per_group_mean": float(np.mean(outs[-1]["attempts"])),
