@@ -33,6 +33,7 @@ EXPORTS = (
     "spt_mlp_train", "spt_mlp_infer", "spt_surrogate_set", "spt_toroidal_q", "spt_ring_modes",
     "spt_image_upload", "spt_image_force", "spt_set_field_shape", "spt_bem_far_rule",
     "spt_bem_multipole", "spt_trace_noalloc", "spt_bem_clusters", "spt_bem_near_width",
+    "spt_set_lockstep_binning",
 )
 MLP_FOURIER, MLP_WIDTH, MLP_DEPTH = 32, 128, 4
 
@@ -158,6 +159,7 @@ def load():
             "spt_surrogate_set": (ctypes.c_int, [_p, P(MlpCfg_t), _p]),
             "spt_toroidal_q": (ctypes.c_int, [_p, _p, ctypes.c_int64, ctypes.c_int32, _p, _p]),
             "spt_set_field_shape": (ctypes.c_int, [_p, ctypes.c_int32, ctypes.c_int32]),
+            "spt_set_lockstep_binning": (ctypes.c_int, [_p, ctypes.c_int32]),
             "spt_image_upload": (ctypes.c_int, [_p, _p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                 ctypes.c_double, ctypes.c_double, _p]),
             "spt_image_force": (ctypes.c_int, [_p, P(Geometry_t), _p, _p, _p, _p, ctypes.c_int32, ctypes.c_double,

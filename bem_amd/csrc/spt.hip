@@ -111,6 +111,7 @@ struct BemView {
 };
 constexpr int kCoarsePer = 14;
 constexpr int kMultipoleMaxTerms = 1024;
+constexpr double kMultipoleTail = 1e-12;  // multipole_pert's truncation bound (relative to max |m_l|)
 constexpr int kMaxClusters = 64;
 constexpr int kClusterMaxTerms = 256;
 constexpr double kRefNearWidth = 0.5;
@@ -322,29 +323,38 @@ __device__ __forceinline__ double4 load_node(const double4* nd, int q) { return 
 // quadrature node, so E_r = (1/4 pi r^2) sum m_l (l+1) q^l P_l and
 // E_theta = (sin theta / 4 pi r^2) sum m_l q^l P_l' (q = r_ref / r), with
 // P_{l+1} = a_l c P_l - b_l P_{l-1} and P'_{l+1} = P'_{l-1} + (2l+1) P_l.
+// Terms stop after the first l whose tail bound sum_{k > l} (k+1) q^k is
+// <= kMultipoleTail (|P_k| <= 1 and |sin t P'_k| <= k + 1, so the dropped
+// terms of either component are below kMultipoleTail max_k |m_k| / 4 pi r^2).
 // Returns false (nothing written) where r^2 < mp_rmin2.
+__device__ __forceinline__ bool multipole_zone(const BemView& B, double x, double y, double z) {
+  const double rho2 = x * x + y * y;
+  return rho2 + z * z >= B.mp_rmin2;
+}
 __device__ __forceinline__ bool multipole_pert(const BemView& B, double x, double y, double z, double& ox, double& oy,
                                                double& oz) {
   const double rho2 = x * x + y * y;
   const double r2 = rho2 + z * z;
-  if (!(r2 >= B.mp_rmin2)) return false;
+  if (!(r2 >= B.mp_rmin2)) return false;  // multipole_zone, the same arithmetic
   const double r = sqrt(r2), ir = 1.0 / r, rho = sqrt(rho2);
   const double c = z * ir, sn = rho * ir, q = B.mp_rref * ir;
+  const double omq = 1.0 - q;
+  const double tolw = kMultipoleTail * omq * omq;
   double pl = 1.0, plm = 0.0, dl = 0.0, dlm = 0.0, qp = 1.0, sa = 0.0, sb = 0.0;
   const cseg_ptr t = (cseg_ptr)B.mp;
-#pragma unroll 2
   for (int l = 0; l < B.mp_n; ++l) {
     const double m = t[4 * l], a = t[4 * l + 1], b = t[4 * l + 2];
     const double mq = m * qp;
     sa = fma(mq * (double)(l + 1), pl, sa);
     sb = fma(mq, dl, sb);
+    qp *= q;
+    if (qp * fma(-(double)(l + 1), q, (double)(l + 2)) <= tolw) break;
     const double pn = fma(a * c, pl, -b * plm);
     const double dn = fma((double)(2 * l + 1), pl, dlm);
     plm = pl;
     pl = pn;
     dlm = dl;
     dl = dn;
-    qp *= q;
   }
   const double k = 1.0 / (4.0 * kPi * r2);
   const double er = k * sa, et = k * sn * sb;
@@ -463,6 +473,37 @@ __device__ __forceinline__ void cluster_expansion(const BemView& B, int c, doubl
   acc.ez_ax += er * cs - et * sn;
 }
 
+// Squared distance from (rho, z) to segment g, the projection clamped to the
+// segment (reference bem/axisymmetric.py:341-389).
+__device__ __forceinline__ double seg_dist2(const double* g, double rho, double z) {
+  const double ax = g[S_ABR], ay = g[S_ABZ];
+  double t = ((rho - g[S_P0R]) * ax + (z - g[S_P0Z]) * ay) / g[S_ABAB];
+  t = fmin(fmax(t, 0.0), 1.0);
+  const double dr = rho - (g[S_P0R] + t * ax);
+  const double dz = z - (g[S_P0Z] + t * ay);
+  return dr * dr + dz * dz;
+}
+
+// Bin of a point for the binned lockstep gather: its distance band (d/L in
+// [2^(b + kBinBandLo), 2^(b + kBinBandLo + 1)), open-ended at both ends)
+// major, its closest segment (>> shift) minor, so consecutive bins are
+// neighbouring segments at similar distances.
+constexpr int kBinBands = 20;
+constexpr int kBinBandLo = -12;
+__device__ __forceinline__ int bin_key(int bseg, double bd2, double L, int nsk, int shift) {
+  const double r = bd2 / (L * L);
+  int band = 0;
+  if (!(r < 1e300)) {
+    band = kBinBands - 1;
+  } else if (r > 0.0) {
+    int e;
+    (void)frexp(r, &e);              // log2 r in [e - 1, e)
+    band = ((e - 1) >> 1) - kBinBandLo;  // floor(log2 sqrt r) - kBinBandLo
+    band = band < 0 ? 0 : (band >= kBinBands ? kBinBands - 1 : band);
+  }
+  return band * nsk + (bseg >> shift);
+}
+
 // Perturbation field -grad S[sigma] at m <= NMAX points held in LDS, by all
 // T threads of the block: S threads ("slices") per point, slice s taking
 // segments s, s+S, ... Two lane layouts: slice-major (S = T/m, point p's
@@ -476,19 +517,25 @@ __device__ __forceinline__ void cluster_expansion(const BemView& B, int c, doubl
 // shared out over the point's slices afterwards. Slice partials are
 // combined by a fixed-pairing tree, so results are deterministic (they
 // depend on S, i.e. on the launch shape, in the last bits). m must be
-// block-uniform.
-template <int T, int NMAX, int PR, bool FAR>
+// block-uniform. Far-rule gathers pick each point's rule per lane, so a
+// point's result does not depend on the other points of its block. BIN:
+// the binned lockstep gather's form, always wave-major (lane = point, so
+// the result is independent of m too) with the points valid[p] != 0 of
+// 0..63, and each point's bin (bin_key) written to okey[p].
+template <int T, int NMAX, int PR, bool FAR, bool BIN = false>
 __device__ void bem_pert_block(const BemView& B, int m, const double* px, const double* py, const double* pz,
-                               double* ox, double* oy, double* oz, BemShared<T, NMAX>& sh) {
+                               double* ox, double* oy, double* oz, BemShared<T, NMAX>& sh, const int* valid = nullptr,
+                               int* okey = nullptr, int key_nsk = 0, int key_shift = 0) {
   static_assert(T % 64 == 0, "whole wavefronts");
+  static_assert(!BIN || (FAR && NMAX == 64), "the binned gather is a far-rule gather of 64 points");
   if (m <= 0) return;
   fill_log_table<T>(sh.logt);  // read after the barrier that ends the closest-segment pass
   const int tid = threadIdx.x;
-  const bool wave_major = m > 32 && m <= 64 && B.n_seg >= 4 * (T / 64);
+  const bool wave_major = BIN || (m > 32 && m <= 64 && B.n_seg >= 4 * (T / 64));
   const int S = wave_major ? T / 64 : (m >= T ? 1 : T / m);
   const int p = wave_major ? (tid & 63) : tid / S;
   const int s = wave_major ? (tid >> 6) : tid - p * S;
-  const bool on = p < m;
+  const bool on = p < m && (!BIN || valid[p] != 0);
   // thread holding slice k of this thread's point: slot0 + k * sstep
   const int slot0 = wave_major ? p : p * S, sstep = wave_major ? 64 : 1;
   auto slot = [=](int k) { return slot0 + k * sstep; };
@@ -499,13 +546,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
     double bd2 = INFINITY;
     int bseg = 0;
     auto closest = [&](int j) {
-      const double* g = B.segs + (size_t)j * SEG_STRIDE;
-      const double ax = g[S_ABR], ay = g[S_ABZ];
-      double t = ((rho - g[S_P0R]) * ax + (z - g[S_P0Z]) * ay) / g[S_ABAB];
-      t = fmin(fmax(t, 0.0), 1.0);
-      const double dr = rho - (g[S_P0R] + t * ax);
-      const double dz = z - (g[S_P0Z] + t * ay);
-      const double d2 = dr * dr + dz * dz;
+      const double d2 = seg_dist2(B.segs + (size_t)j * SEG_STRIDE, rho, z);
       if (d2 < bd2) {
         bd2 = d2;
         bseg = j;
@@ -546,6 +587,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
     const double L = g[S_L];
     const double thr = 1e-9 * L;
     const double off = bd2 < thr * thr ? 1e-4 * L : 0.0;
+    if (BIN) okey[p] = bin_key(bseg, bd2, L, key_nsk, key_shift);
     double re = rho + off * g[S_NR];
     if (rho == 0.0) re = 0.0;
     sh.re[p] = re;
@@ -575,9 +617,10 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
         double proj, dist;
         segment_proj(re, ze, g, proj, dist);
         if (FAR) {
-          // Far rule: one rule per wavefront, the finest any of its lanes
-          // needs (a finer rule than a lane's own only adds accuracy), so
-          // the node loop is a uniform branch with scalar table loads.
+          // Far rule, picked per lane: lanes needing different rules run
+          // them one after the other (each a branch with scalar table
+          // loads), so a point's sum does not depend on its neighbours.
+          // The near-axis test is per wavefront: it changes no result.
           const bool near = dist < L;
           if (near) {
             if (n_near < kNearSlots)
@@ -587,12 +630,15 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
                                    dist, sh.logt, nw));
           }
           const int rl = near ? 3 : (dist >= B.k2 * L ? 2 : (dist >= B.k4 * L ? 1 : 0));
-          const unsigned long long b0 = __ballot(rl == 0), b1 = __ballot(rl == 1), b2 = __ballot(rl == 2);
           const bool axw =
               __ballot(!near && ring_axis_possible(re2, ze, g[S_P0R], g[S_P0Z], g[S_ABR], g[S_ABZ])) != 0ull;
-          if ((b0 | b1 | b2) && !near)
-            coarse_segment(re, re2, ze, (cnode_ptr)B.coarse + (size_t)j * kCoarsePer, b0 ? 0 : (b1 ? 1 : 2), axw,
-                           acc, sh.logt);
+          const cnode_ptr cn = (cnode_ptr)B.coarse + (size_t)j * kCoarsePer;
+          if (rl == 0)
+            coarse_segment(re, re2, ze, cn, 0, axw, acc, sh.logt);
+          else if (rl == 1)
+            coarse_segment(re, re2, ze, cn, 1, axw, acc, sh.logt);
+          else if (rl == 2)
+            coarse_segment(re, re2, ze, cn, 2, axw, acc, sh.logt);
         } else if (dist < L) {
           if (n_near < kNearSlots)
             sh.u.n.seg[n_near++][tid] = j;
@@ -1489,6 +1535,10 @@ __global__ __launch_bounds__(T) void trace_kernel(TraceParams P) {
 __global__ void reset_int_kernel(int* p, int v) { *p = v; }
 __global__ void reset_u64_kernel(unsigned long long* p) { *p = 0ULL; }
 __global__ void reset_u32_kernel(unsigned int* p) { *p = 0u; }
+__global__ void fill_u32_kernel(unsigned* p, int n, unsigned v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
 
 // ----------------------------------------------- standalone field kernels
 template <int T, int NMAX>
@@ -1634,6 +1684,36 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void geometry_field_kernel(
 constexpr int kSlotsLockstep = 60;
 constexpr int LS_RUNNING = 1, LS_EXHAUSTED = 2;
 
+// Binned gather of far-rule lockstep traces. At the start of every attempt
+// the particles whose state needs the BEM perturbation are counted into
+// bins (bin_key of their last gather: distance band, closest segment; in
+// the multipole zone, bins of log r) and given consecutive slots bin by
+// bin; at every stage the kernel that computes the stage position writes it
+// into the particle's slot record, the gather kernel reads 64 consecutive
+// records per block (one bin or neighbouring bins: the lanes of a wavefront
+// need the same segments, rules, expansions or multipole terms) and writes
+// the perturbation and the new bin into the same slot, and the stage's
+// acceleration kernel reads it back. Slot order within a bin comes from
+// atomics; a far-rule gather's result does not depend on the point's
+// block-mates, so traces stay bit-reproducible.
+constexpr int kBinMaxKeys = 8192;
+constexpr int kBinThreads = 1024;
+struct BinView {
+  int32_t* key;      // per particle: bin of its last gather (nullptr: binning off)
+  int32_t* slot;     // per particle: its slot this attempt, -1 none (-2: eligible, between the count and slot kernels)
+  uint8_t* src;      // per particle, this stage: 1 = the field comes from its slot's output record
+  unsigned* hist;    // n_keys: eligible particles per bin (zero between attempts)
+  unsigned* cursor;  // n_keys: next free slot per bin
+  unsigned* total;   // slots in use this attempt (grows by particles that need a gather without a slot)
+  double4* rin;      // per slot: stage position, kind (bits: particle index >= 0 gather, -2 - index
+                     // multipole zone, -1 nothing)
+  double4* rout;     // per slot: perturbation, bin (bits)
+  int n_keys, nsk, shift;
+  int key_mp0;       // first multipole-zone bin (n_keys when the zone is off)
+  int append_only;   // 1: no slots at the start of an attempt, every record through the append path
+};
+constexpr int kBinMpKeys = 64;  // multipole-zone bins: 16 per octave of r^2 beyond r_min, open-ended
+
 struct LockParams {
   TraceParams T;
   const int32_t* pgroup;   // particle -> group
@@ -1651,6 +1731,7 @@ struct LockParams {
   Mlp64View V64;           // and FP64 forms
   const double* imgF;      // image-charge BEM force per particle (n x 3), or nullptr
   const int32_t* order;    // ls_field_kernel's particle order (a permutation of 0..n-1), or nullptr
+  BinView X;               // the binned gather (X.key == nullptr: ls_field_kernel)
 };
 
 // Particle taken by position k of ls_field_kernel's sweep.
@@ -1747,6 +1828,192 @@ __device__ __forceinline__ void ls_keep_field(const LockParams& L, int64_t gi, i
   } else if (st == 6) {
     for (int c = 0; c < 3; ++c) LKB(57 + c) = LKB(51 + c);
   }
+}
+
+// ---- binned gather kernels
+
+// Initial bins: the closest segment over all segments at the stage-0
+// position (first index wins) and the distance band.
+__global__ __launch_bounds__(256) void ls_bin_key_kernel(LockParams L) {
+  const int64_t gi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (gi >= L.T.n_total) return;
+  const BemView& B = L.T.bem;
+  const double x = LKB(48), y = LKB(49), z = LKB(50);
+  const double rho = hypot(x, y);
+  double bd2 = INFINITY;
+  int bseg = 0;
+  for (int j = 0; j < B.n_seg; ++j) {
+    const double d2 = seg_dist2(B.segs + (size_t)j * SEG_STRIDE, rho, z);
+    if (d2 < bd2) {
+      bd2 = d2;
+      bseg = j;
+    }
+  }
+  L.X.key[gi] = bin_key(bseg, bd2, B.segs[(size_t)bseg * SEG_STRIDE + S_L], L.X.nsk, L.X.shift);
+}
+
+// Does the particle's state (the stage-0 position) take a slot this attempt?
+__device__ __forceinline__ bool bin_eligible(const LockParams& L, int64_t gi, double x, double y, double z) {
+  if (!(L.g_flags[L.pgroup[gi]] & LS_RUNNING) || !L.active[gi]) return false;
+  return bem_needed(L.T.G, x, y, z);
+}
+
+// Bin of a point in the multipole zone (r^2 >= mp_rmin2).
+__device__ __forceinline__ int bin_key_mp(const BinView& X, const BemView& B, double r2) {
+  const double u = 16.0 * log2(r2 / B.mp_rmin2);
+  const int k = u >= (double)(kBinMpKeys - 1) ? kBinMpKeys - 1 : (u > 0.0 ? (int)u : 0);
+  return X.key_mp0 + k;
+}
+
+// Eligible particles per bin (block histogram in LDS, then one global atomic
+// per bin and block).
+__global__ __launch_bounds__(kBinThreads) void ls_bin_count_kernel(LockParams L) {
+  __shared__ unsigned cnt[kBinMaxKeys];
+  const BinView& X = L.X;
+  for (int k = threadIdx.x; k < X.n_keys; k += kBinThreads) cnt[k] = 0u;
+  __syncthreads();
+  const int64_t gi = (int64_t)blockIdx.x * kBinThreads + threadIdx.x;
+  if (gi < L.T.n_total) {
+    const bool e = !X.append_only && bin_eligible(L, gi, LKB(48), LKB(49), LKB(50));
+    X.slot[gi] = e ? -2 : -1;
+    if (e) atomicAdd(&cnt[X.key[gi]], 1u);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < X.n_keys; k += kBinThreads)
+    if (cnt[k]) atomicAdd(&X.hist[k], cnt[k]);
+}
+
+// First slot of every bin (exclusive prefix sum of the counts, one block),
+// the slot total; clears the counts for the next attempt.
+__global__ __launch_bounds__(kBinThreads) void ls_bin_scan_kernel(LockParams L) {
+  __shared__ unsigned part[kBinThreads];
+  const BinView& X = L.X;
+  const int tid = threadIdx.x;
+  const int per = (X.n_keys + kBinThreads - 1) / kBinThreads;
+  const int k0 = tid * per, k1 = min(k0 + per, X.n_keys);
+  unsigned own = 0u;
+  for (int k = k0; k < k1; ++k) own += X.hist[k];
+  part[tid] = own;
+  __syncthreads();
+  for (int o = 1; o < kBinThreads; o <<= 1) {
+    const unsigned v = tid >= o ? part[tid - o] : 0u;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  unsigned run = part[tid] - own;
+  for (int k = k0; k < k1; ++k) {
+    const unsigned h = X.hist[k];
+    X.cursor[k] = run;
+    run += h;
+    X.hist[k] = 0u;
+  }
+  if (tid == kBinThreads - 1) *X.total = part[tid];
+}
+
+// Stage st's field for a particle of a running group whose stage position
+// (x, y, z) is in slots 48..50: written here where the BEM perturbation is
+// not needed (inactive, the state's known field at stage 0, outside the BEM
+// region), else the position goes into the particle's slot record (a slot
+// at the end of the list for a particle without one), marked for the
+// gather or for the multipole zone. Every slot of the attempt gets its
+// record at every stage.
+__device__ void ls_bin_pack(const LockParams& L, int64_t gi, int st, double x, double y, double z) {
+  const BinView& X = L.X;
+  int sl = X.slot[gi];
+  int kind = 0;  // 1 gather, 2 multipole zone
+  if (L.active[gi]) {
+    if (st == 0 && L.fvalid[gi]) {  // the field at the state is known
+      for (int c = 0; c < 3; ++c) LKB(51 + c) = LKB(54 + c);
+    } else if (bem_needed(L.T.G, x, y, z)) {
+      kind = (L.T.bem.mp && multipole_zone(L.T.bem, x, y, z)) ? 2 : 1;
+    } else {
+      double ex, ey, ez;
+      geometry_field_point(L.T.G, x, y, z, 0.0, 0.0, 0.0, ex, ey, ez);
+      LKB(51) = ex;
+      LKB(52) = ey;
+      LKB(53) = ez;
+      ls_keep_field(L, gi, st);
+    }
+  }
+  if (kind) {
+    if (sl < 0) {
+      sl = (int)atomicAdd(X.total, 1u);
+      X.slot[gi] = sl;
+    }
+    X.rin[sl] = make_double4(x, y, z, __longlong_as_double(kind == 1 ? (long long)gi : -2ll - (long long)gi));
+  } else if (sl >= 0) {
+    X.rin[sl].w = __longlong_as_double(-1ll);
+  }
+  X.src[gi] = kind ? 1 : 0;
+}
+
+// Slots for the eligible particles, bin by bin (LDS rank within the block,
+// one global atomic per bin and block), then stage 0's field or record.
+__global__ __launch_bounds__(kBinThreads) void ls_bin_slot_kernel(LockParams L) {
+  __shared__ unsigned cnt[kBinMaxKeys];
+  const BinView& X = L.X;
+  for (int k = threadIdx.x; k < X.n_keys; k += kBinThreads) cnt[k] = 0u;
+  __syncthreads();
+  const int64_t gi = (int64_t)blockIdx.x * kBinThreads + threadIdx.x;
+  const bool mine = gi < L.T.n_total;
+  const bool e = mine && X.slot[gi] == -2;
+  const int k = e ? X.key[gi] : 0;
+  const unsigned r = e ? atomicAdd(&cnt[k], 1u) : 0u;
+  __syncthreads();
+  for (int kk = threadIdx.x; kk < X.n_keys; kk += kBinThreads)
+    if (cnt[kk]) cnt[kk] = atomicAdd(&X.cursor[kk], cnt[kk]);
+  __syncthreads();
+  if (!mine) return;
+  if (e) X.slot[gi] = (int)(cnt[k] + r);
+  if (L.g_flags[L.pgroup[gi]] & LS_RUNNING) ls_bin_pack(L, gi, 0, LKB(48), LKB(49), LKB(50));
+}
+
+// The multipole-zone records of this stage, one per thread (they sit
+// together at the end of the slot list, after the distance-band bins).
+__global__ __launch_bounds__(256) void ls_bin_mp_kernel(LockParams L) {
+  const BinView& X = L.X;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (int64_t)*X.total) return;
+  const double4 r = X.rin[i];
+  double qx, qy, qz;
+  if (__double_as_longlong(r.w) <= -2 && multipole_pert(L.T.bem, r.x, r.y, r.z, qx, qy, qz))
+    X.rout[i] = make_double4(
+        qx, qy, qz, __longlong_as_double((long long)bin_key_mp(X, L.T.bem, fma(r.x, r.x, fma(r.y, r.y, r.z * r.z)))));
+}
+
+// This stage's gather records: 64 consecutive slots per block, lane = slot
+// (multipole-zone records are ls_bin_mp_kernel's, but a record that the
+// zone test rejects falls back to the gather).
+template <int T>
+__global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_bin_field_kernel(LockParams L) {
+  __shared__ FieldShared<T, 64> sh;
+  __shared__ int okey[64];
+  const BinView& X = L.X;
+  const int64_t base = (int64_t)blockIdx.x * 64;
+  const int64_t tot = (int64_t)*X.total;
+  if (base >= tot) return;
+  const int m = (int)min((int64_t)64, tot - base);
+  const int tid = threadIdx.x;
+  if (tid < 64) {
+    int v = 0;
+    if (tid < m) {
+      const double4 r = X.rin[base + tid];
+      const long long w = __double_as_longlong(r.w);
+      if (w >= 0 || (w <= -2 && !multipole_zone(L.T.bem, r.x, r.y, r.z))) {
+        v = 1;
+        sh.px[tid] = r.x;
+        sh.py[tid] = r.y;
+        sh.pz[tid] = r.z;
+      }
+    }
+    sh.need[tid] = v;
+  }
+  if (!__syncthreads_or(tid < 64 && sh.need[tid])) return;
+  bem_pert_block<T, 64, SPT_PREC_FP64, true, true>(L.T.bem, 64, sh.px, sh.py, sh.pz, sh.ox, sh.oy, sh.oz, sh.bem,
+                                                   sh.need, okey, X.nsk, X.shift);
+  if (tid < m && sh.need[tid])
+    X.rout[base + tid] = make_double4(sh.ox[tid], sh.oy[tid], sh.oz[tid], __longlong_as_double((long long)okey[tid]));
 }
 
 // External field at the stage positions of every active particle of every
@@ -1934,6 +2201,16 @@ __global__ __launch_bounds__(TG) void ls_accel_kernel(LockParams L, int st) {
     sz[tid] = LKB(50);
     sq[tid] = L.T.charge[gi];
     act[tid] = L.active[gi];
+    if (L.X.key && L.X.src[gi]) {  // the binned gather's output for this stage
+      const double4 r = L.X.rout[L.X.slot[gi]];
+      double ex, ey, ez;
+      geometry_field_point(L.T.G, sx[tid], sy[tid], sz[tid], r.x, r.y, r.z, ex, ey, ez);
+      LKB(51) = ex;
+      LKB(52) = ey;
+      LKB(53) = ez;
+      ls_keep_field(L, gi, st);
+      L.X.key[gi] = (int)__double_as_longlong(r.w);
+    }
   }
   __syncthreads();
   if (!mine) return;
@@ -1961,6 +2238,7 @@ __global__ __launch_bounds__(TG) void ls_accel_kernel(LockParams L, int st) {
       LKB(ns * 6 + c) = active ? vs : 0.0;
       LKB(48 + c) = ps;
     }
+    if (L.X.key) ls_bin_pack(L, gi, ns, LKB(48), LKB(49), LKB(50));
   } else if (st == 5) {
     for (int c = 0; c < 3; ++c) {
       double sp = c_B[0] * LKB(c), sv = c_B[0] * LKB(3 + c);
@@ -1975,6 +2253,7 @@ __global__ __launch_bounds__(TG) void ls_accel_kernel(LockParams L, int st) {
       LKB(36 + c) = active ? vn : 0.0;
       LKB(48 + c) = pn;
     }
+    if (L.X.key) ls_bin_pack(L, gi, 6, LKB(48), LKB(49), LKB(50));
   }
 }
 
@@ -2297,6 +2576,14 @@ struct spt_ctx {
   unsigned int* h_done = nullptr;
   uint8_t* d_sur = nullptr;
   uint8_t* d_fvalid = nullptr;
+  // binned gather of far-rule lockstep traces (spt_set_lockstep_binning)
+  int bin_mode = 1;  // 0 off, 1 on, 2 append path only (a test of slot-order independence)
+  unsigned* d_bin_ctl = nullptr;  // hist, cursor (kBinMaxKeys each), total
+  int32_t* d_bin_key = nullptr;
+  int32_t* d_bin_slot = nullptr;
+  uint8_t* d_bin_src = nullptr;
+  double4* d_bin_rec = nullptr;   // rin then rout (n each)
+  size_t bin_cap = 0;
   // surrogate used by spt_trace
   bool have_sur = false;
   spt_mlp_cfg sur_cfg;
@@ -2440,6 +2727,11 @@ void spt_destroy(spt_ctx* c) {
   if (c->h_done) (void)hipHostFree(c->h_done);
   if (c->d_sur) (void)hipFree(c->d_sur);
   if (c->d_fvalid) (void)hipFree(c->d_fvalid);
+  if (c->d_bin_ctl) (void)hipFree(c->d_bin_ctl);
+  if (c->d_bin_key) (void)hipFree(c->d_bin_key);
+  if (c->d_bin_slot) (void)hipFree(c->d_bin_slot);
+  if (c->d_bin_src) (void)hipFree(c->d_bin_src);
+  if (c->d_bin_rec) (void)hipFree(c->d_bin_rec);
   if (c->d_sur_p) (void)hipFree(c->d_sur_p);
   if (c->d_sur_split) (void)hipFree(c->d_sur_split);
   if (c->d_sur_frag) (void)hipFree(c->d_sur_frag);
@@ -2480,6 +2772,13 @@ int spt_set_field_shape(spt_ctx* c, int32_t threads, int32_t points_per_block) {
     return fail(c, SPT_EINVAL, "points_per_block must be 0 or a power of two <= 64");
   c->tune_threads = threads;
   c->tune_pb = points_per_block;
+  return SPT_OK;
+}
+
+int spt_set_lockstep_binning(spt_ctx* c, int32_t on) {
+  if (!c) return SPT_EINVAL;
+  if (on < 0 || on > 2) return fail(c, SPT_EINVAL, "spt_set_lockstep_binning: on must be 0, 1 or 2");
+  c->bin_mode = on;
   return SPT_OK;
 }
 
@@ -2970,23 +3269,43 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
   reset_u64_kernel<<<1, 1, 0, s>>>(L.T.ev_count);
   if (L.T.n_out > 0) reset_u64_kernel<<<1, 1, 0, s>>>(L.T.snap_count);
   ls_init_kernel<TG><<<n_groups, TG, 0, s>>>(L);
+  const bool bin = L.X.key != nullptr;
+  const unsigned nb_bin = (unsigned)((n + kBinThreads - 1) / kBinThreads);
+  const unsigned nb_slots = (unsigned)((n + 63) / 64);
+  if (bin) {
+    fill_u32_kernel<<<(L.X.n_keys + 255) / 256, 256, 0, s>>>(L.X.hist, L.X.n_keys, 0u);
+    ls_bin_key_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(L);
+  }
   HIPCHK(c, hipGetLastError());
   // kLockstepUnroll attempts of every running group. Captured once in a HIP
   // graph and replayed; with the image-charge BEM (whose solve is a rocBLAS
   // GEMM, kept out of graphs) the same sequence is issued eagerly.
   auto enqueue = [&]() -> int {
     for (int u = 0; u < kLockstepUnroll; ++u) {
+      if (bin) {
+        ls_bin_count_kernel<<<nb_bin, kBinThreads, 0, s>>>(L);
+        ls_bin_scan_kernel<<<1, kBinThreads, 0, s>>>(L);
+        ls_bin_slot_kernel<<<nb_bin, kBinThreads, 0, s>>>(L);
+      }
       for (int st = 0; st < 7; ++st) {
-        if (L.sur) {
+        if (bin) {
+          if (L.T.bem.mp) ls_bin_mp_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(L);
+          if (fs.T == 192)
+            ls_bin_field_kernel<192><<<nb_slots, 192, 0, s>>>(L);
+          else
+            ls_bin_field_kernel<256><<<nb_slots, 256, 0, s>>>(L);
+        } else if (L.sur) {
           if (FPR == SPT_PREC_FP64)
             ls_surrogate_kernel<true><<<(unsigned)((n + MLP64_PTS - 1) / MLP64_PTS), 64, 0, s>>>(L, st);
           else
             ls_surrogate_kernel<false><<<(unsigned)((n + MLP_WAVE_PTS - 1) / MLP_WAVE_PTS), 64, 0, s>>>(L, st);
         }
-        if (fs.T == 192)
+        if (bin) {
+        } else if (fs.T == 192) {
           ls_field_kernel<192, 64, FPR><<<nb_field, 192, 0, s>>>(L, PB, st);
-        else
+        } else {
           ls_field_kernel<256, 64, FPR><<<nb_field, 256, 0, s>>>(L, PB, st);
+        }
         if (L.imgF) {
           const int r = launch_image_force(c, L.T.G, stage_pos_view(L), L.T.charge, L.active, L.T.offsets,
                                            L.g_flags, LS_RUNNING, n_groups, L.T.a2, (double*)L.imgF, s);
@@ -3035,6 +3354,29 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
   return rc;
+}
+
+// Buffers of the binned gather for n particles (BinView).
+static int ensure_bin_scratch(spt_ctx* c, int64_t n) {
+  if (!c->d_bin_ctl) {
+    HIPCHK(c, hipMalloc(&c->d_bin_ctl, sizeof(unsigned) * (2 * kBinMaxKeys + 64)));
+  }
+  if ((size_t)n > c->bin_cap) {
+    if (c->d_bin_key) (void)hipFree(c->d_bin_key);
+    if (c->d_bin_slot) (void)hipFree(c->d_bin_slot);
+    if (c->d_bin_src) (void)hipFree(c->d_bin_src);
+    if (c->d_bin_rec) (void)hipFree(c->d_bin_rec);
+    c->d_bin_key = c->d_bin_slot = nullptr;
+    c->d_bin_src = nullptr;
+    c->d_bin_rec = nullptr;
+    c->bin_cap = 0;
+    HIPCHK(c, hipMalloc(&c->d_bin_key, sizeof(int32_t) * n));
+    HIPCHK(c, hipMalloc(&c->d_bin_slot, sizeof(int32_t) * n));
+    HIPCHK(c, hipMalloc(&c->d_bin_src, n));
+    HIPCHK(c, hipMalloc(&c->d_bin_rec, sizeof(double4) * 2 * n));
+    c->bin_cap = n;
+  }
+  return SPT_OK;
 }
 
 static int ensure_lockstep_scratch(spt_ctx* c, int64_t n, int n_groups) {
@@ -3213,6 +3555,28 @@ static int trace_impl(spt_ctx* c, const spt_trace_args* a, bool no_alloc) {
     L.sur = a->use_surrogate ? c->d_sur : nullptr;
     L.fvalid = c->d_fvalid;
     L.order = a->field_order;
+    memset(&L.X, 0, sizeof(L.X));
+    const bool bin = c->bin_mode != 0 && (a->geom.kind == SPT_GEOM_BEM_POST || a->geom.kind == SPT_GEOM_BEM_WELL) &&
+                     P.bem.coarse && !a->use_surrogate && a->precision != SPT_PREC_MIXED && P.bem.n_seg >= 1;
+    if (bin) {
+      int rb = ensure_bin_scratch(c, a->n_particles);
+      if (rb) return rb;
+      int shift = 0;
+      while (kBinBands * ((P.bem.n_seg + (1 << shift) - 1) >> shift) + kBinMpKeys > kBinMaxKeys) ++shift;
+      L.X.nsk = (P.bem.n_seg + (1 << shift) - 1) >> shift;
+      L.X.shift = shift;
+      L.X.key_mp0 = kBinBands * L.X.nsk;
+      L.X.n_keys = L.X.key_mp0 + (P.bem.mp ? kBinMpKeys : 0);
+      L.X.key = c->d_bin_key;
+      L.X.slot = c->d_bin_slot;
+      L.X.src = c->d_bin_src;
+      L.X.hist = c->d_bin_ctl;
+      L.X.cursor = c->d_bin_ctl + kBinMaxKeys;
+      L.X.total = c->d_bin_ctl + 2 * kBinMaxKeys;
+      L.X.rin = c->d_bin_rec;
+      L.X.rout = c->d_bin_rec + a->n_particles;
+      L.X.append_only = c->bin_mode == 2;
+    }
     memset(&L.V, 0, sizeof(L.V));
     memset(&L.V64, 0, sizeof(L.V64));
     L.imgF = nullptr;

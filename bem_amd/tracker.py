@@ -51,7 +51,7 @@ class SpecificParticleTracer:
                  plummer_radius=DEFAULT_PLUMMER_RADIUS, backend="mi355x", n_workers=1, rtol=None, atol=None,
                  t_max=None, z_max=None, group_sizes=None, block_threads=0, device=0, max_steps=10_000_000,
                  mode="auto", precision="fp64", surrogate=None, far_rule=None, multipole=None, near_width=None,
-                 clusters=None, spatial_order=False):
+                 clusters=None, spatial_order=False, binned=True):
         if n_workers != 1:
             raise ValueError(f"n_workers is only supported for backend='cpu' (got backend={backend!r}, "
                              f"n_workers={n_workers!r})")
@@ -114,6 +114,11 @@ class SpecificParticleTracer:
         self.multipole = multipole
         self.field_shape = None  # (threads, points per block) of the field gather; None = automatic
         self.spatial_order = bool(spatial_order)
+        # far-rule lockstep traces gather the field bin by bin (spt_set_lockstep_binning:
+        # True/1, False/0, or 2 = every gather through the append path)
+        self.binned = int(binned)
+        if self.binned not in (0, 1, 2):
+            raise ValueError(f"binned must be True, False or 2, got {binned!r}")
         self._field_order = None
         self.max_steps = int(max_steps)
         self._setup_state(group_sizes)
@@ -303,6 +308,7 @@ class SpecificParticleTracer:
         ctx.bind_stream()
         shape = getattr(self, "field_shape", None) or (0, 0)
         ctx.check(ctx.lib.spt_set_field_shape(ctx.h, int(shape[0]), int(shape[1])), "spt_set_field_shape")
+        ctx.check(ctx.lib.spt_set_lockstep_binning(ctx.h, self.binned), "spt_set_lockstep_binning")
         k4, k2 = self.far_rule or (0.0, 0.0)
         ctx.check(ctx.lib.spt_bem_far_rule(ctx.h, k4, k2), "spt_bem_far_rule")
         if self.near_width is not None:
@@ -321,6 +327,7 @@ class SpecificParticleTracer:
                 ctx.check(ctx.lib.spt_trace(ctx.h, a), "spt_trace")
         finally:
             ctx.lib.spt_set_field_shape(ctx.h, 0, 0)
+            ctx.lib.spt_set_lockstep_binning(ctx.h, 1)
             ctx.lib.spt_bem_far_rule(ctx.h, 0.0, 0.0)
             ctx.lib.spt_bem_near_width(ctx.h, 0.0)
             ctx.lib.spt_bem_clusters(ctx.h, None, 0, None, None, None, 0, 0.0, 0.0)

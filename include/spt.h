@@ -351,6 +351,19 @@ int spt_image_force(spt_ctx* ctx, const spt_geometry* geom, const double* pos, c
  * rule (about three resident blocks per compute unit, whole rounds). */
 int spt_set_field_shape(spt_ctx* ctx, int32_t threads, int32_t points_per_block);
 
+/* Binned gather of far-rule lockstep traces (no counterpart in the
+ * reference, whose field evaluation takes the particles in index order,
+ * bem/axisymmetric.py:501-605): 1 (the default) bins the particles that
+ * need the gather by distance band and closest segment at the start of
+ * every attempt and gathers them bin by bin from packed slot records; 0
+ * gathers in particle order (field_order applies only then); 2 gives no
+ * slots at attempt starts, so every gather takes the path of a particle
+ * that needs one without a slot (slots in atomic order: a test that results
+ * do not depend on slot order). Used when the far rule is set and the trace
+ * runs neither the surrogate nor MIXED precision. Results do not depend on
+ * it beyond the last bits (0 against 1 or 2); 1 and 2 agree bit for bit. */
+int spt_set_lockstep_binning(spt_ctx* ctx, int32_t on);
+
 /* toroidal_Q (reference bem/toroidal.py:188-219): Q_{m-1/2}(chi) and
  * dQ/dchi for m = 0..n_max (n_max <= 64) at n device values chi >= 1 into
  * device q, dq, each (n_max + 1) x n, mode-major. */

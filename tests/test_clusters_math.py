@@ -91,3 +91,36 @@ def test_truncation_rule_bounds_the_tail():
         N = int(bc.terms_needed(np.array([q]), 1e-11, 256)[0])
         tail = lambda n: sum((l + 1) * q ** l for l in range(n, 4000))  # noqa: E731
         assert tail(N) <= 1e-11 < tail(N - 1), q
+
+
+def test_legendre_bounds_behind_the_tail_rules():
+    """The truncation rules of the cluster expansions and of the multipole
+    zone bound both field components by sum (l + 1) q^l |m_l|: they need
+    |P_l| <= 1 and |sin t P_l'(cos t)| <= l + 1 on [-1, 1] (l <= 256, on a
+    fine grid; the largest ratio is about 0.58)."""
+    x = np.linspace(-1.0, 1.0, 100001)
+    P = np.zeros((258, x.size))
+    P[0], P[1] = 1.0, x
+    for l in range(1, 257):
+        P[l + 1] = ((2 * l + 1) * x * P[l] - l * P[l - 1]) / (l + 1)
+    dP = np.zeros_like(P)
+    dP[1] = 1.0
+    for l in range(1, 257):
+        dP[l + 1] = dP[l - 1] + (2 * l + 1) * P[l]
+    s = np.sqrt(1.0 - x * x)
+    assert np.abs(P).max() <= 1.0 + 1e-12
+    ratio = max(np.abs(s * dP[l]).max() / (l + 1) for l in range(258))
+    assert ratio <= 1.0, ratio
+
+
+def test_multipole_truncation_is_far_below_the_budget():
+    """multipole_pert stops where sum_{k > l} (k + 1) q^k <= 1e-12, so each
+    dropped component is below 1e-12 max|m_l| / 4 pi r^2; for the 400-panel
+    post's charge at its validated radius (1.083 r_ref) that is below
+    1e-12 of E_gun (1e8 V/m), six orders under the 1e-6 field budget."""
+    from bem_amd.bem import multipole as mpl
+
+    d = golden("bem_post400.npz")
+    m, r_ref = mpl.moments(d["profile"], d["sigma"], 8, 128)
+    bound = 1e-12 * np.abs(m).max() / (4 * np.pi * (1.083 * r_ref) ** 2)
+    assert bound <= 1e-12 * 1e8, bound
