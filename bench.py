@@ -56,7 +56,7 @@ TRAFFIC_SOURCE = ("FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 Mi
                   "profiles/bem_field_gather_post400_262k.md")
 FAST_TRAFFIC_BYTES = 13.4e6
 FAST_TRAFFIC_SOURCE = ("FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB copy), fast-form gather at "
-                       "(192, 64), profiles/bem_field_gather_fast_form_post400_262k.md")
+                       "(256, 64) on the points in bin order, profiles/bem_field_gather_fast_form_post400_262k.md")
 
 
 def workload(n_el=262144, group=16, n_segments=400, seed=0, rtol=1e-8):
@@ -155,14 +155,40 @@ def gather_work_per_point(geo, pts, k4, k2, near_width=0.5, clusters=None, sampl
             float(terms.mean()))
 
 
-def field_gather_timing(wl, geo, n_pts=262144, reps=10, shape=None, budget=None):
+def bin_order(geo, pts):
+    """The order in which the binned lockstep gather takes points: by
+    distance band (floor log2 of the distance to the closest segment over
+    its length, from -12 to 7) then closest segment (first index on ties),
+    as the library's bin key; ties keep the point order."""
+    prof = np.asarray(geo.solution.profile)
+    p0, t = prof[:-1], np.diff(prof, axis=0)
+    L = np.linalg.norm(t, axis=1)
+    u = t / L[:, None]
+    rho, z = np.hypot(pts[:, 0], pts[:, 1]), pts[:, 2]
+    key = np.empty(len(pts), dtype=np.int64)
+    for a in range(0, len(pts), 4096):
+        dr, dz = rho[a:a + 4096, None] - p0[None, :, 0], z[a:a + 4096, None] - p0[None, :, 1]
+        s0 = np.clip(dr * u[None, :, 0] + dz * u[None, :, 1], 0.0, L[None])
+        D = np.hypot(dr - s0 * u[None, :, 0], dz - s0 * u[None, :, 1])
+        j = D.argmin(1)
+        d = D[np.arange(len(j)), j] / L[j]
+        band = np.clip(np.floor(np.log2(np.maximum(d, 1e-300))) + 12, 0, 19).astype(np.int64)
+        key[a:a + 4096] = band * len(L) + j
+    return np.argsort(key, kind="stable")
+
+
+def field_gather_timing(wl, geo, n_pts=262144, reps=10, shape=None, budget=None, binned=False):
     """Average launch duration of the BEM field gather over gather_points(),
     timed with HIP events on the stream the kernel is launched on (the
     library is bound to torch's current stream); with a FieldBudget, the
-    far-rule gather of the fast form (far rule, near width, clusters)."""
+    far-rule gather of the fast form (far rule, near width, clusters);
+    binned: the points in bin order, as the binned lockstep gather takes
+    them."""
     import contextlib
 
     pts = gather_points(wl, n_pts)
+    if binned:
+        pts = np.ascontiguousarray(pts[bin_order(geo, pts)])
     ctx = geo._ctx()
     cm = (geo.far_rule(*budget.far_rule, near_width=budget.near_width, clusters=budget.clusters) if budget
           else contextlib.nullcontext())
@@ -411,7 +437,7 @@ def main():
         import torch
 
         bud = None if args.plain_gather else bbudget.fit(geo, tol=args.tol, use_multipole=False, n_val=1 << 12)
-        field_ms, n_pts = field_gather_timing(wl, geo, reps=3, shape=shape_arg, budget=bud)
+        field_ms, n_pts = field_gather_timing(wl, geo, reps=3, shape=shape_arg, budget=bud, binned=bud is not None)
         src = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
@@ -466,7 +492,16 @@ def main():
     o3, w3 = timed_traces(t3, 1, 0)
     net_off = dict(value=o3[0]["particle_stages"] / w3, wall_s=w3, attempts_mean=float(o3[0]["attempts"].mean()),
                    screen_t_rel_diff_vs_plain=screen_time_diff(plain_outs[0], o3[0]))
-    say(f"bench: fast trace without the network {w3:.3f} s")
+    # a trace that runs the network gathers in particle order (the binned
+    # gather is for traces without it): the network's gain is measured
+    # against the fast form gathered the same way
+    t3u = make_tracer(wl, geo, **bud.tracer_kwargs())
+    t3u.field_shape = tr.field_shape
+    t3u.binned = False
+    o3u, w3u = timed_traces(t3u, 1, 0)
+    net_off["unbinned"] = dict(value=o3u[0]["particle_stages"] / w3u, wall_s=w3u,
+                               screen_t_rel_diff_vs_binned=screen_time_diff(o3[0], o3u[0]))
+    say(f"bench: fast trace without the network {w3:.3f} s (particle-order gather {w3u:.3f} s)")
     # what the network gains at the tolerances where validation gives it a
     # band: over the plain method (network + the reference's fixed rule vs the
     # plain trace) and over the fast form without it
@@ -488,10 +523,11 @@ def main():
                                                      screen_t_rel_diff_vs_plain=screen_time_diff(plain_outs[0],
                                                                                                  o4[0])),
                               with_fast_form=dict(wall_s=w, value=o[0]["particle_stages"] / w,
-                                                  network_gain_wall=w3 / w,
+                                                  network_gain_wall=w3u / w, network_gain_wall_vs_binned=w3 / w,
                                                   screen_t_rel_diff_vs_plain=screen_time_diff(plain_outs[0], o[0]))))
             say(f"bench: network at tol {tol:g}: band {s.cfg['band']:.2e} m, with the plain gather {w4:.2f} s "
-                f"(plain {plain_wall:.2f} s), with the fast form {w:.3f} s (without the network {w3:.3f} s)")
+                f"(plain {plain_wall:.2f} s), with the fast form {w:.3f} s (without the network {w3u:.3f} s in "
+                f"particle order, {w3:.3f} s binned)")
         cost = surrogate_cost(wl, geo, sur, bud)
         say(f"bench: per point in the network's domain: fast gather {cost['fast_gather_ns_per_point']:.2f} ns, "
             f"network {cost['network_fp64_ns_per_point']:.2f} ns")
@@ -500,7 +536,7 @@ def main():
     pts = gather_points(wl)
     n_pts = len(pts)
     fast_shape = tuple(tr.field_shape or (0, 0))
-    fast_ms, _ = field_gather_timing(wl, geo, shape=fast_shape, budget=bud)
+    fast_ms, _ = field_gather_timing(wl, geo, shape=fast_shape, budget=bud, binned=True)
     ring, near, terms = gather_work_per_point(geo, pts, *bud.far_rule, near_width=bud.near_width,
                                               clusters=bud.clusters)
     fast_flops = n_pts * ((ring + near) * C_RING + terms * C_TERM)
@@ -510,9 +546,10 @@ def main():
                  "traffic_source": FAST_TRAFFIC_SOURCE,
                  "kernel": "bem_pert_kernel, far-rule instantiation with the field budget (far rule "
                            f"{bud.far_rule}, near width {bud.near_width}, {bud.clusters.n_clusters} cluster "
-                           "expansions): the same device code as the value trace's ls_field_kernel",
+                           "expansions) on the points in bin order: the device code of the value trace's "
+                           "ls_bin_field_kernel (64 points per block, wave-major, per-lane rules)",
                  "kernel_ms_mean": fast_ms, "points_per_launch": n_pts, "field_shape": list(fast_shape),
-                 "traffic_field_shape": [192, 64],
+                 "traffic_field_shape": [256, 64],
                  "ring_nodes_per_point": ring, "near_nodes_per_point": near, "expansion_terms_per_point": terms,
                  "flops_per_launch": fast_flops,
                  "note": "FP64 VALU-bound (MI355X FP64 vector peak = FP64 MFMA peak = 78.6 TFLOP/s); algorithmic "
