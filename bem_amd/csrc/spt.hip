@@ -1969,32 +1969,31 @@ __global__ __launch_bounds__(kBinThreads) void ls_bin_slot_kernel(LockParams L) 
   if (L.g_flags[L.pgroup[gi]] & LS_RUNNING) ls_bin_pack(L, gi, 0, LKB(48), LKB(49), LKB(50));
 }
 
-// The multipole-zone records of this stage, one per thread (they sit
-// together at the end of the slot list, after the distance-band bins).
-__global__ __launch_bounds__(256) void ls_bin_mp_kernel(LockParams L) {
-  const BinView& X = L.X;
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= (int64_t)*X.total) return;
-  const double4 r = X.rin[i];
-  double qx, qy, qz;
-  if (__double_as_longlong(r.w) <= -2 && multipole_pert(L.T.bem, r.x, r.y, r.z, qx, qy, qz))
-    X.rout[i] = make_double4(
-        qx, qy, qz, __longlong_as_double((long long)bin_key_mp(X, L.T.bem, fma(r.x, r.x, fma(r.y, r.y, r.z * r.z)))));
-}
-
-// This stage's gather records: 64 consecutive slots per block, lane = slot
-// (multipole-zone records are ls_bin_mp_kernel's, but a record that the
-// zone test rejects falls back to the gather).
+// This stage's slot records. Blocks below nb_gather: gather records, 64
+// consecutive slots per block, lane = slot (a multipole-zone record that
+// the zone test rejects falls back to the gather). Blocks from nb_gather
+// on: multipole-zone records, one per thread (they sit together at the end
+// of the slot list, after the distance-band bins).
 template <int T>
-__global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_bin_field_kernel(LockParams L) {
+__global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_bin_field_kernel(LockParams L, int nb_gather) {
   __shared__ FieldShared<T, 64> sh;
   __shared__ int okey[64];
   const BinView& X = L.X;
-  const int64_t base = (int64_t)blockIdx.x * 64;
   const int64_t tot = (int64_t)*X.total;
+  const int tid = threadIdx.x;
+  if ((int)blockIdx.x >= nb_gather) {
+    const int64_t i = (int64_t)(blockIdx.x - nb_gather) * T + tid;
+    if (i >= tot) return;
+    const double4 r = X.rin[i];
+    double qx, qy, qz;
+    if (__double_as_longlong(r.w) <= -2 && multipole_pert(L.T.bem, r.x, r.y, r.z, qx, qy, qz))
+      X.rout[i] = make_double4(
+          qx, qy, qz, __longlong_as_double((long long)bin_key_mp(X, L.T.bem, fma(r.x, r.x, fma(r.y, r.y, r.z * r.z)))));
+    return;
+  }
+  const int64_t base = (int64_t)blockIdx.x * 64;
   if (base >= tot) return;
   const int m = (int)min((int64_t)64, tot - base);
-  const int tid = threadIdx.x;
   if (tid < 64) {
     int v = 0;
     if (tid < m) {
@@ -2183,28 +2182,35 @@ __global__ __launch_bounds__(64) void ls_surrogate_kernel(LockParams L, int st) 
 }
 
 // Acceleration of stage st for one group (reference geometry.py:289-299,
-// batched_rk45.py:362-363), then the next stage's state.
-template <int TG, int PR>
+// batched_rk45.py:362-363), then the next stage's state. GPB groups of at
+// most TG / GPB particles per block (GPB > 1 packs small groups so a
+// wavefront's lanes are not mostly idle; each group's arithmetic is the
+// same as with GPB = 1).
+template <int TG, int PR, int GPB = 1>
 __global__ __launch_bounds__(TG) void ls_accel_kernel(LockParams L, int st) {
+  constexpr int W = TG / GPB;  // lanes per group
   __shared__ double sx[TG], sy[TG], sz[TG], sq[TG];
   __shared__ int act[TG];
-  const int g = blockIdx.x;
-  if (!(L.g_flags[g] & LS_RUNNING)) return;
-  const int tid = threadIdx.x;
-  const int64_t off = L.T.offsets[g];
-  const int n = (int)(L.T.offsets[g + 1] - off);
+  const int gs = threadIdx.x / W;
+  const int g = blockIdx.x * GPB + gs;
+  const bool run = g < L.T.n_groups && (L.g_flags[g] & LS_RUNNING);
+  if (GPB == 1 && !run) return;  // block-uniform
+  const int tid = threadIdx.x - gs * W;
+  const int64_t off = run ? L.T.offsets[g] : 0;
+  const int n = run ? (int)(L.T.offsets[g + 1] - off) : 0;
   const bool mine = tid < n;
   const int64_t gi = off + tid;
+  const int lt = gs * W + tid;  // this thread's LDS slot
   if (mine) {
-    sx[tid] = LKB(48);
-    sy[tid] = LKB(49);
-    sz[tid] = LKB(50);
-    sq[tid] = L.T.charge[gi];
-    act[tid] = L.active[gi];
+    sx[lt] = LKB(48);
+    sy[lt] = LKB(49);
+    sz[lt] = LKB(50);
+    sq[lt] = L.T.charge[gi];
+    act[lt] = L.active[gi];
     if (L.X.key && L.X.src[gi]) {  // the binned gather's output for this stage
       const double4 r = L.X.rout[L.X.slot[gi]];
       double ex, ey, ez;
-      geometry_field_point(L.T.G, sx[tid], sy[tid], sz[tid], r.x, r.y, r.z, ex, ey, ez);
+      geometry_field_point(L.T.G, sx[lt], sy[lt], sz[lt], r.x, r.y, r.z, ex, ey, ez);
       LKB(51) = ex;
       LKB(52) = ey;
       LKB(53) = ez;
@@ -2214,10 +2220,11 @@ __global__ __launch_bounds__(TG) void ls_accel_kernel(LockParams L, int st) {
   }
   __syncthreads();
   if (!mine) return;
-  const bool active = act[tid];
+  const bool active = act[lt];
   const double hr = L.g_hr[g];
   double Fc[3], Fi[3];
-  pair_forces_p<PR>(L.T.G, tid, n, sx, sy, sz, sq, act, L.T.a2, Fc, Fi);
+  const int b = gs * W;
+  pair_forces_p<PR>(L.T.G, tid, n, sx + b, sy + b, sz + b, sq + b, act + b, L.T.a2, Fc, Fi);
   if (L.imgF)
     for (int c = 0; c < 3; ++c) Fi[c] = L.imgF[3 * gi + c];
   const double mass = L.T.mass[gi];
@@ -3258,7 +3265,7 @@ constexpr int kLockstepUnroll = 4;  // attempts per captured graph replay
 // PR: SPT_PREC_FP64, SPT_PREC_MIXED (FP32 far-field sums and pairs) or
 // SPT_PREC_PAIR32 (FP64 field, FP32 pairs).
 template <int TG, int PR>
-static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, FieldShape fs) {
+static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, FieldShape fs, int64_t max_size) {
   constexpr int FPR = PR == SPT_PREC_MIXED ? SPT_PREC_MIXED : SPT_PREC_FP64;  // field
   constexpr int APR = PR == SPT_PREC_FP64 ? SPT_PREC_FP64 : SPT_PREC_MIXED;   // pairs
   hipStream_t s = c->own;
@@ -3270,6 +3277,7 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
   if (L.T.n_out > 0) reset_u64_kernel<<<1, 1, 0, s>>>(L.T.snap_count);
   ls_init_kernel<TG><<<n_groups, TG, 0, s>>>(L);
   const bool bin = L.X.key != nullptr;
+  const bool pack4 = TG == 64 && max_size <= 16;  // four groups per accel block
   const unsigned nb_bin = (unsigned)((n + kBinThreads - 1) / kBinThreads);
   const unsigned nb_slots = (unsigned)((n + 63) / 64);
   if (bin) {
@@ -3289,11 +3297,12 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
       }
       for (int st = 0; st < 7; ++st) {
         if (bin) {
-          if (L.T.bem.mp) ls_bin_mp_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(L);
           if (fs.T == 192)
-            ls_bin_field_kernel<192><<<nb_slots, 192, 0, s>>>(L);
+            ls_bin_field_kernel<192><<<nb_slots + (L.T.bem.mp ? (unsigned)((n + 191) / 192) : 0u), 192, 0, s>>>(
+                L, (int)nb_slots);
           else
-            ls_bin_field_kernel<256><<<nb_slots, 256, 0, s>>>(L);
+            ls_bin_field_kernel<256><<<nb_slots + (L.T.bem.mp ? (unsigned)((n + 255) / 256) : 0u), 256, 0, s>>>(
+                L, (int)nb_slots);
         } else if (L.sur) {
           if (FPR == SPT_PREC_FP64)
             ls_surrogate_kernel<true><<<(unsigned)((n + MLP64_PTS - 1) / MLP64_PTS), 64, 0, s>>>(L, st);
@@ -3311,7 +3320,10 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
                                            L.g_flags, LS_RUNNING, n_groups, L.T.a2, (double*)L.imgF, s);
           if (r) return r;
         }
-        ls_accel_kernel<TG, APR><<<n_groups, TG, 0, s>>>(L, st);
+        if (pack4)
+          ls_accel_kernel<64, APR, 4><<<(unsigned)((n_groups + 3) / 4), 64, 0, s>>>(L, st);
+        else
+          ls_accel_kernel<TG, APR><<<n_groups, TG, 0, s>>>(L, st);
       }
       ls_end_kernel<TG><<<n_groups, TG, 0, s>>>(L);
     }
@@ -3600,9 +3612,9 @@ static int trace_impl(spt_ctx* c, const spt_trace_args* a, bool no_alloc) {
     int rc3;
     const int pr = a->precision;
 #define SPT_LS(TG)                                                                                         \
-  (pr == SPT_PREC_MIXED    ? run_lockstep<TG, SPT_PREC_MIXED>(c, L, a->n_groups, a->n_particles, fs)  \
-   : pr == SPT_PREC_PAIR32 ? run_lockstep<TG, SPT_PREC_PAIR32>(c, L, a->n_groups, a->n_particles, fs) \
-                           : run_lockstep<TG, SPT_PREC_FP64>(c, L, a->n_groups, a->n_particles, fs))
+  (pr == SPT_PREC_MIXED    ? run_lockstep<TG, SPT_PREC_MIXED>(c, L, a->n_groups, a->n_particles, fs, mx)  \
+   : pr == SPT_PREC_PAIR32 ? run_lockstep<TG, SPT_PREC_PAIR32>(c, L, a->n_groups, a->n_particles, fs, mx) \
+                           : run_lockstep<TG, SPT_PREC_FP64>(c, L, a->n_groups, a->n_particles, fs, mx))
     if (mx <= 64)
       rc3 = SPT_LS(64);
     else if (mx <= 128)
