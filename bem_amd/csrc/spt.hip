@@ -341,20 +341,23 @@ __device__ __forceinline__ bool multipole_pert(const BemView& B, double x, doubl
   const double omq = 1.0 - q;
   const double tolw = kMultipoleTail * omq * omq;
   double pl = 1.0, plm = 0.0, dl = 0.0, dlm = 0.0, qp = 1.0, sa = 0.0, sb = 0.0;
+  double lp1 = 1.0, tl1 = 1.0;  // l + 1 and 2l + 1, exact
   const cseg_ptr t = (cseg_ptr)B.mp;
   for (int l = 0; l < B.mp_n; ++l) {
     const double m = t[4 * l], a = t[4 * l + 1], b = t[4 * l + 2];
     const double mq = m * qp;
-    sa = fma(mq * (double)(l + 1), pl, sa);
+    sa = fma(mq * lp1, pl, sa);
     sb = fma(mq, dl, sb);
     qp *= q;
-    if (qp * fma(-(double)(l + 1), q, (double)(l + 2)) <= tolw) break;
+    if (qp * fma(-lp1, q, lp1 + 1.0) <= tolw) break;
     const double pn = fma(a * c, pl, -b * plm);
-    const double dn = fma((double)(2 * l + 1), pl, dlm);
+    const double dn = fma(tl1, pl, dlm);
     plm = pl;
     pl = pn;
     dlm = dl;
     dl = dn;
+    lp1 += 1.0;
+    tl1 += 2.0;
   }
   const double k = 1.0 / (4.0 * kPi * r2);
   const double er = k * sa, et = k * sn * sb;
@@ -454,18 +457,21 @@ __device__ __forceinline__ void cluster_expansion(const BemView& B, int c, doubl
   const cseg_ptr mom = (cseg_ptr)B.cl_mom + (size_t)c * B.cl_terms;
   const cseg_ptr cf = (cseg_ptr)B.cl_coef;
   double pl = 1.0, plm = 0.0, dl = 0.0, dlm = 0.0, qp = 1.0, sa = 0.0, sb = 0.0;
+  double lp1 = 1.0, tl1 = 1.0;  // l + 1 and 2l + 1, exact
   for (int l = 0; l < B.cl_terms; ++l) {
     const double mq = mom[l] * qp;
-    sa = fma(mq * (double)(l + 1), pl, sa);
+    sa = fma(mq * lp1, pl, sa);
     sb = fma(mq, dl, sb);
     qp *= q;
-    if (qp * fma(-(double)(l + 1), q, (double)(l + 2)) <= tolw) break;
+    if (qp * fma(-lp1, q, lp1 + 1.0) <= tolw) break;
     const double pn = fma(cf[2 * l] * cs, pl, -cf[2 * l + 1] * plm);
-    const double dn = fma((double)(2 * l + 1), pl, dlm);
+    const double dn = fma(tl1, pl, dlm);
     plm = pl;
     pl = pn;
     dlm = dl;
     dl = dn;
+    lp1 += 1.0;
+    tl1 += 2.0;
   }
   const double k = (ir * ir) * (1.0 / (4.0 * kPi));
   const double er = k * sa, et = k * sn * sb;
@@ -1178,6 +1184,9 @@ __device__ __noinline__ void screens_particle(const ScreenParams& SP, spt_event*
 }
 
 // ------------------------------------------------------------ trace kernel
+#include "spt_toroidal.inc"
+#include "spt_image.inc"
+
 struct TraceParams {
   spt_geometry G;
   BemView bem;
@@ -1660,8 +1669,6 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void geometry_field_kernel(
 }
 
 #include "spt_mlp.inc"
-#include "spt_toroidal.inc"
-#include "spt_image.inc"
 
 // ============================================================ lockstep mode
 // The reference's schedule (batched_rk45.py:219-319): every running group
@@ -3220,14 +3227,11 @@ static PosView stage_pos_view(const LockParams& L) {
   return X;
 }
 
-// The image-charge force for every (running) group: the right-hand sides,
-// the per-mode joint solves sigma_m = A_m^-1 rhs_m of all groups as one
-// strided-batched FP64 GEMM over the modes (A_m^-1 is read once per call
-// instead of once per group; rows = profile nodes, columns = (group,
-// cos/sin)), then the force. The scratch must have been sized by
-// ensure_image_scratch. rocBLAS runs eagerly on s (never inside a captured
-// graph). Columns of groups that are not running hold stale right-hand
-// sides; their sigma is never read.
+// The image-charge force for every (running) group: per group one
+// workgroup forms the right-hand sides and solves sigma_m = A_m^-1 rhs_m
+// (img_rhs_kernel), then another the force (img_force_kernel). The scratch
+// must have been sized by ensure_image_scratch. Groups that are not
+// running are skipped.
 static int launch_image_force(spt_ctx* c, const spt_geometry& G, PosView X, const double* charge,
                               const uint8_t* active, const int64_t* offsets, const int32_t* gflags, int running,
                               int n_groups, double a2, double* F, hipStream_t s) {
@@ -3243,13 +3247,6 @@ static int launch_image_force(spt_ctx* c, const spt_geometry& G, PosView X, cons
     img_rhs_kernel<17><<<n_groups, 256, 0, s>>>(B, X, charge, active, offsets, gflags, running);
   else
     img_rhs_kernel<33><<<n_groups, 256, 0, s>>>(B, X, charge, active, offsets, gflags, running);
-  if (rocblas_set_stream(c->blas, s) != rocblas_status_success) return fail(c, SPT_EHIP, "rocblas_set_stream");
-  const double one = 1.0, zero = 0.0;
-  const rocblas_int P = B.P, M = B.n_modes;
-  const rocblas_status st = rocblas_dgemm_strided_batched(
-      c->blas, rocblas_operation_none, rocblas_operation_none, P, 2 * n_groups, P, &one, B.Ainv, P,
-      (rocblas_stride)P * P, B.rhs, M * P, P, &zero, B.sigma, M * P, P, M);
-  if (st != rocblas_status_success) return fail(c, SPT_EHIP, "image solve (rocblas_dgemm_strided_batched)");
   if (nm <= 9)
     img_force_kernel<9><<<n_groups, 256, 0, s>>>(B, G, X, charge, active, offsets, gflags, running, a2, F);
   else if (nm <= 17)
@@ -3285,9 +3282,8 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
     ls_bin_key_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(L);
   }
   HIPCHK(c, hipGetLastError());
-  // kLockstepUnroll attempts of every running group. Captured once in a HIP
-  // graph and replayed; with the image-charge BEM (whose solve is a rocBLAS
-  // GEMM, kept out of graphs) the same sequence is issued eagerly.
+  // kLockstepUnroll attempts of every running group, captured once in a HIP
+  // graph and replayed.
   auto enqueue = [&]() -> int {
     for (int u = 0; u < kLockstepUnroll; ++u) {
       if (bin) {
@@ -3329,27 +3325,24 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
     }
     return SPT_OK;
   };
-  const bool eager = L.imgF != nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  if (!eager) {
-    HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    const int rq = enqueue();
-    HIPCHK(c, hipStreamEndCapture(s, &graph));
-    if (rq) return rq;
-    HIPCHK(c, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int rq = enqueue();
+  HIPCHK(c, hipStreamEndCapture(s, &graph));
+  if (rq) {
+    (void)hipGraphDestroy(graph);
+    return rq;
+  }
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return fail(c, SPT_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
   }
   int64_t iters = 0;
   int rc = SPT_OK;
   while (true) {
-    hipError_t e = hipSuccess;
-    if (eager) {
-      rc = enqueue();
-      if (rc) break;
-      e = hipGetLastError();
-    } else {
-      e = hipGraphLaunch(exec, s);
-    }
+    hipError_t e = hipGraphLaunch(exec, s);
     if (e == hipSuccess) e = hipMemcpyAsync(c->h_done, L.done, sizeof(unsigned int), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
