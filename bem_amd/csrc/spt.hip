@@ -1219,6 +1219,7 @@ struct TraceParams {
   int64_t snap_cap;
   unsigned long long* snap_count;
   double* t_end;  // per group: t after the last attempt, or nullptr
+  ImageView IB;   // image-charge BEM of the persistent kernel's image instantiations
 };
 
 // Output times at or before t (the ascending list's count below t's
@@ -1350,8 +1351,28 @@ constexpr int kSlots = 54;
 // Particle state lives in global memory (caller's pos/vel plus the stage
 // scratch) so that only a handful of scalars are live across the field
 // gather.
-template <int T, int NMAX, int PR>
-__device__ void run_group(const TraceParams& P, int g, TraceShared<T, NMAX>& sh) {
+// Per-workgroup LDS of the image-charge BEM (NM > 0 modes in registers;
+// NM = 0: none).
+template <int NM>
+union ImgLds {
+  ImgRhsLds<NM> r;
+  ImgForceLds f;
+};
+template <int NM>
+struct ImgWork {
+  ImgLds<NM> u;
+};
+template <>
+struct ImgWork<0> {};
+
+// PR: SPT_PREC_FP64, SPT_PREC_MIXED (FP32 far-field sums and pairs) or
+// SPT_PREC_PAIR32 (FP64 field, FP32 + Kahan pairs). NM > 0: the
+// image-charge BEM force of the group at every stage (T = NMAX = 256).
+template <int T, int NMAX, int PR, int NM = 0>
+__device__ void run_group(const TraceParams& P, int g, TraceShared<T, NMAX>& sh, ImgWork<NM>& iw) {
+  constexpr int FPR = PR == SPT_PREC_MIXED ? SPT_PREC_MIXED : SPT_PREC_FP64;  // field
+  constexpr int APR = PR == SPT_PREC_FP64 ? SPT_PREC_FP64 : SPT_PREC_MIXED;   // pairs
+  static_assert(NM == 0 || (T == 256 && NMAX == 256), "the image-charge BEM runs in 256-thread workgroups");
   const int tid = threadIdx.x;
   const int64_t off = P.offsets[g];
   const int n = (int)(P.offsets[g + 1] - off);
@@ -1418,7 +1439,34 @@ __device__ void run_group(const TraceParams& P, int g, TraceShared<T, NMAX>& sh)
         for (int c = 0; c < 3; ++c) KB(st * 6 + c) = active ? vs[c] : 0.0;
       }
       __syncthreads();
-      stage_field<T, NMAX, PR>(P, off, n, st, sh);
+      stage_field<T, NMAX, FPR>(P, off, n, st, sh);
+      if constexpr (NM > 0) {  // the image-charge force at the stage positions, into cx, cy, cz
+        auto mem = [&](int e, double& x, double& y, double& z, double& q) {
+          x = sh.sx[e];
+          y = sh.sy[e];
+          z = sh.sz[e];
+          q = sh.sact[e] ? sh.sq[e] : 0.0;
+        };
+        __syncthreads();
+        img_rhs_group<NM>(P.IB, off, n, g, mem, iw.u.r);
+        __syncthreads();
+        img_solve_group(P.IB, g);
+        __syncthreads();
+        auto memf = [&](int e, double& x, double& y, double& z, double& q, int& a) {
+          x = sh.sx[e];
+          y = sh.sy[e];
+          z = sh.sz[e];
+          a = sh.sact[e];
+          q = a ? sh.sq[e] : 0.0;
+        };
+        auto out = [&](int e, double fx, double fy, double fz) {
+          sh.cx[e] = fx;
+          sh.cy[e] = fy;
+          sh.cz[e] = fz;
+        };
+        img_force_group<NM>(P.IB, P.G, off, n, g, P.a2, memf, out, iw.u.f);
+        __syncthreads();
+      }
       if (mine && active) {  // keep stage 1's field (the state's) and stage 7's (the end point's)
         if (st == 0) {
           KB(48) = sh.fx[tid];
@@ -1433,7 +1481,12 @@ __device__ void run_group(const TraceParams& P, int g, TraceShared<T, NMAX>& sh)
       }
       if (mine) {
         double Fc[3], Fi[3];
-        pair_forces_p<PR>(P.G, tid, n, sh.sx, sh.sy, sh.sz, sh.sq, sh.sact, P.a2, Fc, Fi);
+        pair_forces_p<APR>(P.G, tid, n, sh.sx, sh.sy, sh.sz, sh.sq, sh.sact, P.a2, Fc, Fi);
+        if constexpr (NM > 0) {
+          Fi[0] = sh.cx[tid];
+          Fi[1] = sh.cy[tid];
+          Fi[2] = sh.cz[tid];
+        }
         const double mass = P.mass[gi];
         const double F[3] = {sh.fx[tid], sh.fy[tid], sh.fz[tid]};
         for (int c = 0; c < 3; ++c) {
@@ -1525,9 +1578,10 @@ __device__ void run_group(const TraceParams& P, int g, TraceShared<T, NMAX>& sh)
   __syncthreads();
 }
 
-template <int T, int NMAX, int PR>
+template <int T, int NMAX, int PR, int NM = 0>
 __global__ __launch_bounds__(T) void trace_kernel(TraceParams P) {
   __shared__ TraceShared<T, NMAX> sh;
+  __shared__ ImgWork<NM> iw;
   while (true) {
     if (threadIdx.x == 0) sh.slot = atomicAdd(P.queue, 1);
     __syncthreads();
@@ -1535,7 +1589,7 @@ __global__ __launch_bounds__(T) void trace_kernel(TraceParams P) {
     __syncthreads();
     if (q >= P.n_groups) break;
     const int g = P.order ? P.order[q] : q;
-    run_group<T, NMAX, PR>(P, g, sh);
+    run_group<T, NMAX, PR, NM>(P, g, sh, iw);
   }
 }
 
@@ -3162,14 +3216,14 @@ int spt_kill_mask(spt_ctx* c, const spt_geometry* g, const double* pos, int64_t 
 
 }  // extern "C"
 
-template <int T, int NMAX, int PR>
+template <int T, int NMAX, int PR, int NM = 0>
 static int launch_trace(spt_ctx* c, const TraceParams& P, int n_groups) {
   int per_cu = 0;
-  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_kernel<T, NMAX, PR>, T, 0));
+  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_kernel<T, NMAX, PR, NM>, T, 0));
   if (per_cu < 1) per_cu = 1;
   int64_t grid = (int64_t)per_cu * c->n_cu;
   if (grid > n_groups) grid = n_groups;
-  trace_kernel<T, NMAX, PR><<<(unsigned)grid, T, 0, c->stream>>>(P);
+  trace_kernel<T, NMAX, PR, NM><<<(unsigned)grid, T, 0, c->stream>>>(P);
   HIPCHK(c, hipGetLastError());
   return SPT_OK;
 }
@@ -3463,10 +3517,17 @@ static int trace_impl(spt_ctx* c, const spt_trace_args* a, bool no_alloc) {
   if (a->mode != SPT_MODE_PERSISTENT && a->mode != SPT_MODE_LOCKSTEP) return fail(c, SPT_EINVAL, "unknown mode");
   if (a->precision != SPT_PREC_FP64 && a->precision != SPT_PREC_MIXED && a->precision != SPT_PREC_PAIR32)
     return fail(c, SPT_EINVAL, "unknown precision");
-  if (a->precision == SPT_PREC_PAIR32 && a->mode != SPT_MODE_LOCKSTEP)
-    return fail(c, SPT_EINVAL, "precision PAIR32 (FP64 field, FP32 pairs) runs in lockstep mode only");
-  if (is_image_bem(a->geom.image) && a->mode != SPT_MODE_LOCKSTEP)
-    return fail(c, SPT_EINVAL, "the image-charge BEM runs in lockstep mode only");
+  if (a->mode == SPT_MODE_PERSISTENT) {
+    if (a->precision == SPT_PREC_PAIR32 && T > 256)
+      return fail(c, SPT_EINVAL, "precision PAIR32 in persistent mode is built for 64-, 128- and 256-thread workgroups");
+    if (is_image_bem(a->geom.image)) {
+      if (a->precision == SPT_PREC_MIXED)
+        return fail(c, SPT_EINVAL, "the image-charge BEM in persistent mode runs with precision FP64 or PAIR32");
+      if (a->block_threads && a->block_threads != 256)
+        return fail(c, SPT_EINVAL, "the image-charge BEM in persistent mode runs in 256-thread workgroups");
+      T = 256;
+    }
+  }
   if (a->use_surrogate) {
     if (a->mode != SPT_MODE_LOCKSTEP) return fail(c, SPT_EINVAL, "the field surrogate runs in lockstep mode only");
     if (!c->have_sur) return fail(c, SPT_EINVAL, "use_surrogate set but no surrogate (spt_surrogate_set)");
@@ -3619,14 +3680,37 @@ static int trace_impl(spt_ctx* c, const spt_trace_args* a, bool no_alloc) {
     HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_out, 0));
     return rc3;
   }
+  const bool p32 = a->precision == SPT_PREC_PAIR32;
+  if (is_image_bem(a->geom.image)) {
+    if (no_alloc) return fail(c, SPT_EINVAL, "spt_trace_noalloc does not run the image-charge BEM");
+    int ri = ensure_image_scratch(c, a->n_groups, a->n_particles);
+    if (ri) return ri;
+    P.IB = image_view(c);
+    if (a->geom.image == SPT_IMAGE_BEM_MIRROR) {
+      P.IB.mirror_mode = 1;
+      P.IB.plane_z = -a->geom.z0;
+    }
+  }
   reset_int_kernel<<<1, 1, 0, c->stream>>>(P.queue, 0);
   reset_u64_kernel<<<1, 1, 0, c->stream>>>(P.ev_count);
   if (P.n_out > 0) reset_u64_kernel<<<1, 1, 0, c->stream>>>(P.snap_count);
+  if (is_image_bem(a->geom.image)) {
+    if (c->img_modes <= 17)
+      return p32 ? launch_trace<256, 256, SPT_PREC_PAIR32, 17>(c, P, a->n_groups)
+                 : launch_trace<256, 256, SPT_PREC_FP64, 17>(c, P, a->n_groups);
+    return p32 ? launch_trace<256, 256, SPT_PREC_PAIR32, 33>(c, P, a->n_groups)
+               : launch_trace<256, 256, SPT_PREC_FP64, 33>(c, P, a->n_groups);
+  }
   switch (T) {
-    case 64: return launch_trace<64, 64, SPT_PREC_FP64>(c, P, a->n_groups);
-    case 128: return launch_trace<128, 128, SPT_PREC_FP64>(c, P, a->n_groups);
+    case 64:
+      return p32 ? launch_trace<64, 64, SPT_PREC_PAIR32>(c, P, a->n_groups)
+                 : launch_trace<64, 64, SPT_PREC_FP64>(c, P, a->n_groups);
+    case 128:
+      return p32 ? launch_trace<128, 128, SPT_PREC_PAIR32>(c, P, a->n_groups)
+                 : launch_trace<128, 128, SPT_PREC_FP64>(c, P, a->n_groups);
     case 256:
       return a->precision == SPT_PREC_MIXED ? launch_trace<256, 256, SPT_PREC_MIXED>(c, P, a->n_groups)
+             : p32                          ? launch_trace<256, 256, SPT_PREC_PAIR32>(c, P, a->n_groups)
                                             : launch_trace<256, 256, SPT_PREC_FP64>(c, P, a->n_groups);
     case 512: return launch_trace<512, 256, SPT_PREC_FP64>(c, P, a->n_groups);
     default: return launch_trace<1024, 64, SPT_PREC_FP64>(c, P, a->n_groups);

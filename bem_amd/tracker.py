@@ -13,11 +13,12 @@ device time), and ``mode``: "persistent" (one workgroup integrates whole
 groups from a device queue), "lockstep" (every running group makes one
 attempt per iteration, per-stage kernels replayed from a HIP graph; the
 field gather spreads each particle over many threads) or "auto" (lockstep
-when there are fewer groups than twice the compute units); and
+when there are fewer groups than twice the compute units, or with FP32 pairs
+or the image-charge BEM); and
 ``precision``: "fp64" (the plain method) or "mixed" (FP32 ring-kernel sums
 over segments at least two panel lengths away and FP32 pair forces, both
 Kahan-compensated; FP64 elsewhere) or "pairs32" (FP64 field, FP32 Kahan
-pair forces; lockstep); and ``surrogate``: a trained
+pair forces); and ``surrogate``: a trained
 bem_amd.surrogate.FieldSurrogate for the geometry, used in lockstep mode
 for the BEM perturbation wherever its validated band allows; and
 ``far_rule``: None (the reference's fixed far rule) or (k4, k2), the
@@ -88,8 +89,6 @@ class SpecificParticleTracer:
 
         self.precision = precision
         self._precision = _precision_code(precision, trace=True)
-        if precision == "pairs32" and mode == "persistent":
-            raise ValueError("precision 'pairs32' (FP64 field, FP32 pair forces) runs in lockstep mode only")
         if surrogate is not None:
             if mode == "persistent":
                 raise ValueError("a field surrogate runs in lockstep mode only (mode='lockstep' or 'auto')")
@@ -187,14 +186,13 @@ class SpecificParticleTracer:
         return np.argsort(-sizes, kind="stable").astype(np.int32)
 
     def _mode_code(self, ctx):
-        if getattr(self.geometry, "image_profile", None) is not None and self.mode == "persistent":
-            raise ValueError("the image-charge BEM runs in lockstep mode only")
-        if (self.surrogate is not None or self.precision == "pairs32"
-                or getattr(self.geometry, "image_profile", None) is not None):
+        if self.surrogate is not None:
             return _lib.MODE_LOCKSTEP
         if self.mode == "persistent":
             return _lib.MODE_PERSISTENT
         if self.mode == "lockstep":
+            return _lib.MODE_LOCKSTEP
+        if self.precision == "pairs32" or getattr(self.geometry, "image_profile", None) is not None:
             return _lib.MODE_LOCKSTEP
         import ctypes
 

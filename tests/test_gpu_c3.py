@@ -1,8 +1,8 @@
 """C3's shape under test on the GPU: the cylindrical well (R 50 nm, H 30 nm)
 with uneven CSR groups whose sizes span 1..256 and per-group emission
 offsets, in the 64/128/256-thread instantiations of both trace kernels,
-with and without the image-charge BEM (z0 3 nm); and the 2000-panel well of
-the benchmark's C3 sample.
+with and without the image-charge BEM (z0 3 nm, both kernels); and the
+2000-panel well of the benchmark's C3 sample.
 
 Each compared group equals an oracle run of that group alone: the
 reference's groups never interact (reference batched_rk45.py:4-22), and
@@ -107,13 +107,15 @@ def _compare_groups(geo, og, sizes, compare, screens, t_max, max_steps, seed, **
     return out
 
 
-def test_image_bem_groups_up_to_256_match_oracle(gpu):
-    """Lockstep with the image-charge BEM, largest group 256 (the 256-thread
-    instantiation): groups of 256 and 180 members, and a single electron."""
+@pytest.mark.parametrize("mode", ["lockstep", "persistent"])
+def test_image_bem_groups_up_to_256_match_oracle(gpu, mode):
+    """The image-charge BEM, largest group 256 (lockstep: the 256-thread
+    instantiation; persistent: 256-thread workgroups that solve their own
+    group): groups of 256 and 180 members, and a single electron."""
     geo, og = _well_image()
     sizes = [7, 256, 1, 180, 33, 97, 140, 2]
-    _say("image BEM, sizes", sizes)
-    out = _compare_groups(geo, og, sizes, [1, 3, 2], [-25e-9, -15e-9], 8e-14, 6, seed=31)
+    _say(f"image BEM [{mode}], sizes", sizes)
+    out = _compare_groups(geo, og, sizes, [1, 3, 2], [-25e-9, -15e-9], 8e-14, 6, seed=31, mode=mode)
     assert np.all(out["attempts"] == 6)
 
 

@@ -165,10 +165,12 @@ def test_c5_forms_match_reference_run_within_bounds(post400):
     tr = _tracer(post400, src, 1e-8, mode="lockstep", t_max=t_max)
     screens, _ = tr.run()
     _check_exact(tr.last_run, d, [0, 1, 2], screens)
-    out = _tracer(post400, src, 1e-8, mode="lockstep", precision="pairs32", t_max=t_max).trace()
-    dt = _check_bound(out, d, [0, 1, 2], 1e-7)
-    _say(f"C5 golden groups, FP32 pairs: crossing time max rel diff {dt:.2e}, attempts {out['attempts'].tolist()}")
-    assert dt <= 1e-7
+    for mode in ("lockstep", "persistent"):
+        out = _tracer(post400, src, 1e-8, mode=mode, precision="pairs32", t_max=t_max).trace()
+        dt = _check_bound(out, d, [0, 1, 2], 1e-7)
+        _say(f"C5 golden groups, FP32 pairs [{mode}]: crossing time max rel diff {dt:.2e}, "
+             f"attempts {out['attempts'].tolist()}")
+        assert dt <= 1e-7
     mp = mpl.fit(post400, n_terms=128, tol=1e-8)
     assert not mp.inert
     out = _tracer(post400, src, 1e-8, mode="lockstep", precision="pairs32", t_max=t_max, far_rule=(2.0, 16.0),

@@ -66,25 +66,26 @@ def test_image_force_matches_reference(gpu):
     assert np.all(F[~d["active"]] == 0.0)
 
 
-def test_image_trace_matches_reference_run(gpu):
+@pytest.mark.parametrize("mode", ["lockstep", "persistent"])
+def test_image_trace_matches_reference_run(gpu, mode):
+    """Both trace kernels: lockstep (per-stage kernels over all groups) and
+    the persistent kernel (each workgroup forms its group's right-hand
+    sides, solves and forces at every stage)."""
     from bem_amd import SpecificParticleTracer
     from bem_amd.particles import ParticleGroup
 
-    _say("image trace")
+    _say(f"image trace [{mode}]")
     d, geo = _geo()
     r = golden("run_wellimg.npz")
     src = {k[3:]: r[k] for k in r.files if k.startswith("in_")}
     n = len(src["x"])
     pg = ParticleGroup(dict(src, weight=np.full(n, 1.602176634e-19), status=np.ones(n), species="electron"))
-    with pytest.raises(ValueError, match="lockstep"):
-        SpecificParticleTracer(pg, int(r["n_emit"]), geo, list(r["screens"]), rtol=float(r["rtol"]),
-                               atol=float(r["atol"]), t_max=float(r["t_max"]), mode="persistent").trace()
     tr = SpecificParticleTracer(pg, int(r["n_emit"]), geo, list(r["screens"]), rtol=float(r["rtol"]),
-                                atol=float(r["atol"]), t_max=float(r["t_max"]))
+                                atol=float(r["atol"]), t_max=float(r["t_max"]), mode=mode)
     screens, _ = tr.run()
     out = tr.last_run
     G = tr.n_groups
-    _say(f"image trace: attempts {out['attempts'].tolist()} vs {r['attempts'].tolist()}")
+    _say(f"image trace [{mode}]: attempts {out['attempts'].tolist()} vs {r['attempts'].tolist()}")
     np.testing.assert_array_equal(out["attempts"], r["attempts"])
     np.testing.assert_array_equal(out["accepted"], r["accepted"])
     pos = out["pos"].reshape(G, -1, 3)
@@ -123,18 +124,19 @@ def test_tip_image_operators_and_force_match_reference(gpu, tag):
     assert np.all(F[~d["active"]] == 0.0)
 
 
-def test_tip_mirror_image_trace_matches_reference_run(gpu):
+@pytest.mark.parametrize("mode", ["lockstep", "persistent"])
+def test_tip_mirror_image_trace_matches_reference_run(gpu, mode):
     from bem_amd import SpecificParticleTracer
     from bem_amd.particles import ParticleGroup
 
-    _say("tip mirror-image trace")
+    _say(f"tip mirror-image trace [{mode}]")
     d, geo = _tip_geo("doubled")
     r = golden("run_tipimg.npz")
     src = {k[3:]: r[k] for k in r.files if k.startswith("in_")}
     n = len(src["x"])
     pg = ParticleGroup(dict(src, weight=np.full(n, 1.602176634e-19), status=np.ones(n), species="electron"))
     tr = SpecificParticleTracer(pg, int(r["n_emit"]), geo, list(r["screens"]), rtol=float(r["rtol"]),
-                                atol=float(r["atol"]), t_max=float(r["t_max"]))
+                                atol=float(r["atol"]), t_max=float(r["t_max"]), mode=mode)
     screens, _ = tr.run()
     out = tr.last_run
     G = tr.n_groups
