@@ -9,12 +9,14 @@ of:
 * mode "persistent" with each workgroup size that holds the largest group
   (64, 128, 256 threads), one group per workgroup from a device queue;
 * mode "lockstep" with the BEM field gather at 192 or 256 threads per
-  block and 1/2x, 1x, 2x the automatic points per block.
+  block and 1/4x .. 4x the automatic points per block (the binned gather of
+  far-rule traces always takes 64 points per block: only its thread count
+  is tuned).
 
 The winner is written back to the tracer (mode, block_threads,
-field_shape) and returned with the whole timing table. Forms that run in
-lockstep only (surrogate, image-charge BEM, FP32 pairs) skip the
-persistent candidates.
+field_shape) and returned with the whole timing table. Traces with the
+field surrogate (lockstep only) skip the persistent candidates, and so do
+far-rule traces, whose binned gather is lockstep's.
 """
 import numpy as np
 
@@ -37,15 +39,20 @@ def candidates(tracer, n_cu):
     biggest = int(sizes.max())
     n = int(sizes.sum())
     out = []
-    lockstep_only = (tracer.surrogate is not None or tracer.precision == "pairs32"
-                     or getattr(tracer.geometry, "image_profile", None) is not None)
-    if not lockstep_only:
+    binned = bool(getattr(tracer, "binned", 0)) and tracer.far_rule is not None and tracer.surrogate is None \
+        and tracer.precision != "mixed"
+    image = getattr(tracer.geometry, "image_profile", None) is not None
+    if tracer.surrogate is None and not binned:
         for T in (64, 128, 256):
-            if biggest <= T and not (tracer.precision == "mixed" and T != 256):
-                out.append(dict(mode="persistent", block_threads=T, field_shape=None))
+            if biggest > T or (tracer.precision == "mixed" and T != 256) or (image and T != 256):
+                continue
+            if image and tracer.precision == "mixed":
+                continue
+            out.append(dict(mode="persistent", block_threads=T, field_shape=None))
     pb0 = _auto_points_per_block(n, n_cu)
+    pbs = [64] if binned else sorted({max(1, pb0 // 4), max(1, pb0 // 2), pb0, min(64, pb0 * 2), min(64, pb0 * 4)})
     for T in (192, 256):
-        for pb in sorted({max(1, pb0 // 4), max(1, pb0 // 2), pb0, min(64, pb0 * 2), min(64, pb0 * 4)}):
+        for pb in pbs:
             out.append(dict(mode="lockstep", block_threads=0, field_shape=(T, pb)))
     return out
 

@@ -228,28 +228,64 @@ class SpecificParticleTracer:
             self._field_order = torch.argsort(best, stable=True).to(torch.int32).contiguous()
         return self._field_order
 
+    def _device_inputs(self, ctx):
+        """The trace's inputs in device memory, uploaded once per tracer and
+        device: initial state, birth times, charges, masses, group offsets
+        and queue order, output times."""
+        import torch
+
+        dev = ctx.torch_device
+        key = str(dev)
+        cache = getattr(self, "_dev_in", None)
+        if cache is not None and cache[0] == key:
+            return cache[1]
+
+        def up(a, dt=torch.float64):
+            return torch.as_tensor(a, device=dev).to(dt).contiguous()
+
+        outs = self.output_times_host
+        d = dict(pos0=up(self.pos0), vel0=up(self.vel0), tb=up(self.t_birth), q=up(self.charge),
+                 m=up(self.particle_mass), offs=up(self.offsets, torch.int64),
+                 qorder=up(self._queue_order(), torch.int32),
+                 t_out=up(outs) if outs is not None and len(outs) else None,
+                 stage_weight=up(N_STAGES * np.diff(self.offsets), torch.int64))
+        self._dev_in = (key, d)
+        return d
+
     def trace(self, event_capacity=None, no_alloc=False):
         """Run spt_trace; returns the raw device results as NumPy arrays.
         no_alloc: call spt_trace_noalloc instead (persistent mode, the
         library's static device scratch, at most 155,344 particles)."""
+        return self.to_host(self.trace_device(event_capacity=event_capacity, no_alloc=no_alloc))
+
+    @staticmethod
+    def to_host(out):
+        """trace_device()'s results as NumPy arrays (events and snapshots as
+        structured arrays of _lib.EVENT_DTYPE)."""
+        res = {k: out[k].cpu().numpy() for k in ("pos", "vel", "t_death", "attempts", "accepted", "active_stages",
+                                                   "margin", "status", "t_end")}
+        res.update(events=np.frombuffer(out["events"].cpu().numpy().tobytes(), dtype=_lib.EVENT_DTYPE),
+                   snapshots=np.frombuffer(out["snapshots"].cpu().numpy().tobytes(), dtype=_lib.EVENT_DTYPE),
+                   device_ms=out["device_ms"], particle_stages=out["particle_stages"])
+        return res
+
+    def trace_device(self, event_capacity=None, no_alloc=False):
+        """spt_trace from the device-resident inputs (uploaded once per
+        tracer); the results stay in device memory: torch tensors (the
+        events and snapshots as raw records of _lib.EVENT_DTYPE bytes, the
+        first event_count / snapshot_count of them), plus device_ms and
+        particle_stages."""
         import torch
 
         ctx = self.geometry._ctx()
         dev = ctx.torch_device
         n = self.pos0.shape[0]
         G = self.n_groups
-
-        def up(a, dt=torch.float64):
-            return torch.as_tensor(a, device=dev).to(dt).contiguous()
-
-        pos = up(self.pos0)
-        vel = up(self.vel0)
-        tb = up(self.t_birth)
-        q = up(self.charge)
-        m = up(self.particle_mass)
+        d = self._device_inputs(ctx)
+        pos = d["pos0"].clone()
+        vel = d["vel0"].clone()
+        tb, q, m, offs, qorder, t_out = d["tb"], d["q"], d["m"], d["offs"], d["qorder"], d["t_out"]
         td = torch.empty(n, dtype=torch.float64, device=dev)
-        offs = up(self.offsets, torch.int64)
-        qorder = up(self._queue_order(), torch.int32)
         attempts = torch.zeros(G, dtype=torch.int64, device=dev)
         accepted = torch.zeros(G, dtype=torch.int64, device=dev)
         act = torch.zeros(G, dtype=torch.int64, device=dev)
@@ -260,9 +296,7 @@ class SpecificParticleTracer:
         cap = event_capacity if event_capacity is not None else max(1024, 4 * n * max(ns, 1))
         events = torch.empty((cap, _lib.EVENT_DTYPE.itemsize), dtype=torch.uint8, device=dev)
         ev_count = torch.zeros(1, dtype=torch.int64, device=dev)
-        outs = self.output_times_host
-        n_out = 0 if outs is None else len(outs)
-        t_out = up(outs) if n_out else None
+        n_out = 0 if t_out is None else int(t_out.shape[0])
         snap_cap = n * n_out  # every particle at most once per output time
         snaps = torch.empty((max(snap_cap, 1), _lib.EVENT_DTYPE.itemsize), dtype=torch.uint8, device=dev)
         snap_count = torch.zeros(1, dtype=torch.int64, device=dev)
@@ -333,20 +367,13 @@ class SpecificParticleTracer:
         t1.record()
         ctx.sync()
         torch.cuda.synchronize(dev)
-        n_ev = int(ev_count.item())
+        counts = torch.stack([ev_count[0], snap_count[0], torch.sum(attempts * d["stage_weight"])]).cpu()
+        n_ev, n_sn, stages = (int(v) for v in counts)
         if n_ev > cap:
-            return self.trace(event_capacity=n_ev, no_alloc=no_alloc)
-        ev = np.frombuffer(events[:n_ev].cpu().numpy().tobytes(), dtype=_lib.EVENT_DTYPE)
-        n_sn = min(int(snap_count.item()), snap_cap)
-        snapshots = np.frombuffer(snaps[:n_sn].cpu().numpy().tobytes(), dtype=_lib.EVENT_DTYPE)
-        out = dict(pos=pos.cpu().numpy(), vel=vel.cpu().numpy(), t_death=td.cpu().numpy(),
-                   attempts=attempts.cpu().numpy(), accepted=accepted.cpu().numpy(),
-                   active_stages=act.cpu().numpy(), margin=margin.cpu().numpy(), status=status.cpu().numpy(),
-                   t_end=t_end.cpu().numpy(),
-                   events=ev, snapshots=snapshots, device_ms=t0.elapsed_time(t1))
-        sizes = np.diff(self.offsets)
-        out["particle_stages"] = int(N_STAGES * np.sum(sizes * out["attempts"]))
-        return out
+            return self.trace_device(event_capacity=n_ev, no_alloc=no_alloc)
+        return dict(pos=pos, vel=vel, t_death=td, attempts=attempts, accepted=accepted, active_stages=act,
+                    margin=margin, status=status, t_end=t_end, events=events[:n_ev],
+                    snapshots=snaps[:min(n_sn, snap_cap)], device_ms=t0.elapsed_time(t1), particle_stages=stages)
 
     def run(self, verbose=False):
         """Reference tracker.py:312-386: (screens, trajectories) as lists of

@@ -82,15 +82,34 @@ def make_tracer(wl, geo, surrogate=None, precision="pairs32", mode="lockstep", f
 
 
 def timed_traces(tr, steps, warmup):
+    """K traces with the inputs resident in device memory (uploaded by the
+    tracer once) and the results left there (trace_device), timed between
+    two device synchronisations; the results come back to the host after
+    the timed region."""
     import torch
 
+    tr._device_inputs(tr.geometry._ctx())
     for _ in range(warmup):
-        tr.trace()
+        tr.trace_device()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    outs = [tr.trace() for _ in range(steps)]
+    outs = [tr.trace_device() for _ in range(steps)]
     torch.cuda.synchronize()
-    return outs, time.perf_counter() - t0
+    wall = time.perf_counter() - t0
+    return [tr.to_host(o) for o in outs], wall
+
+
+def host_io_trace(tr):
+    """One trace through the host boundary (host arrays in, host arrays
+    out: uploads and downloads over PCIe included), after one warm-up."""
+    import torch
+
+    tr.trace()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    o = tr.trace()
+    torch.cuda.synchronize()
+    return o, time.perf_counter() - t0
 
 
 def screen_time_diff(a, b):
@@ -476,6 +495,11 @@ def main():
     plain_outs, plain_wall = timed_traces(plain_tr, 1, 0)
     plain_value = plain_outs[0]["particle_stages"] / plain_wall
     say(f"bench: plain trace {plain_wall:.2f} s")
+    # the same fast trace through the host boundary (ParticleGroup arrays in,
+    # NumPy results out): the PCIe-inclusive rate, reported beside value
+    oh, wh = host_io_trace(tr)
+    host_io = dict(value=oh["particle_stages"] / wh, wall_s=wh)
+    say(f"bench: fast trace with host arrays in and out {wh:.3f} s")
     # the value counts only if the fast trace reproduces the plain one: every
     # electron's screen crossing, at times within the tolerance
     n_el = wl["n_el"]
@@ -496,7 +520,7 @@ def main():
     # gather is for traces without it): the network's gain is measured
     # against the fast form gathered the same way
     t3u = make_tracer(wl, geo, **bud.tracer_kwargs())
-    t3u.field_shape = tr.field_shape
+    t3u.field_shape = (tr.field_shape[0], 64) if tr.field_shape else None
     t3u.binned = False
     o3u, w3u = timed_traces(t3u, 1, 0)
     net_off["unbinned"] = dict(value=o3u[0]["particle_stages"] / w3u, wall_s=w3u,
@@ -513,7 +537,7 @@ def main():
                 sweep.append(dict(tol=tol, band_m=None))
                 continue
             t2 = make_tracer(wl, geo, surrogate=s, **bud.tracer_kwargs())
-            t2.field_shape = tr.field_shape
+            t2.field_shape = t3u.field_shape
             o, w = timed_traces(t2, 1, 0)
             t4 = make_tracer(wl, geo, surrogate=s)
             t4.field_shape = plain_tr.field_shape
@@ -587,6 +611,7 @@ def main():
                            "FP64 field and step control, FP32 + Kahan pair forces",
                    "n_quad": n_quad, "field_shape": list(fast_shape), "plain_field_shape": list(plain_shape)},
         "value_check": value_check,
+        "value_host_io": host_io,
         "plain_value": plain_value,
         "gain_vs_plain": value / plain_value,
         "budget": bud.report,
