@@ -1808,11 +1808,39 @@ __global__ void ls_pgroup_kernel(const int64_t* offsets, int n_groups, int32_t* 
 
 #define LKB(sl) L.T.kbuf[(size_t)(sl) * (size_t)L.T.n_total + (size_t)gi]
 
+// Sum and minimum over one group's lanes: the block (GPB = 1) or, for four
+// groups per 64-lane block, the group's 16-lane segment (lane i takes lane
+// i + s, s = 8 .. 1: with zero lanes beyond the group, bit-identical to
+// block_sum<64>'s s = 32 .. 1 tree, whose first two levels add zeros).
+template <int TG, int GPB>
+__device__ __forceinline__ double group_sum(double v, double* red) {
+  if constexpr (GPB == 1) {
+    return block_sum<TG>(v, red);
+  } else {
+    static_assert(TG == 64 && GPB == 4, "four 16-lane groups per wavefront");
+#pragma unroll
+    for (int s = 8; s > 0; s >>= 1) v = v + __shfl_down(v, s, 16);
+    return __shfl(v, 0, 16);
+  }
+}
+template <int TG, int GPB>
+__device__ __forceinline__ double group_min(double v, double* red) {
+  if constexpr (GPB == 1) {
+    return block_min<TG>(v, red);
+  } else {
+    static_assert(TG == 64 && GPB == 4, "four 16-lane groups per wavefront");
+#pragma unroll
+    for (int s = 8; s > 0; s >>= 1) v = fmin(v, __shfl_down(v, s, 16));
+    return __shfl(v, 0, 16);
+  }
+}
+
 // Start an attempt for group g at (t, h): birth checkpoint, clipped step,
 // frozen active set, stage-0 state (reference batched_rk45.py:223-249,348).
-template <int TG>
-__device__ void ls_begin(const LockParams& L, int g, int n, double t, double h, int64_t attempts, double* red) {
-  const int tid = threadIdx.x;
+// tid: the thread's member index within the group.
+template <int TG, int GPB = 1>
+__device__ void ls_begin(const LockParams& L, int g, int n, int tid, double t, double h, int64_t attempts,
+                         double* red) {
   const bool mine = tid < n;
   const int64_t gi = L.T.offsets[g] + tid;
   if (attempts >= L.T.max_steps) {
@@ -1824,8 +1852,8 @@ __device__ void ls_begin(const LockParams& L, int g, int n, double t, double h, 
     return;
   }
   const double tb = mine ? L.T.t_birth[gi] : INFINITY;
-  const double n_reached = block_sum<TG>((mine && tb <= t) ? 1.0 : 0.0, red);
-  const double nxt_cp = block_min<TG>((mine && tb > t) ? tb : INFINITY, red);
+  const double n_reached = group_sum<TG, GPB>((mine && tb <= t) ? 1.0 : 0.0, red);
+  const double nxt_cp = group_min<TG, GPB>((mine && tb > t) ? tb : INFINITY, red);
   const bool exhausted = n_reached >= (double)n;
   const double next = exhausted ? L.T.t_max : nxt_cp;
   const int optr = outputs_reached(L.T, t);
@@ -1833,7 +1861,7 @@ __device__ void ls_begin(const LockParams& L, int g, int n, double t, double h, 
   const double target = fmin(fmin(next, L.T.t_max), nout);
   const double hr = fmin(h, target - t);
   const bool active = mine && (t >= tb) && (t < L.T.t_death[gi]);
-  const double n_act = block_sum<TG>(active ? 1.0 : 0.0, red);
+  const double n_act = group_sum<TG, GPB>(active ? 1.0 : 0.0, red);
   if (mine) {
     L.active[gi] = active ? 1 : 0;
     for (int c = 0; c < 3; ++c) {
@@ -1877,7 +1905,7 @@ __global__ __launch_bounds__(TG) void ls_init_kernel(LockParams L) {
     L.T.status[g] = SPT_STATUS_DONE;
   }
   __syncthreads();
-  ls_begin<TG>(L, g, n, t, h, 0, red);
+  ls_begin<TG>(L, g, n, tid, t, h, 0, red);
 }
 
 // Keep a freshly evaluated stage field (slots 51..53): stage 1's as the
@@ -2327,18 +2355,22 @@ __global__ __launch_bounds__(TG) void ls_accel_kernel(LockParams L, int st) {
 
 // Error norm, accept/reject, commit, birth reset, kill, screens (reference
 // batched_rk45.py:250-317, 372-399), then the group's next attempt.
-template <int TG>
+template <int TG, int GPB = 1>
 __global__ __launch_bounds__(TG) void ls_end_kernel(LockParams L) {
+  constexpr int W = TG / GPB;  // lanes per group
   __shared__ double red[TG];
-  const int g = blockIdx.x;
-  const int flags = L.g_flags[g];
-  if (!(flags & LS_RUNNING)) return;
-  const int tid = threadIdx.x;
-  const int64_t off = L.T.offsets[g];
-  const int n = (int)(L.T.offsets[g + 1] - off);
-  const bool mine = tid < n;
+  const int gs = threadIdx.x / W;
+  const int g = blockIdx.x * GPB + gs;
+  const int flags = g < L.T.n_groups ? L.g_flags[g] : 0;
+  const bool live = flags & LS_RUNNING;
+  if (GPB == 1 && !live) return;  // block-uniform
+  if (GPB > 1 && !__syncthreads_or(live)) return;
+  const int tid = threadIdx.x - gs * W;
+  const int64_t off = live ? L.T.offsets[g] : 0;
+  const int n = live ? (int)(L.T.offsets[g + 1] - off) : 1;
+  const bool mine = live && tid < n;
   const int64_t gi = off + tid;
-  const double hr = L.g_hr[g];
+  const double hr = live ? L.g_hr[g] : 0.0;
   double lp = 0.0, lv = 0.0;
   if (mine) {
     for (int c = 0; c < 3; ++c) {
@@ -2356,7 +2388,8 @@ __global__ __launch_bounds__(TG) void ls_end_kernel(LockParams L) {
       lv += b * b;
     }
   }
-  const double sq = block_sum<TG>(lp, red) + block_sum<TG>(lv, red);
+  const double sq = group_sum<TG, GPB>(lp, red) + group_sum<TG, GPB>(lv, red);
+  if (GPB > 1 && !live) return;  // lanes of a finished group: no barrier follows in the packed form
   double err = sqrt(sq / (6.0 * n));
   if (err == 0.0) err = 2.2250738585072014e-308;
   const bool acc = err <= 1.0;
@@ -2396,7 +2429,7 @@ __global__ __launch_bounds__(TG) void ls_end_kernel(LockParams L) {
   }
   const bool stalled = step_stalled(acc, h, t);
   if (stalled) running = false;
-  __syncthreads();
+  if (GPB == 1) __syncthreads();
   if (tid == 0) {
     L.T.attempts[g] = batch + 1;
     if (acc) L.T.accepted[g] += 1;
@@ -2412,7 +2445,7 @@ __global__ __launch_bounds__(TG) void ls_end_kernel(LockParams L) {
     }
   }
   if (!running) return;
-  ls_begin<TG>(L, g, n, t, h, batch + 1, red);
+  ls_begin<TG, GPB>(L, g, n, tid, t, h, batch + 1, red);
 }
 #undef LKB
 
@@ -3375,7 +3408,10 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
         else
           ls_accel_kernel<TG, APR><<<n_groups, TG, 0, s>>>(L, st);
       }
-      ls_end_kernel<TG><<<n_groups, TG, 0, s>>>(L);
+      if (pack4)
+        ls_end_kernel<64, 4><<<(unsigned)((n_groups + 3) / 4), 64, 0, s>>>(L);
+      else
+        ls_end_kernel<TG><<<n_groups, TG, 0, s>>>(L);
     }
     return SPT_OK;
   };
