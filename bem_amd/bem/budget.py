@@ -39,21 +39,27 @@ class FieldBudget:
                     multipole=None if self.multipole is None or self.multipole.inert else self.multipole)
 
 
-def fit(geometry, tol=1e-6, far_rule=(2.0, 16.0), near_width=2.0, seg_per_cluster=35, kappa=1.35,
-        multipole_terms=128, multipole_tol=None, n_val=1 << 16, use_clusters=True, use_multipole=True):
+def fit(geometry, tol=1e-6, far_rule=(2.0, 16.0), near_width=2.0, seg_per_cluster=50, kappa=1.2,
+        multipole_terms=128, multipole_tol=None, n_val=1 << 16, use_clusters=True, use_multipole=True,
+        cluster_terms=128):
     """Build and validate a budget for a protrusion BEM geometry (the post or
     the hemispherical tip). Raises ValueError when the validated far-rule
     gather (far rule + near width + clusters) misses tol.
 
-    The defaults come from a sweep on the 400-panel post's 262,144-electron
-    trace (far-rule thresholds, near width, cluster size and kappa against
-    the validated error and the trace time): clusters of 35 segments
-    expanded beyond 1.35 r_c, near width 2, far rule (2, 16): field error
-    3.0e-8 of |E|, 130.6 ms per trace (25 segments beyond 1.6 r_c: 143.7
-    ms; near width 2.5: 6.6e-8 and 1.4 % faster; far rule (2, 10): 1.7e-7)."""
+    The defaults come from sweeps on the 400-panel post's 262,144-electron
+    binned trace (far-rule thresholds, near width, cluster size, kappa and
+    expansion length against the validated error and the trace time):
+    clusters of up to 50 segments within each straight or turning run of
+    the profile (the cap is one), each about its smallest enclosing sphere,
+    expanded beyond 1.2 r_c with up to 128 terms; near width 2; far rule
+    (2, 16): field error 1.8e-8 of |E|, 86 ms per trace (35 segments beyond
+    1.35 r_c about their z-midpoints: 92 ms; kappa 1.1 .. 1.15 or 70 .. 100
+    segments: 88 .. 90 ms; near width 2.5: 6.6e-8; far rule (2, 10):
+    1.7e-7)."""
     from ..surrogate import far_rule_error
 
-    cl = _clusters.fit(geometry, seg_per_cluster=seg_per_cluster, kappa=kappa) if use_clusters else None
+    cl = (_clusters.fit(geometry, seg_per_cluster=seg_per_cluster, n_terms=cluster_terms, kappa=kappa)
+          if use_clusters else None)
     err = far_rule_error(geometry, *far_rule, n=n_val, near_width=near_width, clusters=cl)
     if not err["field_err_max"] <= tol:
         raise ValueError(f"the far-rule gather misses the field tolerance {tol:g}: {err['field_err_max']:.3e}")

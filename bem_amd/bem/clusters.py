@@ -61,9 +61,57 @@ class Clusters:
         return self.moments.shape[1]
 
 
-def build(profile, sigma, n_subdiv=8, seg_per_cluster=25, n_terms=64, kappa=1.6, tol=1e-11):
-    """Clusters of seg_per_cluster consecutive segments (the last one takes
-    the remainder) of the fixed-rule node table."""
+def _regions(prof):
+    """Runs of consecutive segments that are all straight-continued (the
+    same direction as their predecessor, to 1e-9 rad) or all turning:
+    (start, end) pairs covering the profile."""
+    t = np.diff(prof, axis=0)
+    ang = np.arctan2(t[:, 1], t[:, 0])
+    turn = np.abs(np.angle(np.exp(1j * np.diff(ang)))) > 1e-9  # segment j + 1 turns from segment j
+    kind = np.concatenate([[turn[0] if len(turn) else False], turn])
+    out, a = [], 0
+    for j in range(1, len(kind)):
+        if kind[j] != kind[a]:
+            out.append((a, j))
+            a = j
+    out.append((a, len(kind)))
+    return out
+
+
+def boundaries(profile, seg_per_cluster, split_regions=True):
+    """Cluster boundaries: runs of seg_per_cluster segments (a short last run
+    joins the one before), within each straight or turning region of the
+    profile when split_regions (so the hemispherical cap of a post is a
+    cluster of its own, whose enclosing sphere is the cap's)."""
+    prof = np.asarray(profile, dtype=np.float64)
+    n_seg = len(prof) - 1
+    regions = _regions(prof) if split_regions else [(0, n_seg)]
+    bounds = [0]
+    for a, b in regions:
+        cuts = list(range(a, b, seg_per_cluster)) + [b]
+        if len(cuts) > 2 and cuts[-1] - cuts[-2] < seg_per_cluster // 2:
+            cuts.pop(-2)  # no runt cluster at the end of a region
+        bounds += cuts[1:]
+    return bounds
+
+
+def enclosing_centre(r, z):
+    """The axis point (0, z_c) whose sphere through the farthest of the
+    points (r, z) is smallest: minimises the convex max_q r_q^2 + (z_q -
+    z_c)^2 by ternary search."""
+    lo, hi = float(z.min()), float(z.max())
+    for _ in range(200):
+        m1, m2 = lo + (hi - lo) / 3.0, hi - (hi - lo) / 3.0
+        if np.max(r * r + (z - m1) ** 2) < np.max(r * r + (z - m2) ** 2):
+            hi = m2
+        else:
+            lo = m1
+    return 0.5 * (lo + hi)
+
+
+def build(profile, sigma, n_subdiv=8, seg_per_cluster=25, n_terms=128, kappa=1.6, tol=1e-11, split_regions=True):
+    """Clusters of the fixed-rule node table (see boundaries()), each about
+    the axis point of its smallest enclosing sphere."""
     prof = np.asarray(profile, dtype=np.float64)
     n_seg = len(prof) - 1
     if n_seg < 1 or seg_per_cluster < 1:
@@ -72,14 +120,12 @@ def build(profile, sigma, n_subdiv=8, seg_per_cluster=25, n_terms=64, kappa=1.6,
         raise ValueError("kappa must exceed 1 (the expansion converges outside r_c)")
     rho, z, w = quadrature_nodes(prof, sigma, n_subdiv)
     nq = 4 * n_subdiv
-    bounds = list(range(0, n_seg, seg_per_cluster)) + [n_seg]
-    if len(bounds) > 2 and bounds[-1] - bounds[-2] < seg_per_cluster // 2:
-        bounds.pop(-2)  # no runt cluster at the end
+    bounds = boundaries(prof, seg_per_cluster, split_regions)
     zc, rc, mom = [], [], []
     for a, b in zip(bounds[:-1], bounds[1:]):
         sl = slice(a * nq, b * nq)
         r_, z_, w_ = rho[sl], z[sl], w[sl]
-        c = 0.5 * (z_.min() + z_.max())
+        c = enclosing_centre(r_, z_)
         d = np.hypot(r_, z_ - c)
         R = float(d.max()) * (1.0 + 1e-12)
         cos = np.divide(z_ - c, d, out=np.ones_like(d), where=d > 0)
@@ -95,11 +141,11 @@ def build(profile, sigma, n_subdiv=8, seg_per_cluster=25, n_terms=64, kappa=1.6,
     return Clusters(bounds, zc, rc, np.array(mom), kappa, tol)
 
 
-def fit(geometry, seg_per_cluster=25, n_terms=64, kappa=1.6, tol=1e-11):
+def fit(geometry, seg_per_cluster=25, n_terms=128, kappa=1.6, tol=1e-11, split_regions=True):
     """Clusters of a BEM geometry's uploaded table (its profile, sigma and
     n_subdiv)."""
     prof, sig, nsub = geometry.bem_table()
-    cl = build(prof, sig, nsub, seg_per_cluster, n_terms, kappa, tol)
+    cl = build(prof, sig, nsub, seg_per_cluster, n_terms, kappa, tol, split_regions)
     cl.geometry = geometry
     return cl
 

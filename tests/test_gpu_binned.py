@@ -47,7 +47,7 @@ def test_far_gather_is_order_independent(post, shape):
     from bem_amd.bem import clusters as bcl
     from bem_amd.bem.multipole import _pert_field
 
-    cl = bcl.fit(post, seg_per_cluster=35, kappa=1.35)
+    cl = bcl.fit(post, seg_per_cluster=50, kappa=1.2)
     rng = np.random.default_rng(11)
     n = 64 * 96  # whole blocks at either shape, so every block has the same slice count
     th = np.deg2rad(60.0) * rng.random(n)
