@@ -33,7 +33,7 @@ EXPORTS = (
     "spt_mlp_train", "spt_mlp_infer", "spt_surrogate_set", "spt_toroidal_q", "spt_ring_modes",
     "spt_image_upload", "spt_image_force", "spt_set_field_shape", "spt_bem_far_rule",
     "spt_bem_multipole", "spt_trace_noalloc", "spt_bem_clusters", "spt_bem_near_width",
-    "spt_set_lockstep_binning",
+    "spt_set_lockstep_binning", "spt_bem_tiles", "spt_bem_tiles_enable", "spt_bem_remainder",
 )
 MLP_FOURIER, MLP_WIDTH, MLP_DEPTH = 32, 128, 4
 
@@ -123,6 +123,13 @@ def load():
             return _lib
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build() (hipcc, gfx950)")
+        # torch first: its HIP runtime (soname libamdhip64.so.7, like the
+        # system one the library links) is then the one the library binds
+        # to, so both work on the same device memory and streams; loaded the
+        # other way round the two runtimes clash and the library sees no
+        # device
+        import torch  # noqa: F401
+
         lib = ctypes.CDLL(LIB_PATH)
         P = ctypes.POINTER
         sig = {
@@ -138,6 +145,10 @@ def load():
             "spt_bem_clusters": (ctypes.c_int, [_p, _p, ctypes.c_int32, _p, _p, _p, ctypes.c_int32, ctypes.c_double,
                                                 ctypes.c_double]),
             "spt_bem_near_width": (ctypes.c_int, [_p, ctypes.c_double]),
+            "spt_bem_tiles": (ctypes.c_int, [_p, _p, ctypes.c_int32, _p, _p, ctypes.c_int32, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_double]),
+            "spt_bem_tiles_enable": (ctypes.c_int, [_p, ctypes.c_int32]),
+            "spt_bem_remainder": (ctypes.c_int, [_p, _p, _p, ctypes.c_int64, _p]),
             "spt_bem_assemble": (ctypes.c_int, [_p, _p, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _p]),
             "spt_bem_solve": (ctypes.c_int, [_p, _p, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_int32, _p]),
@@ -194,6 +205,7 @@ class Context:
             raise RuntimeError(f"spt_create failed ({rc})")
         self.h = h
         self.bem_key = None
+        self.tiles_key = None  # (tiles token, bem_key) of the uploaded tiles
 
     @classmethod
     def get(cls, device=0):

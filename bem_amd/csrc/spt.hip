@@ -108,7 +108,24 @@ struct BemView {
   int cl_n, cl_terms;
   double cl_tol;
   double near_w;            // near-rule interval width of far-rule gathers (the reference's: 0.5)
+  // Chebyshev tiles (far-rule gathers only; tl_tree nullptr: off): a
+  // quadtree over the square [tl_x0, tl_x0 + tl_size) x [tl_z0, tl_z0 +
+  // tl_size) of (rho, z). tl_tree[k] >= 0: index of node k's first child
+  // (children in the order low/high rho, then low/high z); -1: no tile;
+  // <= -2: tile -2 - tl_tree[k]. A point in tile t sums its window's
+  // segments tl_win[t].x .. tl_win[t].y - 1 with the far-rule gather's
+  // rules and takes every other segment's fixed-rule field from the tile's
+  // table: kTileP x kTileP Chebyshev coefficients of E_rho, then of E_z,
+  // over the cell (tl_coef + t * kTileCoef).
+  const int* tl_tree;
+  const int2* tl_win;
+  const double* tl_coef;
+  double tl_x0, tl_z0, tl_size;
 };
+constexpr int kTileP = 12;                         // Chebyshev degree + 1 per direction
+constexpr int kTileCoef = 2 * kTileP * kTileP;     // doubles per tile
+constexpr int kTileMaxDepth = 24;
+constexpr int kTileMaxTiles = 6144;
 constexpr int kCoarsePer = 14;
 constexpr int kMultipoleMaxTerms = 1024;
 constexpr double kMultipoleTail = 1e-12;  // multipole_pert's truncation bound (relative to max |m_l|)
@@ -479,6 +496,131 @@ __device__ __forceinline__ void cluster_expansion(const BemView& B, int c, doubl
   acc.ez_ax += er * cs - et * sn;
 }
 
+// Tile holding (rho, z), or -1 (outside the quadtree, or a cell without a
+// tile); (cx, cz) and h: the cell's centre and half-size. The halving
+// arithmetic is the table builder's (bem_amd.bem.tiles), so both place a
+// point in the same cell.
+__device__ __forceinline__ int tile_find(const BemView& B, double rho, double z, double& cx, double& cz, double& h) {
+  double x0 = B.tl_x0, z0 = B.tl_z0, s = B.tl_size;
+  if (!(rho >= x0 && rho < x0 + s && z >= z0 && z < z0 + s)) return -1;
+  int k = 0;
+  for (int depth = 0; depth <= kTileMaxDepth; ++depth) {
+    const int v = B.tl_tree[k];
+    if (v < 0) {
+      h = 0.5 * s;
+      cx = x0 + h;
+      cz = z0 + h;
+      return v <= -2 ? -2 - v : -1;
+    }
+    s *= 0.5;
+    const int qx = rho >= x0 + s ? 1 : 0, qz = z >= z0 + s ? 1 : 0;
+    if (qx) x0 += s;
+    if (qz) z0 += s;
+    k = v + qx + 2 * qz;
+  }
+  return -1;
+}
+
+// Rows a = s, s + S, ... of tile t's Chebyshev sums at (u, v) in [-1, 1]^2
+// (T_0 = 1, T_1 = x, T_{k+1} = 2x T_k - T_{k-1}): sum_a T_a(u) sum_b
+// c[a][b] T_b(v) for E_rho and E_z, added to the accumulator's final-unit
+// terms. C: the coefficient pointer type (constant address space where the
+// tile is uniform over the wavefront, so the loads are scalar).
+template <class C>
+__device__ __forceinline__ void tile_rows(C c, double u, double v, int s, int S, RingAcc& acc) {
+  double tv[kTileP];
+  tv[0] = 1.0;
+  tv[1] = v;
+#pragma unroll
+  for (int b = 2; b < kTileP; ++b) tv[b] = fma(2.0 * v, tv[b - 1], -tv[b - 2]);
+  double ta = 1.0, tam = u;  // T_a(u) and T_{a-1}(u) (T_{-1} = T_1)
+  double er = 0.0, ez = 0.0;
+#pragma unroll 1
+  for (int a = 0; a < kTileP; ++a) {
+    if (a >= s && (a - s) % S == 0) {
+      double rr = 0.0, rz = 0.0;
+#pragma unroll
+      for (int b = 0; b < kTileP; ++b) {
+        rr = fma(c[a * kTileP + b], tv[b], rr);
+        rz = fma(c[kTileP * kTileP + a * kTileP + b], tv[b], rz);
+      }
+      er = fma(ta, rr, er);
+      ez = fma(ta, rz, ez);
+    }
+    const double tn = a == 0 ? u : fma(2.0 * u, ta, -tam);
+    tam = ta;
+    ta = tn;
+  }
+  acc.er_ax += er;
+  acc.ez_ax += ez;
+}
+
+__device__ __forceinline__ void tile_eval(const BemView& B, int t, double u, double v, int s, int S, RingAcc& acc) {
+  const int t0 = __builtin_amdgcn_readfirstlane(t);
+  if (__ballot(t != t0) == 0ull)
+    tile_rows((cseg_ptr)B.tl_coef + (size_t)t0 * kTileCoef, u, v, s, S, acc);
+  else
+    tile_rows(B.tl_coef + (size_t)t * kTileCoef, u, v, s, S, acc);
+}
+
+// Tile of a point whose tile has an empty window (its whole perturbation
+// comes from the table), or -1.
+__device__ __forceinline__ int tile_only(const BemView& B, double x, double y, double z, double& cx, double& cz,
+                                         double& h) {
+  if (!B.tl_tree) return -1;
+  const int t = tile_find(B, hypot(x, y), z, cx, cz, h);
+  if (t < 0) return -1;
+  const int2 w = B.tl_win[t];
+  return w.x == w.y ? t : -1;
+}
+
+// Perturbation at a point of a table-only tile t (tile_only), by one
+// thread: every row of the table, then the axis rule of the gather.
+__device__ __forceinline__ void tile_point(const BemView& B, int t, double x, double y, double z, double cx, double cz,
+                                           double h, double& ox, double& oy, double& oz) {
+  const double rho = hypot(x, y), ih = 1.0 / h;
+  RingAcc acc;
+  acc.zero();
+  tile_rows(B.tl_coef + (size_t)t * kTileCoef, (rho - cx) * ih, (z - cz) * ih, 0, 1, acc);
+  double er, ez;
+  ring_finish(acc, rho, er, ez);
+  const double phi = atan2(y, x);
+  ox = rho == 0.0 ? 0.0 : er * cos(phi);
+  oy = rho == 0.0 ? 0.0 : er * sin(phi);
+  oz = ez;
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+// f(j) for this lane's segments j = s (mod S), ascending, with tiles on: a
+// tile lane (tile) its window [jlo, jhi), any other lane every segment
+// outside its far clusters (far). The loop runs over the wavefront's hull
+// of these ranges and the lanes pick their own segments, so a lane's
+// sequence does not depend on its neighbours; with s wavefront-uniform
+// (wave-major) so is j, and the table reads in f are scalar loads.
+template <class F>
+__device__ __forceinline__ void tile_lane_segments(const BemView& B, bool on, bool tile, int jlo, int jhi,
+                                                   unsigned long long far, int s, int S, bool uniform, F&& f) {
+  int lo = on ? jlo : B.n_seg, hi = on ? jhi : 0;
+  lo = wave_min(lo);
+  hi = wave_max(hi);
+  if (uniform) {
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    hi = __builtin_amdgcn_readfirstlane(hi);
+  }
+  for (int j = lo + ((s - lo) % S + S) % S; j < hi; j += S) {
+    const bool mine = on && (tile ? (j >= jlo && j < jhi) : !(B.cl && ((far >> B.seg_cl[j]) & 1ull)));
+    if (mine) f(j);
+  }
+}
+
 // Squared distance from (rho, z) to segment g, the projection clamped to the
 // segment (reference bem/axisymmetric.py:341-389).
 __device__ __forceinline__ double seg_dist2(const double* g, double rho, double z) {
@@ -525,20 +667,27 @@ __device__ __forceinline__ int bin_key(int bseg, double bd2, double L, int nsk, 
 // depend on S, i.e. on the launch shape, in the last bits). m must be
 // block-uniform. Far-rule gathers pick each point's rule per lane, so a
 // point's result does not depend on the other points of its block. BIN:
-// the binned lockstep gather's form, always wave-major (lane = point, so
-// the result is independent of m too) with the points valid[p] != 0 of
-// 0..63, and each point's bin (bin_key) written to okey[p].
+// the binned lockstep gather's form, with the points valid[p] != 0 of
+// 0..NMAX-1 and each point's bin (bin_key, or key_tile0 + its tile)
+// written to okey[p]: for NMAX = 64 wave-major (lane = point, so the result
+// is independent of m too), for NMAX = 16 slice-major with S = T/16 slices
+// per point whatever the number of valid points (the few-records form:
+// lower latency per point). With tiles on (far-rule gathers), a point inside a tile searches
+// and sums only its tile's window and adds the tile's table: the window
+// holds every segment within max(alpha s, L) of the cell, so the closest
+// segment wherever the on-surface offset applies, and every segment the
+// near rule would take, are among them.
 template <int T, int NMAX, int PR, bool FAR, bool BIN = false>
 __device__ void bem_pert_block(const BemView& B, int m, const double* px, const double* py, const double* pz,
                                double* ox, double* oy, double* oz, BemShared<T, NMAX>& sh, const int* valid = nullptr,
-                               int* okey = nullptr, int key_nsk = 0, int key_shift = 0) {
+                               int* okey = nullptr, int key_nsk = 0, int key_shift = 0, int key_tile0 = 0) {
   static_assert(T % 64 == 0, "whole wavefronts");
-  static_assert(!BIN || (FAR && NMAX == 64), "the binned gather is a far-rule gather of 64 points");
+  static_assert(!BIN || (FAR && (NMAX == 64 || NMAX == 16)), "the binned gather is a far-rule gather of 64 or 16 points");
   if (m <= 0) return;
   fill_log_table<T>(sh.logt);  // read after the barrier that ends the closest-segment pass
   const int tid = threadIdx.x;
-  const bool wave_major = BIN || (m > 32 && m <= 64 && B.n_seg >= 4 * (T / 64));
-  const int S = wave_major ? T / 64 : (m >= T ? 1 : T / m);
+  const bool wave_major = BIN ? NMAX == 64 : (m > 32 && m <= 64 && B.n_seg >= 4 * (T / 64));
+  const int S = wave_major ? T / 64 : (BIN ? T / NMAX : (m >= T ? 1 : T / m));
   const int p = wave_major ? (tid & 63) : tid / S;
   const int s = wave_major ? (tid >> 6) : tid - p * S;
   const bool on = p < m && (!BIN || valid[p] != 0);
@@ -546,7 +695,42 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
   const int slot0 = wave_major ? p : p * S, sstep = wave_major ? 64 : 1;
   auto slot = [=](int k) { return slot0 + k * sstep; };
   const double nw = FAR ? B.near_w : kRefNearWidth;  // near-rule interval width
-  if (on) {
+  // Tiles: this thread's point's tile (-1: none), its window and cell.
+  const bool tiles = FAR && B.tl_tree != nullptr;
+  const int su = wave_major ? __builtin_amdgcn_readfirstlane(s) : s;
+  int tleaf = -1, jlo = 0, jhi = B.n_seg;
+  double tcx = 0.0, tcz = 0.0, th = 1.0;
+  if (tiles && on) {
+    tleaf = tile_find(B, hypot(px[p], py[p]), pz[p], tcx, tcz, th);
+    if (tleaf >= 0) {
+      const int2 w = B.tl_win[tleaf];
+      jlo = w.x;
+      jhi = w.y;
+    }
+  }
+  const bool tl = tleaf >= 0;
+  if (tiles) {
+    // all lanes together (tile_lane_segments reduces over the wavefront)
+    double rho = 0.0, z = 0.0;
+    if (on) {
+      rho = hypot(px[p], py[p]);
+      z = pz[p];
+    }
+    double bd2 = INFINITY;
+    int bseg = 0;
+    const unsigned long long farc = (on && !tl && B.cl) ? far_clusters(B, rho * rho, z) : 0ull;
+    tile_lane_segments(B, on, tl, jlo, jhi, farc, su, S, wave_major, [&](int j) {
+      const double d2 = seg_dist2(B.segs + (size_t)j * SEG_STRIDE, rho, z);
+      if (d2 < bd2) {
+        bd2 = d2;
+        bseg = j;
+      }
+    });
+    if (on) {
+      sh.u.a.bd2[tid] = bd2;
+      sh.u.a.bseg[tid] = bseg;
+    }
+  } else if (on) {
     const double x = px[p], y = py[p], z = pz[p];
     const double rho = hypot(x, y);
     double bd2 = INFINITY;
@@ -593,7 +777,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
     const double L = g[S_L];
     const double thr = 1e-9 * L;
     const double off = bd2 < thr * thr ? 1e-4 * L : 0.0;
-    if (BIN) okey[p] = bin_key(bseg, bd2, L, key_nsk, key_shift);
+    if (BIN) okey[p] = tl ? key_tile0 + tleaf : bin_key(bseg, bd2, L, key_nsk, key_shift);
     double re = rho + off * g[S_NR];
     if (rho == 0.0) re = 0.0;
     sh.re[p] = re;
@@ -610,14 +794,10 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
   acc32.zero();
   const double re = on ? sh.re[p] : 0.0, ze = on ? sh.ze[p] : 0.0, re2 = re * re;
   int n_near = 0;  // near segments handed to the point's cooperative pass
-  const unsigned long long farm = (FAR && B.cl && on) ? far_clusters(B, re2, ze) : 0ull;
-  if (on) {
-    // Many segments per slice: stride over whole segments from j0. In the
-    // wave-major layout j0 is made wavefront-uniform (readfirstlane), so
-    // the table reads below compile to scalar loads. Far-rule gathers skip
-    // the segments of far clusters (their expansions are added below).
-    auto whole_segments = [&](int j0) {
-      auto seg = [&](int j) {
+  const unsigned long long farm = (FAR && B.cl && on && !tl) ? far_clusters(B, re2, ze) : 0ull;
+  // One segment's contribution to this thread's point (its far nodes, or
+  // its near rule handed to the cooperative pass).
+  auto seg = [&](int j) {
         const cseg_ptr g = (cseg_ptr)B.segs + (size_t)j * SEG_STRIDE;
         const double L = g[S_L];
         double proj, dist;
@@ -676,7 +856,16 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
             }
           }
         }
-      };
+  };
+  if (tiles && B.n_seg >= 4 * S) {
+    // tile lanes: their window; the others: outside their far clusters
+    tile_lane_segments(B, on, tl, jlo, jhi, farm, su, S, wave_major, seg);
+  } else if (on) {
+    // Many segments per slice: stride over whole segments from j0. In the
+    // wave-major layout j0 is made wavefront-uniform (readfirstlane), so
+    // the table reads in seg compile to scalar loads. Far-rule gathers skip
+    // the segments of far clusters (their expansions are added below).
+    auto whole_segments = [&](int j0) {
       if (FAR)
         local_segments(B, farm, j0, S, seg);
       else
@@ -698,6 +887,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
         const int j = u / n_sub;
         const int sub = u - j * n_sub;
         if (FAR && B.cl && ((farm >> B.seg_cl[j]) & 1ull)) continue;
+        if (tl && (j < jlo || j >= jhi)) continue;  // outside the tile's window: in its table
         const double* g = B.segs + (size_t)j * SEG_STRIDE;
         const double L = g[S_L];
         double proj, dist;
@@ -743,6 +933,11 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
   if (FAR && B.cl && on) {
     for (int c = s; c < B.cl_n; c += S)
       if ((farm >> c) & 1ull) cluster_expansion(B, c, re, re2, ze, acc);
+  }
+  if (tl) {
+    // the tile's table at the (possibly offset) evaluation point
+    const double ih = 1.0 / th;
+    tile_eval(B, tleaf, (re - tcx) * ih, (ze - tcz) * ih, s, S, acc);
   }
   sh.u.n.cnt[tid] = n_near;
   {
@@ -1622,12 +1817,21 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void bem_pert_kernel(BemVie
   const int64_t base = (int64_t)blockIdx.x * PB;
   const int m_all = (int)min((int64_t)PB, n - base);
   const int tid = threadIdx.x;
-  // Points in the multipole zone (if any) are done here; the rest go to
-  // the gather in their original order.
+  // Points in the multipole zone or in a table-only tile (if any) are done
+  // here, as the binned lockstep gather does them; the rest go to the
+  // gather in their original order.
   if (tid < m_all) {
     const double x = pos[3 * (base + tid)], y = pos[3 * (base + tid) + 1], z = pos[3 * (base + tid) + 2];
     double qx, qy, qz;
-    const bool far = B.mp && multipole_pert(B, x, y, z, qx, qy, qz);
+    bool far = B.mp && multipole_pert(B, x, y, z, qx, qy, qz);
+    if (!far) {
+      double cx, cz, h;
+      const int t = tile_only(B, x, y, z, cx, cz, h);
+      if (t >= 0) {
+        tile_point(B, t, x, y, z, cx, cz, h, qx, qy, qz);
+        far = true;
+      }
+    }
     if (far) {
       E[3 * (base + tid)] = qx;
       E[3 * (base + tid) + 1] = qy;
@@ -1770,9 +1974,25 @@ struct BinView {
                      // multipole zone, -1 nothing)
   double4* rout;     // per slot: perturbation, bin (bits)
   int n_keys, nsk, shift;
+  int key_tile0;     // first tile bin (tile t: key_tile0 + t), after the distance-band bins
   int key_mp0;       // first multipole-zone bin (n_keys when the zone is off)
   int append_only;   // 1: no slots at the start of an attempt, every record through the append path
+  int32_t* glist;    // the slots of this stage's gather records, in no particular order
+  unsigned* gcount;  // per stage index 0..6: records in glist (kListOff: the list is off this attempt)
+  unsigned* list_on; // this attempt fills glist (set with the counts by the scan kernel)
+  unsigned* n_start; // eligible particles in gather bins at the attempt's start (the count kernel's)
+  int key_tab0;      // first bin after the gather bins (the table-only tiles' and multipole bins follow)
 };
+// At most this many gather records: the few-records form (16 points per
+// block, 16 threads each; chosen from the count alone, so results do not
+// depend on slot order).
+constexpr unsigned kFewRecords = 8192;
+// An attempt's records go into the compact list only when its gather bins
+// hold at most kListFrom slots at its start (the point mix changes little
+// within an attempt); otherwise every stage's count is kListOff and the
+// field kernel takes the slot-order form.
+constexpr unsigned kListFrom = kFewRecords / 2;
+constexpr unsigned kListOff = 0x40000000u;
 constexpr int kBinMpKeys = 64;  // multipole-zone bins: 16 per octave of r^2 beyond r_min, open-ended
 
 struct LockParams {
@@ -1929,6 +2149,14 @@ __global__ __launch_bounds__(256) void ls_bin_key_kernel(LockParams L) {
   const BemView& B = L.T.bem;
   const double x = LKB(48), y = LKB(49), z = LKB(50);
   const double rho = hypot(x, y);
+  if (B.tl_tree) {
+    double cx, cz, h;
+    const int t = tile_find(B, rho, z, cx, cz, h);
+    if (t >= 0) {
+      L.X.key[gi] = L.X.key_tile0 + t;
+      return;
+    }
+  }
   double bd2 = INFINITY;
   int bseg = 0;
   for (int j = 0; j < B.n_seg; ++j) {
@@ -1958,18 +2186,23 @@ __device__ __forceinline__ int bin_key_mp(const BinView& X, const BemView& B, do
 // per bin and block).
 __global__ __launch_bounds__(kBinThreads) void ls_bin_count_kernel(LockParams L) {
   __shared__ unsigned cnt[kBinMaxKeys];
+  __shared__ unsigned n_gather;  // eligible particles whose bin is a gather bin
   const BinView& X = L.X;
   for (int k = threadIdx.x; k < X.n_keys; k += kBinThreads) cnt[k] = 0u;
+  if (threadIdx.x == 0) n_gather = 0u;
   __syncthreads();
   const int64_t gi = (int64_t)blockIdx.x * kBinThreads + threadIdx.x;
   if (gi < L.T.n_total) {
-    const bool e = !X.append_only && bin_eligible(L, gi, LKB(48), LKB(49), LKB(50));
+    const bool el = bin_eligible(L, gi, LKB(48), LKB(49), LKB(50));
+    const bool e = !X.append_only && el;
     X.slot[gi] = e ? -2 : -1;
     if (e) atomicAdd(&cnt[X.key[gi]], 1u);
+    if (el && X.key[gi] < X.key_tab0) atomicAdd(&n_gather, 1u);
   }
   __syncthreads();
   for (int k = threadIdx.x; k < X.n_keys; k += kBinThreads)
     if (cnt[k]) atomicAdd(&X.hist[k], cnt[k]);
+  if (threadIdx.x == 0 && n_gather) atomicAdd(X.n_start, n_gather);
 }
 
 // First slot of every bin (exclusive prefix sum of the counts, one block),
@@ -1998,24 +2231,40 @@ __global__ __launch_bounds__(kBinThreads) void ls_bin_scan_kernel(LockParams L) 
     X.hist[k] = 0u;
   }
   if (tid == kBinThreads - 1) *X.total = part[tid];
+  if (tid == 0) {
+    // whether this attempt's stages fill the compact list: from the count of
+    // particles in gather bins (the same with slots or the append path)
+    const bool on = *X.n_start <= kListFrom;
+    *X.n_start = 0u;
+    *X.list_on = on ? 1u : 0u;
+    for (int k = 0; k < 7; ++k) X.gcount[k] = on ? 0u : kListOff;
+  }
 }
+
+// Slot record tags (the fourth word's bits): >= 0 the gather (the
+// particle's index), else one of these.
+constexpr long long kRecNone = -1;       // no field needed this stage
+constexpr long long kRecMultipole = -2;  // multipole zone, one record per thread
+constexpr long long kRecTable = -3;      // a table-only tile, one record per thread
 
 // Stage st's field for a particle of a running group whose stage position
 // (x, y, z) is in slots 48..50: written here where the BEM perturbation is
 // not needed (inactive, the state's known field at stage 0, outside the BEM
 // region), else the position goes into the particle's slot record (a slot
 // at the end of the list for a particle without one), marked for the
-// gather or for the multipole zone. Every slot of the attempt gets its
-// record at every stage.
+// gather, for the multipole zone or for a table-only tile. Every slot of
+// the attempt gets its record at every stage.
 __device__ void ls_bin_pack(const LockParams& L, int64_t gi, int st, double x, double y, double z) {
   const BinView& X = L.X;
   int sl = X.slot[gi];
-  int kind = 0;  // 1 gather, 2 multipole zone
+  int kind = 0;  // 1 gather, 2 multipole zone, 3 table-only tile
   if (L.active[gi]) {
     if (st == 0 && L.fvalid[gi]) {  // the field at the state is known
       for (int c = 0; c < 3; ++c) LKB(51 + c) = LKB(54 + c);
     } else if (bem_needed(L.T.G, x, y, z)) {
-      kind = (L.T.bem.mp && multipole_zone(L.T.bem, x, y, z)) ? 2 : 1;
+      double cx, cz, h;
+      kind = (L.T.bem.mp && multipole_zone(L.T.bem, x, y, z)) ? 2
+             : (tile_only(L.T.bem, x, y, z, cx, cz, h) >= 0 ? 3 : 1);
     } else {
       double ex, ey, ez;
       geometry_field_point(L.T.G, x, y, z, 0.0, 0.0, 0.0, ex, ey, ez);
@@ -2025,14 +2274,34 @@ __device__ void ls_bin_pack(const LockParams& L, int64_t gi, int st, double x, d
       ls_keep_field(L, gi, st);
     }
   }
+  // the gather records' slots into the compact list (one atomic per
+  // wavefront) when this attempt fills it (a word every thread reads)
+  if (*X.list_on) {
+    const unsigned long long mask = __ballot(kind == 1);
+    if (mask) {
+      const int lane = __lane_id();
+      const int leader = __ffsll((long long)mask) - 1;
+      unsigned base = 0u;
+      if (lane == leader) base = atomicAdd(&X.gcount[st], (unsigned)__popcll(mask));
+      base = __shfl(base, leader);
+      if (kind == 1) {
+        if (sl < 0) {
+          sl = (int)atomicAdd(X.total, 1u);
+          X.slot[gi] = sl;
+        }
+        X.glist[base + __popcll(mask & ((1ull << lane) - 1ull))] = sl;
+      }
+    }
+  }
   if (kind) {
     if (sl < 0) {
       sl = (int)atomicAdd(X.total, 1u);
       X.slot[gi] = sl;
     }
-    X.rin[sl] = make_double4(x, y, z, __longlong_as_double(kind == 1 ? (long long)gi : -2ll - (long long)gi));
+    X.rin[sl] = make_double4(x, y, z,
+                             __longlong_as_double(kind == 1 ? (long long)gi : (kind == 2 ? kRecMultipole : kRecTable)));
   } else if (sl >= 0) {
-    X.rin[sl].w = __longlong_as_double(-1ll);
+    X.rin[sl].w = __longlong_as_double(kRecNone);
   }
   X.src[gi] = kind ? 1 : 0;
 }
@@ -2058,28 +2327,71 @@ __global__ __launch_bounds__(kBinThreads) void ls_bin_slot_kernel(LockParams L) 
   if (L.g_flags[L.pgroup[gi]] & LS_RUNNING) ls_bin_pack(L, gi, 0, LKB(48), LKB(49), LKB(50));
 }
 
-// This stage's slot records. Blocks below nb_gather: gather records, 64
-// consecutive slots per block, lane = slot (a multipole-zone record that
-// the zone test rejects falls back to the gather). Blocks from nb_gather
-// on: multipole-zone records, one per thread (they sit together at the end
-// of the slot list, after the distance-band bins).
+// Stage st's slot records. Blocks from nb_gather on: the multipole-zone and
+// table-only records, one per thread (they sit together after the windowed
+// tiles' bins and at the end of the slot list). Blocks below nb_gather: the
+// gather records (tag >= 0; the packing's zone and tile tests ran on the
+// same position, so the other tags never need the gather), either 64
+// consecutive slots per block, lane = slot, or -- with at most kFewRecords
+// of them this stage -- 16 entries of the compact list per block, 16
+// threads per point: the tail of a trace has few points left near the
+// surface, and a lone block's latency then sets the launch's time. The
+// choice depends on the count alone (not on slot order), so binned traces
+// stay reproducible.
 template <int T>
-__global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_bin_field_kernel(LockParams L, int nb_gather) {
-  __shared__ FieldShared<T, 64> sh;
+__global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_bin_field_kernel(LockParams L, int nb_gather, int st) {
+  __shared__ union {
+    FieldShared<T, 64> wide;
+    FieldShared<T, 16> few;
+  } sh;
   __shared__ int okey[64];
   const BinView& X = L.X;
   const int64_t tot = (int64_t)*X.total;
+  const unsigned ng = X.gcount[st];
   const int tid = threadIdx.x;
   if ((int)blockIdx.x >= nb_gather) {
     const int64_t i = (int64_t)(blockIdx.x - nb_gather) * T + tid;
     if (i >= tot) return;
     const double4 r = X.rin[i];
+    const long long w = __double_as_longlong(r.w);
     double qx, qy, qz;
-    if (__double_as_longlong(r.w) <= -2 && multipole_pert(L.T.bem, r.x, r.y, r.z, qx, qy, qz))
+    if (w == kRecMultipole && multipole_pert(L.T.bem, r.x, r.y, r.z, qx, qy, qz)) {
       X.rout[i] = make_double4(
           qx, qy, qz, __longlong_as_double((long long)bin_key_mp(X, L.T.bem, fma(r.x, r.x, fma(r.y, r.y, r.z * r.z)))));
+    } else if (w == kRecTable) {
+      double cx, cz, h;
+      const int t = tile_only(L.T.bem, r.x, r.y, r.z, cx, cz, h);
+      if (t >= 0) {
+        tile_point(L.T.bem, t, r.x, r.y, r.z, cx, cz, h, qx, qy, qz);
+        X.rout[i] = make_double4(qx, qy, qz, __longlong_as_double((long long)(X.key_tile0 + t)));
+      }
+    }
     return;
   }
+  if (ng <= kFewRecords && ng <= 16u * (unsigned)nb_gather) {
+    FieldShared<T, 16>& f = sh.few;
+    const unsigned base = blockIdx.x * 16u;
+    if (base >= ng) return;
+    if (tid < 16) {
+      int sl = -1;
+      if (base + tid < ng) {
+        sl = X.glist[base + tid];
+        const double4 r = X.rin[sl];
+        f.px[tid] = r.x;
+        f.py[tid] = r.y;
+        f.pz[tid] = r.z;
+      }
+      f.cidx[tid] = sl;
+      f.need[tid] = sl >= 0;
+    }
+    __syncthreads();
+    bem_pert_block<T, 16, SPT_PREC_FP64, true, true>(L.T.bem, 16, f.px, f.py, f.pz, f.ox, f.oy, f.oz, f.bem, f.need,
+                                                     okey, X.nsk, X.shift, X.key_tile0);
+    if (tid < 16 && f.need[tid])
+      X.rout[f.cidx[tid]] = make_double4(f.ox[tid], f.oy[tid], f.oz[tid], __longlong_as_double((long long)okey[tid]));
+    return;
+  }
+  FieldShared<T, 64>& f = sh.wide;
   const int64_t base = (int64_t)blockIdx.x * 64;
   if (base >= tot) return;
   const int m = (int)min((int64_t)64, tot - base);
@@ -2087,21 +2399,20 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_bin_field_kernel(Lo
     int v = 0;
     if (tid < m) {
       const double4 r = X.rin[base + tid];
-      const long long w = __double_as_longlong(r.w);
-      if (w >= 0 || (w <= -2 && !multipole_zone(L.T.bem, r.x, r.y, r.z))) {
+      if (__double_as_longlong(r.w) >= 0) {
         v = 1;
-        sh.px[tid] = r.x;
-        sh.py[tid] = r.y;
-        sh.pz[tid] = r.z;
+        f.px[tid] = r.x;
+        f.py[tid] = r.y;
+        f.pz[tid] = r.z;
       }
     }
-    sh.need[tid] = v;
+    f.need[tid] = v;
   }
-  if (!__syncthreads_or(tid < 64 && sh.need[tid])) return;
-  bem_pert_block<T, 64, SPT_PREC_FP64, true, true>(L.T.bem, 64, sh.px, sh.py, sh.pz, sh.ox, sh.oy, sh.oz, sh.bem,
-                                                   sh.need, okey, X.nsk, X.shift);
-  if (tid < m && sh.need[tid])
-    X.rout[base + tid] = make_double4(sh.ox[tid], sh.oy[tid], sh.oz[tid], __longlong_as_double((long long)okey[tid]));
+  if (!__syncthreads_or(tid < 64 && f.need[tid])) return;
+  bem_pert_block<T, 64, SPT_PREC_FP64, true, true>(L.T.bem, 64, f.px, f.py, f.pz, f.ox, f.oy, f.oz, f.bem, f.need,
+                                                   okey, X.nsk, X.shift, X.key_tile0);
+  if (tid < m && f.need[tid])
+    X.rout[base + tid] = make_double4(f.ox[tid], f.oy[tid], f.oz[tid], __longlong_as_double((long long)okey[tid]));
 }
 
 // External field at the stage positions of every active particle of every
@@ -2566,6 +2877,45 @@ __global__ void assemble_combine_kernel(const double* prof, int n, const double*
 // while length > refine_ratio * distance to the midpoint, depth <=
 // max_depth) with Gauss-4 leaves and sigma linear along the segment
 // (reference bem/axisymmetric.py:275-302). Depth-first, left half first.
+// The part of the fixed-rule perturbation field a tile tabulates: at point i
+// (rho, z), the sum over every segment outside [win[i].x, win[i].y) of its
+// sub-panel Gauss nodes (reference bem/axisymmetric.py:560-564; these
+// segments lie at least a panel length from the point, where the reference
+// takes exactly this rule). One wavefront per point, lanes over the nodes,
+// a fixed butterfly over the lanes.
+__global__ __launch_bounds__(256) void remainder_kernel(BemView B, const double* pts, const int2* win, int64_t n,
+                                                         double* out) {
+  __shared__ double2 logt[SPT_LOGT_N];
+  fill_log_table<256>(logt);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;
+  const double re = pts[2 * i], ze = pts[2 * i + 1], re2 = re * re;
+  const int2 w = win[i];
+  RingAcc acc;
+  acc.zero();
+  const int nq = B.nq, nn = B.n_seg * nq;
+  for (int q = lane; q < nn; q += 64) {
+    const int j = q / nq;
+    if (j >= w.x && j < w.y) continue;
+    const double4 v = B.nodes[q];
+    ring_accum(re, re2, ze - v.y, v.x, v.z, acc, logt);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    acc.er_main += __shfl_xor(acc.er_main, o);
+    acc.ez_main += __shfl_xor(acc.ez_main, o);
+    acc.er_ax += __shfl_xor(acc.er_ax, o);
+    acc.ez_ax += __shfl_xor(acc.ez_ax, o);
+  }
+  if (lane == 0) {
+    double er, ez;
+    ring_finish(acc, re, er, ez);
+    out[2 * i] = er;
+    out[2 * i + 1] = ez;
+  }
+}
+
 __global__ void potential_kernel(const double* prof, const double* sigma, int n, double refine_ratio, int max_depth,
                                  const double* pos, int64_t np, double* phi) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2657,6 +3007,15 @@ struct spt_ctx {
   int cl_n = 0, cl_terms = 0;
   double cl_tol = 0.0;
   double near_w = kRefNearWidth;      // spt_bem_near_width
+  // spt_bem_tiles: quadtree, windows and coefficients (one allocation
+  // each), used by far-rule gathers while tl_on
+  int* d_tl_tree = nullptr;
+  int2* d_tl_win = nullptr;
+  double* d_tl_coef = nullptr;
+  int tl_n = 0;
+  int tl_nwin = 0;  // tiles with a window (numbered first)
+  bool tl_on = false;
+  double tl_x0 = 0.0, tl_z0 = 0.0, tl_size = 0.0;
   // scratch
   double* d_kbuf = nullptr;
   size_t kbuf_elems = 0;
@@ -2684,6 +3043,7 @@ struct spt_ctx {
   int32_t* d_bin_slot = nullptr;
   uint8_t* d_bin_src = nullptr;
   double4* d_bin_rec = nullptr;   // rin then rout (n each)
+  int32_t* d_bin_glist = nullptr; // the compact list of gather records (n)
   size_t bin_cap = 0;
   // surrogate used by spt_trace
   bool have_sur = false;
@@ -2760,7 +3120,25 @@ static BemView bem_view(const spt_ctx* c) {
   B.cl_terms = c->cl_terms;
   B.cl_tol = c->cl_tol;
   B.near_w = c->far_k4 > 0.0 ? c->near_w : kRefNearWidth;
+  const bool tl = c->far_k4 > 0.0 && c->tl_on && c->tl_n > 0;
+  B.tl_tree = tl ? c->d_tl_tree : nullptr;
+  B.tl_win = tl ? c->d_tl_win : nullptr;
+  B.tl_coef = tl ? c->d_tl_coef : nullptr;
+  B.tl_x0 = c->tl_x0;
+  B.tl_z0 = c->tl_z0;
+  B.tl_size = c->tl_size;
   return B;
+}
+
+static void free_tiles(spt_ctx* c) {
+  if (c->d_tl_tree) (void)hipFree(c->d_tl_tree);
+  if (c->d_tl_win) (void)hipFree(c->d_tl_win);
+  if (c->d_tl_coef) (void)hipFree(c->d_tl_coef);
+  c->d_tl_tree = nullptr;
+  c->d_tl_win = nullptr;
+  c->d_tl_coef = nullptr;
+  c->tl_n = 0;
+  c->tl_on = false;
 }
 
 static void free_bem(spt_ctx* c) {
@@ -2771,8 +3149,9 @@ static void free_bem(spt_ctx* c) {
   if (c->d_coarse) (void)hipFree(c->d_coarse);
   c->d_prof = c->d_sigma = c->d_segs = nullptr;
   c->d_nodes = c->d_coarse = nullptr;
-  c->mp_n = 0;  // the expansions belong to the uploaded charge
+  c->mp_n = 0;  // the expansions and tiles belong to the uploaded charge
   c->cl_n = 0;
+  free_tiles(c);
   c->bem_nodes = 0;
 }
 
@@ -2833,6 +3212,7 @@ void spt_destroy(spt_ctx* c) {
   if (c->d_bin_slot) (void)hipFree(c->d_bin_slot);
   if (c->d_bin_src) (void)hipFree(c->d_bin_src);
   if (c->d_bin_rec) (void)hipFree(c->d_bin_rec);
+  if (c->d_bin_glist) (void)hipFree(c->d_bin_glist);
   if (c->d_sur_p) (void)hipFree(c->d_sur_p);
   if (c->d_sur_split) (void)hipFree(c->d_sur_split);
   if (c->d_sur_frag) (void)hipFree(c->d_sur_frag);
@@ -3006,6 +3386,63 @@ int spt_bem_near_width(spt_ctx* c, double width) {
   if (width == 0.0) width = kRefNearWidth;
   if (!(width > 0.0 && width <= 4.0)) return fail(c, SPT_EINVAL, "spt_bem_near_width: need 0 < width <= 4 (0: 0.5)");
   c->near_w = width;
+  return SPT_OK;
+}
+
+int spt_bem_tiles(spt_ctx* c, const int32_t* tree, int32_t n_tree, const int32_t* win, const double* coef,
+                  int32_t n_tiles, double x0, double z0, double size) {
+  if (!c) return SPT_EINVAL;
+  if (c->bem_nodes == 0) return fail(c, SPT_ENOBEM, "spt_bem_tiles: no BEM table uploaded");
+  const int n_seg = c->bem_nodes - 1;
+  if (!tree || !win || !coef || n_tree < 1 || n_tiles < 1 || n_tiles > kTileMaxTiles || !(size > 0.0) ||
+      !std::isfinite(x0) || !std::isfinite(z0) || !std::isfinite(size))
+    return fail(c, SPT_EINVAL, "spt_bem_tiles: need a tree, 1 <= n_tiles <= 6144 and a finite box of positive size");
+  // The device walk trusts the tree: children in range and after their
+  // parent (so every walk ends), tiles in range, windows within the profile.
+  for (int k = 0; k < n_tree; ++k) {
+    const int v = tree[k];
+    if (v >= 0 ? (v <= k || v > n_tree - 4) : (v < -1 && -2 - v >= n_tiles))
+      return fail(c, SPT_EINVAL, "spt_bem_tiles: tree node " + std::to_string(k) + " out of range");
+  }
+  for (int t = 0; t < n_tiles; ++t)
+    if (win[2 * t] < 0 || win[2 * t + 1] < win[2 * t] || win[2 * t + 1] > n_seg)
+      return fail(c, SPT_EINVAL, "spt_bem_tiles: window of tile " + std::to_string(t) + " out of range");
+  HIPCHK(c, spt_begin(c));
+  free_tiles(c);
+  HIPCHK(c, hipMalloc(&c->d_tl_tree, sizeof(int) * n_tree));
+  HIPCHK(c, hipMalloc(&c->d_tl_win, sizeof(int2) * n_tiles));
+  HIPCHK(c, hipMalloc(&c->d_tl_coef, sizeof(double) * kTileCoef * (size_t)n_tiles));
+  HIPCHK(c, hipMemcpyAsync(c->d_tl_tree, tree, sizeof(int) * n_tree, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_tl_win, win, sizeof(int2) * n_tiles, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_tl_coef, coef, sizeof(double) * kTileCoef * (size_t)n_tiles, hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->tl_n = n_tiles;
+  c->tl_nwin = 0;
+  while (c->tl_nwin < n_tiles && win[2 * c->tl_nwin + 1] > win[2 * c->tl_nwin]) ++c->tl_nwin;
+  c->tl_x0 = x0;
+  c->tl_z0 = z0;
+  c->tl_size = size;
+  c->tl_on = true;
+  return SPT_OK;
+}
+
+int spt_bem_tiles_enable(spt_ctx* c, int32_t on) {
+  if (!c) return SPT_EINVAL;
+  if (on && c->tl_n == 0) return fail(c, SPT_EINVAL, "spt_bem_tiles_enable: no tiles uploaded");
+  c->tl_on = on != 0;
+  return SPT_OK;
+}
+
+int spt_bem_remainder(spt_ctx* c, const double* pts, const int32_t* win, int64_t n, double* E) {
+  if (!c) return SPT_EINVAL;
+  if (!c->bem_nodes) return fail(c, SPT_ENOBEM, "no BEM table uploaded");
+  if (n < 0 || (n > 0 && (!pts || !win || !E))) return fail(c, SPT_EINVAL, "spt_bem_remainder: bad arguments");
+  if (n == 0) return SPT_OK;
+  HIPCHK(c, spt_begin(c));
+  remainder_kernel<<<(unsigned)((n + 3) / 4), 256, 0, c->stream>>>(bem_view(c), pts,
+                                                                      reinterpret_cast<const int2*>(win), n, E);
+  HIPCHK(c, hipGetLastError());
   return SPT_OK;
 }
 
@@ -3364,8 +3801,12 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
   const bool pack4 = TG == 64 && max_size <= 16;  // four groups per accel block
   const unsigned nb_bin = (unsigned)((n + kBinThreads - 1) / kBinThreads);
   const unsigned nb_slots = (unsigned)((n + 63) / 64);
+  // blocks for the records evaluated one per thread (multipole zone, table-only tiles)
+  const bool one_per_thread = L.T.bem.mp || L.T.bem.tl_tree;
   if (bin) {
     fill_u32_kernel<<<(L.X.n_keys + 255) / 256, 256, 0, s>>>(L.X.hist, L.X.n_keys, 0u);
+    fill_u32_kernel<<<1, 64, 0, s>>>(L.X.n_start, 1, 0u);
+
     ls_bin_key_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(L);
   }
   HIPCHK(c, hipGetLastError());
@@ -3381,11 +3822,11 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
       for (int st = 0; st < 7; ++st) {
         if (bin) {
           if (fs.T == 192)
-            ls_bin_field_kernel<192><<<nb_slots + (L.T.bem.mp ? (unsigned)((n + 191) / 192) : 0u), 192, 0, s>>>(
-                L, (int)nb_slots);
+            ls_bin_field_kernel<192><<<nb_slots + (one_per_thread ? (unsigned)((n + 191) / 192) : 0u), 192, 0, s>>>(
+                L, (int)nb_slots, st);
           else
-            ls_bin_field_kernel<256><<<nb_slots + (L.T.bem.mp ? (unsigned)((n + 255) / 256) : 0u), 256, 0, s>>>(
-                L, (int)nb_slots);
+            ls_bin_field_kernel<256><<<nb_slots + (one_per_thread ? (unsigned)((n + 255) / 256) : 0u), 256, 0, s>>>(
+                L, (int)nb_slots, st);
         } else if (L.sur) {
           if (FPR == SPT_PREC_FP64)
             ls_surrogate_kernel<true><<<(unsigned)((n + MLP64_PTS - 1) / MLP64_PTS), 64, 0, s>>>(L, st);
@@ -3461,12 +3902,14 @@ static int ensure_bin_scratch(spt_ctx* c, int64_t n) {
     if (c->d_bin_slot) (void)hipFree(c->d_bin_slot);
     if (c->d_bin_src) (void)hipFree(c->d_bin_src);
     if (c->d_bin_rec) (void)hipFree(c->d_bin_rec);
-    c->d_bin_key = c->d_bin_slot = nullptr;
+    if (c->d_bin_glist) (void)hipFree(c->d_bin_glist);
+    c->d_bin_key = c->d_bin_slot = c->d_bin_glist = nullptr;
     c->d_bin_src = nullptr;
     c->d_bin_rec = nullptr;
     c->bin_cap = 0;
     HIPCHK(c, hipMalloc(&c->d_bin_key, sizeof(int32_t) * n));
     HIPCHK(c, hipMalloc(&c->d_bin_slot, sizeof(int32_t) * n));
+    HIPCHK(c, hipMalloc(&c->d_bin_glist, sizeof(int32_t) * n));
     HIPCHK(c, hipMalloc(&c->d_bin_src, n));
     HIPCHK(c, hipMalloc(&c->d_bin_rec, sizeof(double4) * 2 * n));
     c->bin_cap = n;
@@ -3663,11 +4106,13 @@ static int trace_impl(spt_ctx* c, const spt_trace_args* a, bool no_alloc) {
     if (bin) {
       int rb = ensure_bin_scratch(c, a->n_particles);
       if (rb) return rb;
+      const int n_tile = P.bem.tl_tree ? c->tl_n : 0;  // kTileMaxTiles at most
       int shift = 0;
-      while (kBinBands * ((P.bem.n_seg + (1 << shift) - 1) >> shift) + kBinMpKeys > kBinMaxKeys) ++shift;
+      while (kBinBands * ((P.bem.n_seg + (1 << shift) - 1) >> shift) + n_tile + kBinMpKeys > kBinMaxKeys) ++shift;
       L.X.nsk = (P.bem.n_seg + (1 << shift) - 1) >> shift;
       L.X.shift = shift;
-      L.X.key_mp0 = kBinBands * L.X.nsk;
+      L.X.key_tile0 = kBinBands * L.X.nsk;
+      L.X.key_mp0 = L.X.key_tile0 + n_tile;
       L.X.n_keys = L.X.key_mp0 + (P.bem.mp ? kBinMpKeys : 0);
       L.X.key = c->d_bin_key;
       L.X.slot = c->d_bin_slot;
@@ -3675,6 +4120,11 @@ static int trace_impl(spt_ctx* c, const spt_trace_args* a, bool no_alloc) {
       L.X.hist = c->d_bin_ctl;
       L.X.cursor = c->d_bin_ctl + kBinMaxKeys;
       L.X.total = c->d_bin_ctl + 2 * kBinMaxKeys;
+      L.X.gcount = c->d_bin_ctl + 2 * kBinMaxKeys + 8;  // 7 stage counts
+      L.X.list_on = c->d_bin_ctl + 2 * kBinMaxKeys + 16;
+      L.X.n_start = c->d_bin_ctl + 2 * kBinMaxKeys + 17;
+      L.X.glist = c->d_bin_glist;
+      L.X.key_tab0 = L.X.key_tile0 + (P.bem.tl_tree ? c->tl_nwin : 0);
       L.X.rin = c->d_bin_rec;
       L.X.rout = c->d_bin_rec + a->n_particles;
       L.X.append_only = c->bin_mode == 2;

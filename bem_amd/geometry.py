@@ -64,18 +64,21 @@ class Geometry(ABC):
                 sig_c, sp = _lib.host_ptr(np.asarray(sig, dtype=np.float64))
                 ctx.check(ctx.lib.spt_bem_upload(ctx.h, pp, prof_c.shape[0], sp, int(nsub)), "spt_bem_upload")
                 ctx.bem_key = key
+                ctx.tiles_key = None  # the upload drops any tiles
         return ctx
 
     @contextlib.contextmanager
-    def far_rule(self, k4=2.0, k2=16.0, near_width=None, clusters=None):
+    def far_rule(self, k4=2.0, k2=16.0, near_width=None, clusters=None, tiles=None):
         """Within the block, every field gather on this geometry's context
         sums a segment at distance D >= L over the whole segment by Gauss-8
         below k4 L, Gauss-4 below k2 L and Gauss-2 beyond (spt_bem_far_rule),
         instead of the reference's fixed sub-panel table; near_width widens
         the near rule's intervals (spt_bem_near_width) and clusters (a
         bem_amd.bem.clusters.Clusters of this geometry) replaces runs of
-        distant segments by their expansions (spt_bem_clusters). Restored on
-        exit."""
+        distant segments by their expansions (spt_bem_clusters) and tiles (a
+        bem_amd.bem.tiles.Tiles of this geometry) replace the segments
+        outside a point's tile window by the tile's table (spt_bem_tiles).
+        Restored on exit."""
         ctx = self._ctx()
         ctx.check(ctx.lib.spt_bem_far_rule(ctx.h, float(k4), float(k2)), "spt_bem_far_rule")
         try:
@@ -83,11 +86,14 @@ class Geometry(ABC):
                 ctx.check(ctx.lib.spt_bem_near_width(ctx.h, float(near_width)), "spt_bem_near_width")
             if clusters is not None:
                 clusters.install(ctx)
+            if tiles is not None:
+                tiles.install(ctx)
             yield
         finally:
             ctx.lib.spt_bem_far_rule(ctx.h, 0.0, 0.0)
             ctx.lib.spt_bem_near_width(ctx.h, 0.0)
             ctx.lib.spt_bem_clusters(ctx.h, None, 0, None, None, None, 0, 0.0, 0.0)
+            ctx.lib.spt_bem_tiles_enable(ctx.h, 0)
 
     def _device_field(self, position, precision="fp64"):
         return _DeviceField(self.spt_geometry(), self._ctx(), _precision_code(precision)).evaluate(position)

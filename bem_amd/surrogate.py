@@ -280,14 +280,14 @@ def validate(sur, n_val, d_min, d_max, tol, panel, seed, theta_max=0.5 * np.pi):
 
 
 def far_rule_error(geometry, k4=2.0, k2=16.0, n=1 << 16, d_min=None, d_max=110e-6, seed=11,
-                   theta_max=np.deg2rad(100.0), near_width=None, clusters=None):
+                   theta_max=np.deg2rad(100.0), near_width=None, clusters=None, tiles=None):
     """Field error of the distance-adaptive far rule against the fixed
     rule on fresh points around the cap and the post's side (distance from
     the cap sphere log-uniform in [d_min, d_max], d_min defaulting to 1e-3
     panel lengths; points inside the conductor, where the tracer uses no
     field, are dropped), relative to the full external field magnitude as
-    in validate(). near_width and clusters add the wider near rule and the
-    cluster expansions (geometry.far_rule). Returns max, p99 and median, and
+    in validate(). near_width, clusters and tiles add the wider near rule,
+    the cluster expansions and the tiles (geometry.far_rule). Returns max, p99 and median, and
     the distance from the conductor surface of the worst point."""
     prof = geometry.solution.profile
     panel = float(np.max(np.linalg.norm(np.diff(prof, axis=0), axis=1)))
@@ -300,11 +300,12 @@ def far_rule_error(geometry, k4=2.0, k2=16.0, n=1 << 16, d_min=None, d_max=110e-
     keep = (z > 0.0) & (d >= 0.0)
     rho, z, d = rho[keep], z[keep], d[keep]
     ref = bem_labels(geometry, rho, z).cpu().numpy()
-    with geometry.far_rule(k4, k2, near_width=near_width, clusters=clusters):
+    with geometry.far_rule(k4, k2, near_width=near_width, clusters=clusters, tiles=tiles):
         got = bem_labels(geometry, rho, z).cpu().numpy()
     Et = np.hypot(ref[:, 1], ref[:, 2] + geometry.E_gun)
     err = np.hypot(got[:, 1] - ref[:, 1], got[:, 2] - ref[:, 2]) / Et
     return dict(k4=k4, k2=k2, near_width=near_width, clusters=None if clusters is None else clusters.n_clusters,
+                tiles=None if tiles is None else tiles.n_tiles,
                 n=int(rho.shape[0]), d_min=d_min, d_max=d_max, field_err_max=float(err.max()),
                 field_err_p99=float(np.quantile(err, 0.99)), field_err_median=float(np.median(err)),
                 d_at_max=float(d[np.argmax(err)]))

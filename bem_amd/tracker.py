@@ -25,7 +25,9 @@ for the BEM perturbation wherever its validated band allows; and
 distance-adaptive far rule of spt_bem_far_rule for the BEM gathers of the
 trace; and ``multipole``: None or a validated bem_amd.bem.multipole.Multipole
 of the geometry, whose exterior expansion replaces the segment sums beyond
-its radius (both field-error budgets, not the plain method).
+its radius; ``tiles``: None or a bem_amd.bem.tiles.Tiles of the geometry,
+whose tables replace the segments outside each tile's window (field-error
+budgets, not the plain method).
 """
 import warnings
 
@@ -52,7 +54,7 @@ class SpecificParticleTracer:
                  plummer_radius=DEFAULT_PLUMMER_RADIUS, backend="mi355x", n_workers=1, rtol=None, atol=None,
                  t_max=None, z_max=None, group_sizes=None, block_threads=0, device=0, max_steps=10_000_000,
                  mode="auto", precision="fp64", surrogate=None, far_rule=None, multipole=None, near_width=None,
-                 clusters=None, spatial_order=False, binned=True):
+                 clusters=None, spatial_order=False, binned=True, tiles=None):
         if n_workers != 1:
             raise ValueError(f"n_workers is only supported for backend='cpu' (got backend={backend!r}, "
                              f"n_workers={n_workers!r})")
@@ -102,8 +104,11 @@ class SpecificParticleTracer:
             if geometry.bem_table() is None:
                 raise ValueError("far_rule applies to BEM geometries only")
         self.far_rule = far_rule
-        if (near_width is not None or clusters is not None) and far_rule is None:
-            raise ValueError("near_width and clusters refine the far rule: set far_rule too")
+        if (near_width is not None or clusters is not None or tiles is not None) and far_rule is None:
+            raise ValueError("near_width, clusters and tiles refine the far rule: set far_rule too")
+        if tiles is not None and tiles.geometry is not geometry:
+            raise ValueError("the tiles were built for a different geometry object")
+        self.tiles = tiles
         if clusters is not None and clusters.geometry is not geometry:
             raise ValueError("the cluster expansions were built for a different geometry object")
         self.near_width = near_width
@@ -347,6 +352,8 @@ class SpecificParticleTracer:
             ctx.check(ctx.lib.spt_bem_near_width(ctx.h, float(self.near_width)), "spt_bem_near_width")
         if self.clusters is not None:
             self.clusters.install(ctx)
+        if self.tiles is not None:
+            self.tiles.install(ctx)
         if self.multipole is not None and not self.multipole.inert:
             self.multipole.install(ctx)
         t0 = torch.cuda.Event(enable_timing=True)
@@ -363,6 +370,7 @@ class SpecificParticleTracer:
             ctx.lib.spt_bem_far_rule(ctx.h, 0.0, 0.0)
             ctx.lib.spt_bem_near_width(ctx.h, 0.0)
             ctx.lib.spt_bem_clusters(ctx.h, None, 0, None, None, None, 0, 0.0, 0.0)
+            ctx.lib.spt_bem_tiles_enable(ctx.h, 0)
             ctx.lib.spt_bem_multipole(ctx.h, None, 0, 0.0, 0.0)
         t1.record()
         ctx.sync()

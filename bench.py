@@ -69,7 +69,7 @@ def workload(n_el=262144, group=16, n_segments=400, seed=0, rtol=1e-8):
 
 
 def make_tracer(wl, geo, surrogate=None, precision="pairs32", mode="lockstep", far_rule=None, multipole=None,
-                near_width=None, clusters=None):
+                near_width=None, clusters=None, tiles=None):
     from bem_amd import SpecificParticleTracer
     from bem_amd.particles import ParticleGroup
 
@@ -78,7 +78,8 @@ def make_tracer(wl, geo, surrogate=None, precision="pairs32", mode="lockstep", f
     pg = ParticleGroup(dict(src, weight=np.full(n, 1.602176634e-19), status=np.ones(n), species="electron"))
     return SpecificParticleTracer(pg, wl["group"], geo, wl["screens"], rtol=wl["rtol"], atol=wl["atol"],
                                   z_max=wl["z_max"], mode=mode, precision=precision, surrogate=surrogate,
-                                  far_rule=far_rule, multipole=multipole, near_width=near_width, clusters=clusters)
+                                  far_rule=far_rule, multipole=multipole, near_width=near_width, clusters=clusters,
+                                  tiles=tiles)
 
 
 def timed_traces(tr, steps, warmup):

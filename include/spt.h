@@ -285,6 +285,37 @@ int spt_bem_clusters(spt_ctx* ctx, const int32_t* seg_begin, int32_t n_clusters,
  * 0.5; at most 4. A field-error budget, not a parity mode. */
 int spt_bem_near_width(spt_ctx* ctx, double width);
 
+/* Chebyshev tiles for every later far-rule gather on this context (no
+ * counterpart in the reference, which sums every segment at every point,
+ * reference bem/axisymmetric.py:541-605; only active while
+ * spt_bem_far_rule is). A quadtree over the square [x0, x0 + size) x
+ * [z0, z0 + size) of (rho, z): tree[k] >= 0 is the index of node k's first
+ * child (four in a row: low rho low z, high rho low z, low rho high z, high
+ * rho high z; always after k), -1 a cell without a tile, <= -2 tile
+ * -2 - tree[k]. A point in tile t sums the segments win[2t] .. win[2t+1] - 1
+ * with the far-rule gather's rules and takes the fixed-rule field of every
+ * other segment from the tile's table, coef[t * 288 ..]: 12 x 12 Chebyshev
+ * coefficients c[a][b] of E_rho, then of E_z, in T_a(u) T_b(v) with (u, v)
+ * the point's position in the cell scaled to [-1, 1]^2 (first index a:
+ * rho). The window must hold every segment within a panel length of the
+ * cell (bem_amd.bem.tiles builds and validates such tables). Host arrays,
+ * copied; 1 <= n_tiles <= 6144. Enables the tiles; spt_bem_upload drops
+ * them. A field-error budget, not a parity mode. */
+int spt_bem_tiles(spt_ctx* ctx, const int32_t* tree, int32_t n_tree, const int32_t* win, const double* coef,
+                  int32_t n_tiles, double x0, double z0, double size);
+
+/* Turns the uploaded tiles off (0) or on again (1) for later far-rule
+ * gathers, keeping the upload. */
+int spt_bem_tiles_enable(spt_ctx* ctx, int32_t on);
+
+/* The field a tile tabulates, for building tables: at each of n device
+ * points pts (rho, z pairs) the fixed-rule perturbation field (E_rho, E_z
+ * pairs into device E) of every segment outside [win[2i], win[2i+1])
+ * (device int32 pairs), summed over their sub-panel Gauss nodes as the
+ * reference sums segments at least a panel length away (reference
+ * bem/axisymmetric.py:560-564). */
+int spt_bem_remainder(spt_ctx* ctx, const double* pts, const int32_t* win, int64_t n, double* E);
+
 /* Collocation matrix of the P1 hat basis (reference
  * bem/axisymmetric.py:275-302,430-441), row-major into host A_out
  * (n_nodes x n_nodes). */
