@@ -2084,11 +2084,7 @@ __device__ void ls_begin(const LockParams& L, int g, int n, int tid, double t, d
   const double n_act = group_sum<TG, GPB>(active ? 1.0 : 0.0, red);
   if (mine) {
     L.active[gi] = active ? 1 : 0;
-    for (int c = 0; c < 3; ++c) {
-      const double ps = L.T.pos[3 * gi + c], vs = L.T.vel[3 * gi + c];
-      LKB(c) = active ? vs : 0.0;
-      LKB(48 + c) = ps;
-    }
+    for (int c = 0; c < 3; ++c) LKB(48 + c) = L.T.pos[3 * gi + c];
   }
   if (tid == 0) {
     L.g_hr[g] = hr;
@@ -2628,39 +2624,55 @@ __global__ __launch_bounds__(TG) void ls_accel_kernel(LockParams L, int st) {
   if (L.imgF)
     for (int c = 0; c < 3; ++c) Fi[c] = L.imgF[3 * gi + c];
   const double mass = L.T.mass[gi];
+  double kst[3];  // this stage's k (velocity part): the acceleration
   for (int c = 0; c < 3; ++c) {
     const double ac = n == 1 ? 0.0 : Fc[c] / mass;
     const double a = L.T.q_over_m * LKB(51 + c) + ac + Fi[c] / mass;
-    LKB(st * 6 + 3 + c) = active ? a : 0.0;
+    kst[c] = active ? a : 0.0;
+    LKB(st * 6 + 3 + c) = kst[c];
   }
-  if (st < 5) {
-    const int ns = st + 1;
+  if (st < 6) {
+    // The position part of stage j's k is stage j's velocity, not stored:
+    // formed again from the stored accelerations with the same products in
+    // the same order as when stage j was made (so bit for bit the same).
     for (int c = 0; c < 3; ++c) {
-      double ps = L.T.pos[3 * gi + c], vs = L.T.vel[3 * gi + c];
-      for (int j = 0; j < ns; ++j) {
-        const double ha = hr * c_A[ns][j];
-        ps = ps + ha * LKB(j * 6 + c);
-        vs = vs + ha * LKB(j * 6 + 3 + c);
+      const double p0 = L.T.pos[3 * gi + c], v0 = L.T.vel[3 * gi + c];
+      double kv[6], kp[6];
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        kv[j] = j < st ? LKB(j * 6 + 3 + c) : (j == st ? kst[c] : 0.0);
+        double v = v0;
+#pragma unroll
+        for (int i = 0; i < j; ++i) v = v + (hr * c_A[j][i]) * kv[i];
+        kp[j] = active ? v : 0.0;
       }
-      LKB(ns * 6 + c) = active ? vs : 0.0;
-      LKB(48 + c) = ps;
-    }
-    if (L.X.key) ls_bin_pack(L, gi, ns, LKB(48), LKB(49), LKB(50));
-  } else if (st == 5) {
-    for (int c = 0; c < 3; ++c) {
-      double sp = c_B[0] * LKB(c), sv = c_B[0] * LKB(3 + c);
-      for (int k = 1; k < 6; ++k) {
-        sp = sp + c_B[k] * LKB(k * 6 + c);
-        sv = sv + c_B[k] * LKB(k * 6 + 3 + c);
+      if (st < 5) {
+        const int ns = st + 1;
+        double ps = p0, vs = v0;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+          if (j < ns) {
+            const double ha = hr * c_A[ns][j];
+            ps = ps + ha * kp[j];
+            vs = vs + ha * kv[j];
+          }
+        }
+        LKB(48 + c) = ps;
+      } else {
+        double sp = c_B[0] * kp[0], sv = c_B[0] * kv[0];
+#pragma unroll
+        for (int k = 1; k < 6; ++k) {
+          sp = sp + c_B[k] * kp[k];
+          sv = sv + c_B[k] * kv[k];
+        }
+        const double pn = p0 + hr * sp;
+        const double vn = v0 + hr * sv;
+        LKB(42 + c) = pn;
+        LKB(45 + c) = vn;
+        LKB(48 + c) = pn;
       }
-      const double pn = L.T.pos[3 * gi + c] + hr * sp;
-      const double vn = L.T.vel[3 * gi + c] + hr * sv;
-      LKB(42 + c) = pn;
-      LKB(45 + c) = vn;
-      LKB(36 + c) = active ? vn : 0.0;
-      LKB(48 + c) = pn;
     }
-    if (L.X.key) ls_bin_pack(L, gi, 6, LKB(48), LKB(49), LKB(50));
+    if (L.X.key) ls_bin_pack(L, gi, st + 1, LKB(48), LKB(49), LKB(50));
   }
 }
 
@@ -2684,11 +2696,28 @@ __global__ __launch_bounds__(TG) void ls_end_kernel(LockParams L) {
   const double hr = live ? L.g_hr[g] : 0.0;
   double lp = 0.0, lv = 0.0;
   if (mine) {
+    const bool active = L.active[gi];
     for (int c = 0; c < 3; ++c) {
-      double ep = c_E[0] * LKB(c), ev = c_E[0] * LKB(3 + c);
+      // the position parts of the k: the stage velocities, formed again as
+      // the accel kernel forms them (stage 6's is the new velocity)
+      const double v0 = L.T.vel[3 * gi + c], vn = LKB(45 + c);
+      double kv[7], kp[7];
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        kv[k] = LKB(k * 6 + 3 + c);
+        double v = v0;
+        if (k < 6) {
+#pragma unroll
+          for (int i = 0; i < k; ++i) v = v + (hr * c_A[k][i]) * kv[i];
+        } else {
+          v = vn;
+        }
+        kp[k] = active ? v : 0.0;
+      }
+      double ep = c_E[0] * kp[0], ev = c_E[0] * kv[0];
       for (int k = 1; k < 7; ++k) {
-        ep = ep + c_E[k] * LKB(k * 6 + c);
-        ev = ev + c_E[k] * LKB(k * 6 + 3 + c);
+        ep = ep + c_E[k] * kp[k];
+        ev = ev + c_E[k] * kv[k];
       }
       ep = hr * ep;
       ev = hr * ev;
