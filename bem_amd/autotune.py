@@ -11,7 +11,7 @@ of:
 * mode "lockstep" with the BEM field gather at 192 or 256 threads per
   block and 1/4x .. 4x the automatic points per block (the binned gather of
   far-rule traces always takes 64 points per block: only its thread count
-  is tuned).
+  is tuned, 64 .. 256, i.e. one to four slices per point).
 
 The winner is written back to the tracer (mode, block_threads,
 field_shape) and returned with the whole timing table. Traces with the
@@ -51,7 +51,7 @@ def candidates(tracer, n_cu):
             out.append(dict(mode="persistent", block_threads=T, field_shape=None))
     pb0 = _auto_points_per_block(n, n_cu)
     pbs = [64] if binned else sorted({max(1, pb0 // 4), max(1, pb0 // 2), pb0, min(64, pb0 * 2), min(64, pb0 * 4)})
-    for T in (192, 256):
+    for T in ((64, 128, 192, 256) if binned else (192, 256)):
         for pb in pbs:
             out.append(dict(mode="lockstep", block_threads=0, field_shape=(T, pb)))
     return out

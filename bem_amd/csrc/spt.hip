@@ -3276,7 +3276,8 @@ int spt_sync(spt_ctx* c) {
 
 int spt_set_field_shape(spt_ctx* c, int32_t threads, int32_t points_per_block) {
   if (!c) return SPT_EINVAL;
-  if (threads != 0 && threads != 192 && threads != 256) return fail(c, SPT_EINVAL, "threads must be 0, 192 or 256");
+  if (threads != 0 && threads != 64 && threads != 128 && threads != 192 && threads != 256)
+    return fail(c, SPT_EINVAL, "threads must be 0, 64, 128, 192 or 256");
   if (points_per_block < 0 || points_per_block > FIELD_NMAX_PUBLIC ||
       (points_per_block & (points_per_block - 1)) != 0)
     return fail(c, SPT_EINVAL, "points_per_block must be 0 or a power of two <= 64");
@@ -3604,7 +3605,14 @@ int spt_bem_pert_field(spt_ctx* c, const double* pos, int64_t n, double* E, int3
   HIPCHK(c, spt_begin(c));
   const FieldShape fs = field_shape(c, n, true);
   const unsigned nb = (unsigned)((n + fs.PB - 1) / fs.PB);
-  if (fs.T == 192) {
+  if ((fs.T == 64 || fs.T == 128) && precision != SPT_PREC_MIXED) {
+    // the binned lockstep gather's narrow shapes (one or two slices per
+    // point at 64 points per block)
+    if (fs.T == 64)
+      bem_pert_kernel<64, FIELD_NMAX, SPT_PREC_FP64><<<nb, 64, 0, c->stream>>>(bem_view(c), pos, n, E, fs.PB);
+    else
+      bem_pert_kernel<128, FIELD_NMAX, SPT_PREC_FP64><<<nb, 128, 0, c->stream>>>(bem_view(c), pos, n, E, fs.PB);
+  } else if (fs.T == 192) {
     if (precision == SPT_PREC_MIXED)
       bem_pert_kernel<192, FIELD_NMAX, SPT_PREC_MIXED><<<nb, 192, 0, c->stream>>>(bem_view(c), pos, n, E, fs.PB);
     else
@@ -3850,12 +3858,15 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
       }
       for (int st = 0; st < 7; ++st) {
         if (bin) {
-          if (fs.T == 192)
-            ls_bin_field_kernel<192><<<nb_slots + (one_per_thread ? (unsigned)((n + 191) / 192) : 0u), 192, 0, s>>>(
-                L, (int)nb_slots, st);
+          const unsigned nb_one = one_per_thread ? (unsigned)((n + fs.T - 1) / fs.T) : 0u;
+          if (fs.T == 64)
+            ls_bin_field_kernel<64><<<nb_slots + nb_one, 64, 0, s>>>(L, (int)nb_slots, st);
+          else if (fs.T == 128)
+            ls_bin_field_kernel<128><<<nb_slots + nb_one, 128, 0, s>>>(L, (int)nb_slots, st);
+          else if (fs.T == 192)
+            ls_bin_field_kernel<192><<<nb_slots + nb_one, 192, 0, s>>>(L, (int)nb_slots, st);
           else
-            ls_bin_field_kernel<256><<<nb_slots + (one_per_thread ? (unsigned)((n + 255) / 256) : 0u), 256, 0, s>>>(
-                L, (int)nb_slots, st);
+            ls_bin_field_kernel<256><<<nb_slots + nb_one, 256, 0, s>>>(L, (int)nb_slots, st);
         } else if (L.sur) {
           if (FPR == SPT_PREC_FP64)
             ls_surrogate_kernel<true><<<(unsigned)((n + MLP64_PTS - 1) / MLP64_PTS), 64, 0, s>>>(L, st);

@@ -48,12 +48,14 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-RK-stages/sec, 1 GPU, surrogate+batched vs plain at field err ≤1e-6"
 PEAK_FP64_TFLOPS = 78.6
 C_RING = 120          # FP64 flops per ring-kernel node (K, E, log, kernel algebra, accumulation)
-C_TERM = 16    # FP64 flops per term of an axis-centred cluster expansion
+C_TERM = 16    # FP64 flops per term of an axis-centred cluster expansion or of the multipole zone
+C_TABLE = 2 * 2 * 12 * 12  # FP64 flops per tile table: two 12 x 12 Chebyshev sums, one FMA per coefficient
 # HBM bytes per 262,144-point field-gather launch from the rocprofv3 counter runs (FETCH_SIZE and WRITE_SIZE at
 # the timed launch shape, scaled by the same counters' reading for a copy of known size)
 TRAFFIC_BYTES = 19.8e6
 TRAFFIC_SOURCE = ("FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB copy), fixed rule at (256, 64), "
                   "profiles/bem_field_gather_post400_262k.md")
+FAST_TRAFFIC_SHAPE = (256, 64)
 FAST_TRAFFIC_BYTES = 13.4e6
 FAST_TRAFFIC_SOURCE = ("FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB copy), fast-form gather at "
                        "(256, 64) on the points in bin order, profiles/bem_field_gather_fast_form_post400_262k.md")
@@ -137,49 +139,82 @@ def gather_points(wl, n_pts=262144):
     return pts
 
 
-def gather_work_per_point(geo, pts, k4, k2, near_width=0.5, clusters=None, sample=8192):
-    """Mean algorithmic work per point of a far-rule gather over the first
-    `sample` points, counted on the host from the same rules the device
-    applies per point: ring-kernel nodes of the far rule on segments outside
-    far clusters (8 per segment at L <= D < k4 L, 4 below k2 L, 2 beyond),
-    sinh-rule nodes of near segments (D < L: 8 per interval of width
-    near_width), and expansion terms of far clusters (the truncation rule of
-    bem_amd.bem.clusters). Returns (ring nodes, near nodes, expansion terms)
-    per point."""
+def point_classes(geo, pts, budget):
+    """How the fast form's gather takes each point, as the library decides:
+    the multipole zone (r >= r_min), a tile (its index; table-only tiles
+    have an empty window) or neither (-1: the far-rule gather with cluster
+    expansions). Returns (in_mp bool, tile int)."""
+    rho, z = np.hypot(pts[:, 0], pts[:, 1]), pts[:, 2]
+    mp = budget.multipole
+    in_mp = (np.hypot(rho, z) >= mp.r_min) if mp is not None and not mp.inert else np.zeros(len(pts), dtype=bool)
+    tile = np.full(len(pts), -1, dtype=np.int64)
+    if budget.tiles is not None:
+        tile, _, _, _ = budget.tiles.find(rho, z)
+    tile = np.where(in_mp, -1, tile)
+    return in_mp, tile
+
+
+def gather_work_per_point(geo, pts, budget, sample=8192):
+    """Mean algorithmic work per point of the fast form's gather over the
+    first `sample` points, counted on the host from the rules the device
+    applies to each point: in the multipole zone its expansion terms (the
+    truncation rule of spt_bem_multipole); in a tile the table (two 12 x 12
+    Chebyshev sums) plus its window's segments; elsewhere every segment
+    outside far clusters plus the far clusters' expansion terms. A summed
+    segment costs its far-rule ring nodes (8 at L <= D < k4 L, 4 below k2 L,
+    2 beyond) or, closer than its length, its sinh-rule nodes (8 per
+    interval of width near_width). Returns per-point means: ring nodes, near
+    nodes, expansion terms (clusters and multipole), tables."""
     from bem_amd.bem import clusters as bcl
 
+    k4, k2 = budget.far_rule
     prof = np.asarray(geo.solution.profile)
     p0, t = prof[:-1], np.diff(prof, axis=0)
     L = np.linalg.norm(t, axis=1)
     u = t / L[:, None]
     q = pts[:sample]
+    in_mp, tile = point_classes(geo, q, budget)
     rho, z = np.hypot(q[:, 0], q[:, 1]), q[:, 2]
     dr, dz = rho[:, None] - p0[None, :, 0], z[:, None] - p0[None, :, 1]
     s0 = np.clip(dr * u[None, :, 0] + dz * u[None, :, 1], 0.0, L[None])
     D = np.hypot(dr - s0 * u[None, :, 0], dz - s0 * u[None, :, 1])
     Lr = L[None]
-    local = np.ones_like(D, dtype=bool)
+    summed = np.ones_like(D, dtype=bool)
     terms = np.zeros(len(q))
-    if clusters is not None:
-        for c in range(clusters.n_clusters):
-            r = np.hypot(rho, z - clusters.zc[c])
-            far = r >= clusters.kappa * clusters.rc[c]
-            local[:, clusters.seg_begin[c]:clusters.seg_begin[c + 1]] &= ~far[:, None]
-            qq = np.where(far, clusters.rc[c] / np.maximum(r, 1e-300), 0.0)
-            terms += np.where(far, bcl.terms_needed(qq, clusters.tol, clusters.n_terms), 0)
+    cl = budget.clusters
+    if cl is not None:
+        fallback = (tile < 0) & ~in_mp
+        for c in range(cl.n_clusters):
+            r = np.hypot(rho, z - cl.zc[c])
+            far = fallback & (r >= cl.kappa * cl.rc[c])
+            summed[:, cl.seg_begin[c]:cl.seg_begin[c + 1]] &= ~far[:, None]
+            qq = np.where(far, cl.rc[c] / np.maximum(r, 1e-300), 0.0)
+            terms += np.where(far, bcl.terms_needed(qq, cl.tol, cl.n_terms), 0)
+    if budget.tiles is not None:
+        win = budget.tiles.tiles[np.maximum(tile, 0), 3:5].astype(np.int64)
+        j = np.arange(len(L))[None, :]
+        in_win = (j >= win[:, :1]) & (j < win[:, 1:])
+        summed = np.where((tile >= 0)[:, None], in_win, summed)
+    summed &= ~in_mp[:, None]
+    mp = budget.multipole
+    if mp is not None and not mp.inert:
+        qq = np.where(in_mp, mp.r_ref / np.maximum(np.hypot(rho, z), 1e-300), 0.0)
+        terms += np.where(in_mp, bcl.terms_needed(qq, 1e-12, len(mp.m)), 0)
     ring = np.where(D < Lr, 0, np.where(D < k4 * Lr, 8, np.where(D < k2 * Lr, 4, 2)))
     d = np.maximum(D, 1e-15 * Lr)
     span = np.arcsinh((Lr - s0) / d) - np.arcsinh(-s0 / d)
-    near = np.where(D < Lr, 8 * np.maximum(1, np.ceil(span / near_width)), 0)
-    return (float(np.where(local, ring, 0).sum(1).mean()), float(np.where(local, near, 0).sum(1).mean()),
-            float(terms.mean()))
+    near = np.where(D < Lr, 8 * np.maximum(1, np.ceil(span / budget.near_width)), 0)
+    return (float(np.where(summed, ring, 0).sum(1).mean()), float(near.sum(1).mean()),
+            float(terms.mean()), float((tile >= 0).mean()))
 
 
-def bin_order(geo, pts):
-    """The order in which the binned lockstep gather takes points: by
+def bin_order(geo, pts, budget=None):
+    """The order in which the binned lockstep gather takes points, by the
+    library's bin key: points outside tiles and the multipole zone by
     distance band (floor log2 of the distance to the closest segment over
     its length, from -12 to 7) then closest segment (first index on ties),
-    as the library's bin key; ties keep the point order."""
+    then tiles by index (windowed tiles first), then the multipole zone by
+    r; ties keep the point order."""
     prof = np.asarray(geo.solution.profile)
     p0, t = prof[:-1], np.diff(prof, axis=0)
     L = np.linalg.norm(t, axis=1)
@@ -194,6 +229,13 @@ def bin_order(geo, pts):
         d = D[np.arange(len(j)), j] / L[j]
         band = np.clip(np.floor(np.log2(np.maximum(d, 1e-300))) + 12, 0, 19).astype(np.int64)
         key[a:a + 4096] = band * len(L) + j
+    if budget is not None:
+        in_mp, tile = point_classes(geo, pts, budget)
+        base = 20 * len(L)
+        key = np.where(tile >= 0, base + tile, key)
+        n_t = 0 if budget.tiles is None else budget.tiles.n_tiles
+        key = np.where(in_mp, base + n_t + np.floor(16 * np.log2(np.maximum(np.hypot(rho, z), 1e-300))).astype(
+            np.int64) + 10000, key)
     return np.argsort(key, kind="stable")
 
 
@@ -201,18 +243,29 @@ def field_gather_timing(wl, geo, n_pts=262144, reps=10, shape=None, budget=None,
     """Average launch duration of the BEM field gather over gather_points(),
     timed with HIP events on the stream the kernel is launched on (the
     library is bound to torch's current stream); with a FieldBudget, the
-    far-rule gather of the fast form (far rule, near width, clusters);
-    binned: the points in bin order, as the binned lockstep gather takes
-    them."""
+    fast form's gather (far rule, near width, clusters, tiles and the
+    multipole zone: table-only and multipole-zone points one per thread,
+    the rest 64 per block as the binned gather takes them); binned: the
+    points in bin order, as the binned lockstep gather takes them."""
     import contextlib
 
     pts = gather_points(wl, n_pts)
     if binned:
-        pts = np.ascontiguousarray(pts[bin_order(geo, pts)])
+        pts = np.ascontiguousarray(pts[bin_order(geo, pts, budget)])
     ctx = geo._ctx()
-    cm = (geo.far_rule(*budget.far_rule, near_width=budget.near_width, clusters=budget.clusters) if budget
-          else contextlib.nullcontext())
-    with cm:
+
+    @contextlib.contextmanager
+    def fast_form():
+        mp = budget.multipole
+        with geo.far_rule(*budget.far_rule, near_width=budget.near_width, clusters=budget.clusters,
+                          tiles=budget.tiles):
+            if mp is not None and not mp.inert:
+                with mp.active():
+                    yield
+            else:
+                yield
+
+    with (fast_form() if budget else contextlib.nullcontext()):
         return _gather_timing(ctx, pts, n_pts, reps, shape)
 
 
@@ -456,15 +509,20 @@ def main():
     if args.field_only:
         import torch
 
-        bud = None if args.plain_gather else bbudget.fit(geo, tol=args.tol, use_multipole=False, n_val=1 << 12)
+        bud = None if args.plain_gather else bbudget.fit(geo, tol=args.tol, n_val=1 << 12)
         field_ms, n_pts = field_gather_timing(wl, geo, reps=3, shape=shape_arg, budget=bud, binned=bud is not None)
         src = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
+        work = None
+        if bud is not None:
+            ring, near, terms, tiled = gather_work_per_point(geo, gather_points(wl), bud)
+            work = dict(ring_nodes=ring, near_nodes=near, expansion_terms=terms, tables=tiled,
+                        flops=n_pts * ((ring + near) * C_RING + terms * C_TERM + tiled * C_TABLE))
         print(json.dumps(dict(field_ms=field_ms, points=n_pts, field_shape=shape_arg,
-                              gather="fixed rule" if bud is None else "fast form", copy_bytes_read=src.numel() * 4,
-                              copy_bytes_written=dst.numel() * 4)))
+                              gather="fixed rule" if bud is None else "fast form", work_per_point=work,
+                              copy_bytes_read=src.numel() * 4, copy_bytes_written=dst.numel() * 4)))
         return
     if args.value_only:
         bud = bbudget.fit(geo, tol=args.tol)
@@ -557,29 +615,31 @@ def main():
         say(f"bench: per point in the network's domain: fast gather {cost['fast_gather_ns_per_point']:.2f} ns, "
             f"network {cost['network_fp64_ns_per_point']:.2f} ns")
     # dominant kernel of the value trace: the fast form's field gather, at the
-    # value trace's launch shape, timed with HIP events on its stream
+    # binned gather's launch shape (the tuned thread count, 64 points per
+    # block), timed with HIP events on its stream
     pts = gather_points(wl)
     n_pts = len(pts)
-    fast_shape = tuple(tr.field_shape or (0, 0))
+    fast_shape = ((tr.field_shape or (256, 64))[0], 64)
     fast_ms, _ = field_gather_timing(wl, geo, shape=fast_shape, budget=bud, binned=True)
-    ring, near, terms = gather_work_per_point(geo, pts, *bud.far_rule, near_width=bud.near_width,
-                                              clusters=bud.clusters)
-    fast_flops = n_pts * ((ring + near) * C_RING + terms * C_TERM)
+    ring, near, terms, tiled = gather_work_per_point(geo, pts, bud)
+    fast_flops = n_pts * ((ring + near) * C_RING + terms * C_TERM + tiled * C_TABLE)
     fa = fast_flops / (fast_ms / 1e3) / 1e12
     fast_roof = {"bound": "fp64_valu", "achieved": fa, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                  "frac": fa / PEAK_FP64_TFLOPS, "traffic": FAST_TRAFFIC_BYTES,
                  "traffic_source": FAST_TRAFFIC_SOURCE,
-                 "kernel": "bem_pert_kernel, far-rule instantiation with the field budget (far rule "
+                 "kernel": "bem_pert_kernel with the field budget (far rule "
                            f"{bud.far_rule}, near width {bud.near_width}, {bud.clusters.n_clusters} cluster "
-                           "expansions) on the points in bin order: the device code of the value trace's "
-                           "ls_bin_field_kernel (64 points per block, wave-major, per-lane rules)",
+                           f"expansions, {bud.tiles.n_tiles} tiles, the multipole zone) on the points in bin "
+                           "order: the device code of the value trace's ls_bin_field_kernel (64 points per "
+                           "block, wave-major, per-lane rules; table-only and multipole-zone points one per "
+                           "thread)",
                  "kernel_ms_mean": fast_ms, "points_per_launch": n_pts, "field_shape": list(fast_shape),
-                 "traffic_field_shape": [256, 64],
+                 "traffic_field_shape": list(FAST_TRAFFIC_SHAPE),
                  "ring_nodes_per_point": ring, "near_nodes_per_point": near, "expansion_terms_per_point": terms,
-                 "flops_per_launch": fast_flops,
+                 "tables_per_point": tiled, "flops_per_launch": fast_flops,
                  "note": "FP64 VALU-bound (MI355X FP64 vector peak = FP64 MFMA peak = 78.6 TFLOP/s); algorithmic "
-                         "flops = (far-rule ring nodes on segments outside far clusters + near-rule nodes) x 120 "
-                         "+ cluster-expansion terms x 16, counted on the host for the same points"}
+                         "flops = (far-rule ring nodes of the segments each point sums + near-rule nodes) x 120 "
+                         "+ expansion terms x 16 + tile tables x 576, counted on the host for the same points"}
     plain_shape = tuple(plain_tr.field_shape or (0, 0))
     field_ms, _ = field_gather_timing(wl, geo, shape=plain_shape)
     n_quad = wl["n_segments"] * 8 * 4

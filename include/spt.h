@@ -377,9 +377,12 @@ int spt_image_force(spt_ctx* ctx, const spt_geometry* geom, const double* pos, c
                     double* F);
 
 /* Launch shape of the BEM field gather (no counterpart in the reference;
- * set by the autotuner in bem_amd/autotune.py): threads per block 192 or
- * 256 and points per block (a power of two <= 64); 0 keeps the automatic
- * rule (about three resident blocks per compute unit, whole rounds). */
+ * set by the autotuner in bem_amd/autotune.py): threads per block 64, 128,
+ * 192 or 256 and points per block (a power of two <= 64); 0 keeps the
+ * automatic rule (about three resident blocks per compute unit, whole
+ * rounds). 64 and 128 threads apply to the binned lockstep gather and the
+ * FP64 spt_bem_pert_field (one or two slices per point at 64 points per
+ * block); the other gathers take 256 for them. */
 int spt_set_field_shape(spt_ctx* ctx, int32_t threads, int32_t points_per_block);
 
 /* Binned gather of far-rule lockstep traces (no counterpart in the
