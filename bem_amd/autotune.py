@@ -39,10 +39,12 @@ def candidates(tracer, n_cu):
     biggest = int(sizes.max())
     n = int(sizes.sum())
     out = []
-    binned = bool(getattr(tracer, "binned", 0)) and tracer.far_rule is not None and tracer.surrogate is None \
+    # a surrogate with no validated band is never installed (the trace runs without it)
+    live_surrogate = tracer.surrogate is not None and not tracer.surrogate.inert
+    binned = bool(getattr(tracer, "binned", 0)) and tracer.far_rule is not None and not live_surrogate \
         and tracer.precision != "mixed"
     image = getattr(tracer.geometry, "image_profile", None) is not None
-    if tracer.surrogate is None and not binned:
+    if not live_surrogate and not binned:
         for T in (64, 128, 256):
             if biggest > T or (tracer.precision == "mixed" and T != 256) or (image and T != 256):
                 continue
