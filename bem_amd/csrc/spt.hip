@@ -277,6 +277,62 @@ __device__ __forceinline__ void near_nodes(double re, double re2, double ze, con
   }
 }
 
+// Pair form of a segment's near-rule nodes, for far-rule gathers: the
+// Gauss-8 points are symmetric, so nodes p and 7 - p of interval k sit at
+// v_k -+ |g_p| half about the interval's midpoint v_k, and
+// exp(v) = exp(v_k) exp(-+|g_p| half): one exp per interval and one per
+// pair instead of one per node. The products round within a few ulp of
+// exp(v), which moves a node by ~1e-16 of its distance. Items m = 4k + p
+// (interval k, pair p of nodes p and 7 - p); this thread takes m = i0,
+// i0 + stride, ... with stride 2 or 4, so the same one or two pairs in
+// every interval.
+template <bool AXIS>
+__device__ __forceinline__ void near_pair_nodes(double re, double re2, double ze, const NearSeg& sg, const NearRule& r,
+                                                int i0, int stride, RingAcc& acc, const double2* lt) {
+  const int np = 4 / stride;
+  double e[2], ei[2], wk[2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    if (a < np) {
+      const int p = i0 + a * stride;
+      e[a] = exp(c_gx8[p] * r.half);  // g_p < 0: node p is below the midpoint
+      ei[a] = rcp64(e[a]);
+      wk[a] = (2 * kPi) * r.d * r.half * c_gw8[p];
+    }
+  }
+  auto node = [&](double ev, double iv, double wq) {
+    const double sh = 0.5 * (ev - iv), ch = 0.5 * (ev + iv);
+    const double s = r.s0 + r.d * sh;
+    const double sr = sg.p0r + s * sg.ur;
+    const double sz = sg.p0z + s * sg.uz;
+    const double dens = sg.sig0 + (s * sg.inv_L) * sg.dsig;
+    const double w = dens * sr * ch * wq;
+    if (AXIS)
+      ring_accum(re, re2, ze - sz, sr, w, acc, lt);
+    else
+      ring_accum_main(re, re2, ze - sz, sr, sr * sr, w, acc, lt);
+  };
+  for (int k = 0; k < r.count; ++k) {
+    const double c = exp(r.lo + (k + 0.5) * r.step), ci = rcp64(c);
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      if (a < np) {
+        node(c * e[a], ci * ei[a], wk[a]);
+        node(c * ei[a], ci * e[a], wk[a]);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void near_pairs(double re, double re2, double ze, const double* g, double sig0, double sig1,
+                                           const NearRule& r, int i0, int stride, RingAcc& acc, const double2* lt) {
+  const NearSeg sg = near_seg(g, sig0, sig1);
+  if (ring_axis_possible(re2, ze, g[S_P0R], g[S_P0Z], g[S_ABR], g[S_ABZ]))
+    near_pair_nodes<true>(re, re2, ze, sg, r, i0, stride, acc, lt);
+  else
+    near_pair_nodes<false>(re, re2, ze, sg, r, i0, stride, acc, lt);
+}
+
 // Whole near segment on one thread; its sum comes back by value (the
 // caller adds it), so the caller's accumulator never goes to memory.
 __device__ __noinline__ RingAcc near_segment(double re, double re2, double ze, const double* g, double sig0,
@@ -952,6 +1008,9 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
     // otherwise fall on the one or two slices holding the point's closest
     // segments.
     int base = 0;
+    // far-rule gathers with 2 or 4 slices per point take the pair form:
+    // every slice then holds the same pairs of each near segment
+    const bool pairs = FAR && (S == 2 || S == 4);
     auto near_slice = [&](int t) {
       const int c = sh.u.n.cnt[t];
       for (int k = 0; k < c; ++k) {
@@ -960,8 +1019,12 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
         double proj, dist;
         segment_proj(re, ze, g, proj, dist);
         const NearRule r = near_rule(g, proj, dist, nw);
-        near_nodes(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], r, (s - base % S + S) % S, S, acc, sh.logt);
-        base += 8 * r.count;
+        if (pairs) {
+          near_pairs(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], r, su, S, acc, sh.logt);
+        } else {
+          near_nodes(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], r, (s - base % S + S) % S, S, acc, sh.logt);
+          base += 8 * r.count;
+        }
       }
     };
     if (wave_major) {
