@@ -116,10 +116,14 @@ struct BemView {
   // segments tl_win[t].x .. tl_win[t].y - 1 with the far-rule gather's
   // rules and takes every other segment's fixed-rule field from the tile's
   // table: kTileP x kTileP Chebyshev coefficients of E_rho, then of E_z,
-  // over the cell (tl_coef + t * kTileCoef).
+  // over the cell (tl_coef + t * kTileCoef). tl_cell[t]: tile t's cell
+  // (low rho, low z, side) as the walk leaves them; tiles tl_nwin .. tl_n - 1
+  // have empty windows.
   const int* tl_tree;
   const int2* tl_win;
   const double* tl_coef;
+  const double4* tl_cell;
+  int tl_n, tl_nwin;
   double tl_x0, tl_z0, tl_size;
 };
 constexpr int kTileP = 12;                         // Chebyshev degree + 1 per direction
@@ -575,6 +579,25 @@ __device__ __forceinline__ int tile_find(const BemView& B, double rho, double z,
     k = v + qx + 2 * qz;
   }
   return -1;
+}
+
+// tile_find with a guess: when (rho, z) lies inside tile t's cell by more
+// than 1e-10 of its side (far beyond the walk's rounding, so the walk would
+// end there too), t and the cell as tile_find gives them, from one load;
+// otherwise the walk. t < 0: no guess.
+__device__ __forceinline__ int tile_find_hint(const BemView& B, int t, double rho, double z, double& cx, double& cz,
+                                              double& h) {
+  if (t >= 0 && t < B.tl_n) {
+    const double4 c = B.tl_cell[t];
+    const double m = 1e-10 * c.z;
+    if (rho >= c.x + m && rho < c.x + c.z - m && z >= c.y + m && z < c.y + c.z - m) {
+      h = 0.5 * c.z;
+      cx = c.x + h;
+      cz = c.y + h;
+      return t;
+    }
+  }
+  return tile_find(B, rho, z, cx, cz, h);
 }
 
 // Rows a = s, s + S, ... of tile t's Chebyshev sums at (u, v) in [-1, 1]^2
@@ -2033,8 +2056,8 @@ struct BinView {
   unsigned* hist;    // n_keys: eligible particles per bin (zero between attempts)
   unsigned* cursor;  // n_keys: next free slot per bin
   unsigned* total;   // slots in use this attempt (grows by particles that need a gather without a slot)
-  double4* rin;      // per slot: stage position, kind (bits: particle index >= 0 gather, -2 - index
-                     // multipole zone, -1 nothing)
+  double4* rin;      // per slot: stage position, kind (bits: particle index >= 0 gather, else the
+                     // kRec* tags)
   double4* rout;     // per slot: perturbation, bin (bits)
   int n_keys, nsk, shift;
   int key_tile0;     // first tile bin (tile t: key_tile0 + t), after the distance-band bins
@@ -2304,7 +2327,13 @@ __global__ __launch_bounds__(kBinThreads) void ls_bin_scan_kernel(LockParams L) 
 // particle's index), else one of these.
 constexpr long long kRecNone = -1;       // no field needed this stage
 constexpr long long kRecMultipole = -2;  // multipole zone, one record per thread
-constexpr long long kRecTable = -3;      // a table-only tile, one record per thread
+constexpr long long kRecTable = -3;      // kRecTable - t: table-only tile t, one record per thread
+
+// The tile a bin key names (key_tile0 + t), or -1: a guess for tile_find_hint.
+__device__ __forceinline__ int bin_tile(const BinView& X, int key) {
+  const int t = key - X.key_tile0;
+  return t >= 0 && key < X.key_mp0 ? t : -1;
+}
 
 // Stage st's field for a particle of a running group whose stage position
 // (x, y, z) is in slots 48..50: written here where the BEM perturbation is
@@ -2312,18 +2341,25 @@ constexpr long long kRecTable = -3;      // a table-only tile, one record per th
 // region), else the position goes into the particle's slot record (a slot
 // at the end of the list for a particle without one), marked for the
 // gather, for the multipole zone or for a table-only tile. Every slot of
-// the attempt gets its record at every stage.
-__device__ void ls_bin_pack(const LockParams& L, int64_t gi, int st, double x, double y, double z) {
+// the attempt gets its record at every stage. sl: the particle's slot (-1:
+// none yet); active: its flag; hint: the tile of its last gather (bin_tile)
+// or -1.
+__device__ void ls_bin_pack(const LockParams& L, int64_t gi, int st, double x, double y, double z, int sl, bool active,
+                            int hint) {
   const BinView& X = L.X;
-  int sl = X.slot[gi];
   int kind = 0;  // 1 gather, 2 multipole zone, 3 table-only tile
-  if (L.active[gi]) {
+  int tile = -1;
+  if (active) {
     if (st == 0 && L.fvalid[gi]) {  // the field at the state is known
       for (int c = 0; c < 3; ++c) LKB(51 + c) = LKB(54 + c);
     } else if (bem_needed(L.T.G, x, y, z)) {
-      double cx, cz, h;
-      kind = (L.T.bem.mp && multipole_zone(L.T.bem, x, y, z)) ? 2
-             : (tile_only(L.T.bem, x, y, z, cx, cz, h) >= 0 ? 3 : 1);
+      if (L.T.bem.mp && multipole_zone(L.T.bem, x, y, z)) {
+        kind = 2;
+      } else {
+        double cx, cz, h;
+        tile = L.T.bem.tl_tree ? tile_find_hint(L.T.bem, hint, hypot(x, y), z, cx, cz, h) : -1;
+        kind = tile >= L.T.bem.tl_nwin ? 3 : 1;  // tiles past tl_nwin: empty windows
+      }
     } else {
       double ex, ey, ez;
       geometry_field_point(L.T.G, x, y, z, 0.0, 0.0, 0.0, ex, ey, ez);
@@ -2357,8 +2393,8 @@ __device__ void ls_bin_pack(const LockParams& L, int64_t gi, int st, double x, d
       sl = (int)atomicAdd(X.total, 1u);
       X.slot[gi] = sl;
     }
-    X.rin[sl] = make_double4(x, y, z,
-                             __longlong_as_double(kind == 1 ? (long long)gi : (kind == 2 ? kRecMultipole : kRecTable)));
+    X.rin[sl] = make_double4(
+        x, y, z, __longlong_as_double(kind == 1 ? (long long)gi : (kind == 2 ? kRecMultipole : kRecTable - tile)));
   } else if (sl >= 0) {
     X.rin[sl].w = __longlong_as_double(kRecNone);
   }
@@ -2382,8 +2418,10 @@ __global__ __launch_bounds__(kBinThreads) void ls_bin_slot_kernel(LockParams L) 
     if (cnt[kk]) cnt[kk] = atomicAdd(&X.cursor[kk], cnt[kk]);
   __syncthreads();
   if (!mine) return;
-  if (e) X.slot[gi] = (int)(cnt[k] + r);
-  if (L.g_flags[L.pgroup[gi]] & LS_RUNNING) ls_bin_pack(L, gi, 0, LKB(48), LKB(49), LKB(50));
+  const int sl = e ? (int)(cnt[k] + r) : X.slot[gi];
+  if (e) X.slot[gi] = sl;
+  if (L.g_flags[L.pgroup[gi]] & LS_RUNNING)
+    ls_bin_pack(L, gi, 0, LKB(48), LKB(49), LKB(50), sl, L.active[gi], bin_tile(X, e ? k : X.key[gi]));
 }
 
 // Stage st's slot records. Blocks from nb_gather on: the multipole-zone and
@@ -2417,13 +2455,12 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_bin_field_kernel(Lo
     if (w == kRecMultipole && multipole_pert(L.T.bem, r.x, r.y, r.z, qx, qy, qz)) {
       X.rout[i] = make_double4(
           qx, qy, qz, __longlong_as_double((long long)bin_key_mp(X, L.T.bem, fma(r.x, r.x, fma(r.y, r.y, r.z * r.z)))));
-    } else if (w == kRecTable) {
-      double cx, cz, h;
-      const int t = tile_only(L.T.bem, r.x, r.y, r.z, cx, cz, h);
-      if (t >= 0) {
-        tile_point(L.T.bem, t, r.x, r.y, r.z, cx, cz, h, qx, qy, qz);
-        X.rout[i] = make_double4(qx, qy, qz, __longlong_as_double((long long)(X.key_tile0 + t)));
-      }
+    } else if (w <= kRecTable && kRecTable - w < (long long)L.T.bem.tl_n) {
+      const int t = (int)(kRecTable - w);  // the packing placed the point in this table-only tile
+      const double4 c = L.T.bem.tl_cell[t];
+      const double h = 0.5 * c.z;
+      tile_point(L.T.bem, t, r.x, r.y, r.z, c.x + h, c.y + h, h, qx, qy, qz);
+      X.rout[i] = make_double4(qx, qy, qz, __longlong_as_double((long long)(X.key_tile0 + t)));
     }
     return;
   }
@@ -2652,41 +2689,64 @@ __global__ __launch_bounds__(TG) void ls_accel_kernel(LockParams L, int st) {
   __shared__ int act[TG];
   const int gs = threadIdx.x / W;
   const int g = blockIdx.x * GPB + gs;
-  const bool run = g < L.T.n_groups && (L.g_flags[g] & LS_RUNNING);
+  // the group's flags and extent, loaded together
+  const bool in = g < L.T.n_groups;
+  const int flags = in ? L.g_flags[g] : 0;
+  const int64_t o0 = in ? L.T.offsets[g] : 0, o1 = in ? L.T.offsets[g + 1] : 0;
+  const bool run = flags & LS_RUNNING;
   if (GPB == 1 && !run) return;  // block-uniform
   const int tid = threadIdx.x - gs * W;
-  const int64_t off = run ? L.T.offsets[g] : 0;
-  const int n = run ? (int)(L.T.offsets[g + 1] - off) : 0;
+  const int64_t off = run ? o0 : 0;
+  const int n = run ? (int)(o1 - o0) : 0;
   const bool mine = tid < n;
   const int64_t gi = off + tid;
   const int lt = gs * W + tid;  // this thread's LDS slot
+  // Everything the stage reads of the particle is loaded up front, so the
+  // loads overlap instead of following one another through the kernel.
+  const bool binned = L.X.key != nullptr;
+  double hr = 0.0, mass = 1.0, p0[3] = {0.0, 0.0, 0.0}, v0[3] = {0.0, 0.0, 0.0}, kprev[5][3];
+  bool active = false;
+  int sl = -1, hint = -1;
   if (mine) {
+    hr = L.g_hr[g];
+    mass = L.T.mass[gi];
+    active = L.active[gi];
+    if (st < 6) {
+      for (int c = 0; c < 3; ++c) {
+        p0[c] = L.T.pos[3 * gi + c];
+        v0[c] = L.T.vel[3 * gi + c];
+      }
+#pragma unroll
+      for (int j = 0; j < 5; ++j)
+        for (int c = 0; c < 3; ++c) kprev[j][c] = j < st ? LKB(j * 6 + 3 + c) : 0.0;
+    }
     sx[lt] = LKB(48);
     sy[lt] = LKB(49);
     sz[lt] = LKB(50);
     sq[lt] = L.T.charge[gi];
-    act[lt] = L.active[gi];
-    if (L.X.key && L.X.src[gi]) {  // the binned gather's output for this stage
-      const double4 r = L.X.rout[L.X.slot[gi]];
+    act[lt] = active;
+    const bool src = binned && L.X.src[gi];
+    if (binned) sl = L.X.slot[gi];
+    if (src) {  // the binned gather's output for this stage
+      const double4 r = L.X.rout[sl];
       double ex, ey, ez;
       geometry_field_point(L.T.G, sx[lt], sy[lt], sz[lt], r.x, r.y, r.z, ex, ey, ez);
       LKB(51) = ex;
       LKB(52) = ey;
       LKB(53) = ez;
       ls_keep_field(L, gi, st);
-      L.X.key[gi] = (int)__double_as_longlong(r.w);
+      const int key = (int)__double_as_longlong(r.w);
+      L.X.key[gi] = key;
+      hint = bin_tile(L.X, key);
     }
   }
   __syncthreads();
   if (!mine) return;
-  const bool active = act[lt];
-  const double hr = L.g_hr[g];
   double Fc[3], Fi[3];
   const int b = gs * W;
   pair_forces_p<PR>(L.T.G, tid, n, sx + b, sy + b, sz + b, sq + b, act + b, L.T.a2, Fc, Fi);
   if (L.imgF)
     for (int c = 0; c < 3; ++c) Fi[c] = L.imgF[3 * gi + c];
-  const double mass = L.T.mass[gi];
   double kst[3];  // this stage's k (velocity part): the acceleration
   for (int c = 0; c < 3; ++c) {
     const double ac = n == 1 ? 0.0 : Fc[c] / mass;
@@ -2698,29 +2758,27 @@ __global__ __launch_bounds__(TG) void ls_accel_kernel(LockParams L, int st) {
     // The position part of stage j's k is stage j's velocity, not stored:
     // formed again from the stored accelerations with the same products in
     // the same order as when stage j was made (so bit for bit the same).
+    double ps[3];  // the next stage's position
     for (int c = 0; c < 3; ++c) {
-      const double p0 = L.T.pos[3 * gi + c], v0 = L.T.vel[3 * gi + c];
+      const double p0c = p0[c], v0c = v0[c];
       double kv[6], kp[6];
 #pragma unroll
       for (int j = 0; j < 6; ++j) {
-        kv[j] = j < st ? LKB(j * 6 + 3 + c) : (j == st ? kst[c] : 0.0);
-        double v = v0;
+        kv[j] = j < st ? kprev[j < 5 ? j : 4][c] : (j == st ? kst[c] : 0.0);
+        double v = v0c;
 #pragma unroll
         for (int i = 0; i < j; ++i) v = v + (hr * c_A[j][i]) * kv[i];
         kp[j] = active ? v : 0.0;
       }
       if (st < 5) {
         const int ns = st + 1;
-        double ps = p0, vs = v0;
+        double pc = p0c;
 #pragma unroll
         for (int j = 0; j < 5; ++j) {
-          if (j < ns) {
-            const double ha = hr * c_A[ns][j];
-            ps = ps + ha * kp[j];
-            vs = vs + ha * kv[j];
-          }
+          if (j < ns) pc = pc + (hr * c_A[ns][j]) * kp[j];
         }
-        LKB(48 + c) = ps;
+        ps[c] = pc;
+        LKB(48 + c) = pc;
       } else {
         double sp = c_B[0] * kp[0], sv = c_B[0] * kv[0];
 #pragma unroll
@@ -2728,14 +2786,15 @@ __global__ __launch_bounds__(TG) void ls_accel_kernel(LockParams L, int st) {
           sp = sp + c_B[k] * kp[k];
           sv = sv + c_B[k] * kv[k];
         }
-        const double pn = p0 + hr * sp;
-        const double vn = v0 + hr * sv;
+        const double pn = p0c + hr * sp;
+        const double vn = v0c + hr * sv;
         LKB(42 + c) = pn;
         LKB(45 + c) = vn;
         LKB(48 + c) = pn;
+        ps[c] = pn;
       }
     }
-    if (L.X.key) ls_bin_pack(L, gi, st + 1, LKB(48), LKB(49), LKB(50));
+    if (binned) ls_bin_pack(L, gi, st + 1, ps[0], ps[1], ps[2], sl, active, hint);
   }
 }
 
@@ -3104,6 +3163,7 @@ struct spt_ctx {
   int* d_tl_tree = nullptr;
   int2* d_tl_win = nullptr;
   double* d_tl_coef = nullptr;
+  double4* d_tl_cell = nullptr;
   int tl_n = 0;
   int tl_nwin = 0;  // tiles with a window (numbered first)
   bool tl_on = false;
@@ -3216,6 +3276,9 @@ static BemView bem_view(const spt_ctx* c) {
   B.tl_tree = tl ? c->d_tl_tree : nullptr;
   B.tl_win = tl ? c->d_tl_win : nullptr;
   B.tl_coef = tl ? c->d_tl_coef : nullptr;
+  B.tl_cell = tl ? c->d_tl_cell : nullptr;
+  B.tl_n = tl ? c->tl_n : 0;
+  B.tl_nwin = tl ? c->tl_nwin : 0;
   B.tl_x0 = c->tl_x0;
   B.tl_z0 = c->tl_z0;
   B.tl_size = c->tl_size;
@@ -3226,9 +3289,11 @@ static void free_tiles(spt_ctx* c) {
   if (c->d_tl_tree) (void)hipFree(c->d_tl_tree);
   if (c->d_tl_win) (void)hipFree(c->d_tl_win);
   if (c->d_tl_coef) (void)hipFree(c->d_tl_coef);
+  if (c->d_tl_cell) (void)hipFree(c->d_tl_cell);
   c->d_tl_tree = nullptr;
   c->d_tl_win = nullptr;
   c->d_tl_coef = nullptr;
+  c->d_tl_cell = nullptr;
   c->tl_n = 0;
   c->tl_on = false;
 }
@@ -3500,19 +3565,56 @@ int spt_bem_tiles(spt_ctx* c, const int32_t* tree, int32_t n_tree, const int32_t
   for (int t = 0; t < n_tiles; ++t)
     if (win[2 * t] < 0 || win[2 * t + 1] < win[2 * t] || win[2 * t + 1] > n_seg)
       return fail(c, SPT_EINVAL, "spt_bem_tiles: window of tile " + std::to_string(t) + " out of range");
+  int nwin = 0;  // tiles with a window come first
+  while (nwin < n_tiles && win[2 * nwin + 1] > win[2 * nwin]) ++nwin;
+  for (int t = nwin; t < n_tiles; ++t)
+    if (win[2 * t + 1] > win[2 * t])
+      return fail(c, SPT_EINVAL, "spt_bem_tiles: tiles with a window must come first (tile " + std::to_string(t) + ")");
+  // every tile's cell with the device walk's arithmetic (tile_find: halve
+  // the side, then add it to the low corner), so a cell lookup and a walk
+  // agree to the bit; a tile reached twice or never is an error
+  std::vector<double4> cell(n_tiles);
+  std::vector<char> seen(n_tiles, 0);
+  {
+    struct Item { int k; double x0, z0, s; };
+    std::vector<Item> stack{{0, x0, z0, size}};
+    while (!stack.empty()) {
+      const Item it = stack.back();
+      stack.pop_back();
+      const int v = tree[it.k];
+      if (v <= -2) {
+        const int t = -2 - v;
+        if (seen[t]) return fail(c, SPT_EINVAL, "spt_bem_tiles: tile " + std::to_string(t) + " in two cells");
+        seen[t] = 1;
+        cell[t] = make_double4(it.x0, it.z0, it.s, 0.0);
+      } else if (v >= 0) {
+        double s = it.s;
+        s *= 0.5;
+        for (int q = 0; q < 4; ++q) {
+          double cx0 = it.x0, cz0 = it.z0;
+          if (q & 1) cx0 += s;
+          if (q & 2) cz0 += s;
+          stack.push_back({v + q, cx0, cz0, s});
+        }
+      }
+    }
+  }
+  for (int t = 0; t < n_tiles; ++t)
+    if (!seen[t]) return fail(c, SPT_EINVAL, "spt_bem_tiles: tile " + std::to_string(t) + " in no cell");
   HIPCHK(c, spt_begin(c));
   free_tiles(c);
   HIPCHK(c, hipMalloc(&c->d_tl_tree, sizeof(int) * n_tree));
   HIPCHK(c, hipMalloc(&c->d_tl_win, sizeof(int2) * n_tiles));
   HIPCHK(c, hipMalloc(&c->d_tl_coef, sizeof(double) * kTileCoef * (size_t)n_tiles));
+  HIPCHK(c, hipMalloc(&c->d_tl_cell, sizeof(double4) * n_tiles));
+  HIPCHK(c, hipMemcpyAsync(c->d_tl_cell, cell.data(), sizeof(double4) * n_tiles, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_tl_tree, tree, sizeof(int) * n_tree, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_tl_win, win, sizeof(int2) * n_tiles, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_tl_coef, coef, sizeof(double) * kTileCoef * (size_t)n_tiles, hipMemcpyHostToDevice,
                            c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->tl_n = n_tiles;
-  c->tl_nwin = 0;
-  while (c->tl_nwin < n_tiles && win[2 * c->tl_nwin + 1] > win[2 * c->tl_nwin]) ++c->tl_nwin;
+  c->tl_nwin = nwin;
   c->tl_x0 = x0;
   c->tl_z0 = z0;
   c->tl_size = size;

@@ -7,6 +7,8 @@
 * every fitted table matches the direct sum inside its cell within 1e-8 of
   |E_gun|, and the device's table-only points reproduce the host's
   Chebyshev evaluation of the same tables (1e-13);
+* spt_bem_tiles refuses tables whose windowed tiles do not come first
+  or whose tree places a tile in two cells;
 * the tiled far-rule gather stays within 3e-8 of |E| of the plain fixed
   rule from 1e-3 panel lengths to 110 um off the cap, a permutation of
   points in windowed tiles permutes the results bit for bit (the binned
@@ -137,6 +139,38 @@ def test_tiled_gather_is_order_independent(post, tiles, shape):
     finally:
         ctx.lib.spt_set_field_shape(ctx.h, 0, 0)
     np.testing.assert_array_equal(b, a[perm])
+
+
+def test_upload_rejects_bad_tables(post, tiles):
+    """spt_bem_tiles checks what the device walk and the cell lookup trust:
+    windowed tiles first, every tile in exactly one cell."""
+    import ctypes
+
+    from bem_amd import _lib
+
+    ctx = post._ctx()
+    Pp = ctypes.c_void_p
+
+    def upload(tree, win):
+        tree = np.ascontiguousarray(tree, dtype=np.int32)
+        win = np.ascontiguousarray(win, dtype=np.int32)
+        return ctx.lib.spt_bem_tiles(ctx.h, tree.ctypes.data_as(Pp), tree.shape[0], win.ctypes.data_as(Pp),
+                                     tiles.coef.ctypes.data_as(Pp), tiles.n_tiles, tiles.x0, tiles.z0, tiles.size)
+
+    tree, win = tiles.tree.copy(), tiles.win.copy()
+    empty = int(np.argmax(win[:, 1] <= win[:, 0]))
+    try:
+        bad = win.copy()
+        bad[[0, empty]] = bad[[empty, 0]]  # an empty window ahead of a windowed tile
+        assert upload(tree, bad) == _lib.SPT_EINVAL
+        twice = tree.copy()
+        leaves = np.nonzero(twice <= -2)[0]
+        twice[leaves[1]] = twice[leaves[0]]  # one tile in two cells, another in none
+        assert upload(twice, win) == _lib.SPT_EINVAL
+        assert upload(tree, win) == _lib.SPT_OK
+    finally:
+        ctx.tiles_key = None  # the next install uploads again
+        ctx.lib.spt_bem_tiles_enable(ctx.h, 0)
 
 
 def test_tiles_off_restores_the_gather(post, tiles):
