@@ -126,6 +126,7 @@ struct BemView {
   int tl_n, tl_nwin;
   double tl_x0, tl_z0, tl_size;
 };
+constexpr int kPackThreads = 256;  // threads of a block of packed 16-lane groups (lockstep accel and end kernels)
 constexpr int kTileP = 12;                         // Chebyshev degree + 1 per direction
 constexpr int kTileCoef = 2 * kTileP * kTileP;     // doubles per tile
 constexpr int kTileMaxDepth = 24;
@@ -2056,9 +2057,10 @@ struct BinView {
   unsigned* hist;    // n_keys: eligible particles per bin (zero between attempts)
   unsigned* cursor;  // n_keys: next free slot per bin
   unsigned* total;   // slots in use this attempt (grows by particles that need a gather without a slot)
-  double4* rin;      // per slot: stage position, kind (bits: particle index >= 0 gather, else the
-                     // kRec* tags)
-  double4* rout;     // per slot: perturbation, bin (bits)
+  double2* rin;      // per slot: stage position (rho, z)
+  int32_t* rtag;     // per slot: record kind (kRecGather, or the other kRec* tags)
+  double2* rout;     // per slot: perturbation (E_rho, E_z)
+  int32_t* rkey;     // per slot: the point's bin
   int n_keys, nsk, shift;
   int key_tile0;     // first tile bin (tile t: key_tile0 + t), after the distance-band bins
   int key_mp0;       // first multipole-zone bin (n_keys when the zone is off)
@@ -2123,7 +2125,7 @@ __device__ __forceinline__ double group_sum(double v, double* red) {
   if constexpr (GPB == 1) {
     return block_sum<TG>(v, red);
   } else {
-    static_assert(TG == 64 && GPB == 4, "four 16-lane groups per wavefront");
+    static_assert(TG / GPB == 16 && TG % 64 == 0, "16-lane groups, four per wavefront");
 #pragma unroll
     for (int s = 8; s > 0; s >>= 1) v = v + __shfl_down(v, s, 16);
     return __shfl(v, 0, 16);
@@ -2134,7 +2136,7 @@ __device__ __forceinline__ double group_min(double v, double* red) {
   if constexpr (GPB == 1) {
     return block_min<TG>(v, red);
   } else {
-    static_assert(TG == 64 && GPB == 4, "four 16-lane groups per wavefront");
+    static_assert(TG / GPB == 16 && TG % 64 == 0, "16-lane groups, four per wavefront");
 #pragma unroll
     for (int s = 8; s > 0; s >>= 1) v = fmin(v, __shfl_down(v, s, 16));
     return __shfl(v, 0, 16);
@@ -2323,11 +2325,13 @@ __global__ __launch_bounds__(kBinThreads) void ls_bin_scan_kernel(LockParams L) 
   }
 }
 
-// Slot record tags (the fourth word's bits): >= 0 the gather (the
-// particle's index), else one of these.
-constexpr long long kRecNone = -1;       // no field needed this stage
-constexpr long long kRecMultipole = -2;  // multipole zone, one record per thread
-constexpr long long kRecTable = -3;      // kRecTable - t: table-only tile t, one record per thread
+// Slot record tags. Records hold (rho, z): the binned gather works in the
+// meridian plane and returns (E_rho, E_z); the acceleration kernel turns
+// them around the axis with the stage position's own x / rho and y / rho.
+constexpr int kRecGather = 0;     // the gather (64 or 16 records per block)
+constexpr int kRecNone = -1;      // no field needed this stage
+constexpr int kRecMultipole = -2; // multipole zone, one record per thread
+constexpr int kRecTable = -3;     // kRecTable - t: table-only tile t, one record per thread
 
 // The tile a bin key names (key_tile0 + t), or -1: a guess for tile_find_hint.
 __device__ __forceinline__ int bin_tile(const BinView& X, int key) {
@@ -2353,7 +2357,8 @@ __device__ void ls_bin_pack(const LockParams& L, int64_t gi, int st, double x, d
     if (st == 0 && L.fvalid[gi]) {  // the field at the state is known
       for (int c = 0; c < 3; ++c) LKB(51 + c) = LKB(54 + c);
     } else if (bem_needed(L.T.G, x, y, z)) {
-      if (L.T.bem.mp && multipole_zone(L.T.bem, x, y, z)) {
+      // the zone test on (rho, 0, z), as the field kernel evaluates it
+      if (L.T.bem.mp && multipole_zone(L.T.bem, hypot(x, y), 0.0, z)) {
         kind = 2;
       } else {
         double cx, cz, h;
@@ -2393,10 +2398,10 @@ __device__ void ls_bin_pack(const LockParams& L, int64_t gi, int st, double x, d
       sl = (int)atomicAdd(X.total, 1u);
       X.slot[gi] = sl;
     }
-    X.rin[sl] = make_double4(
-        x, y, z, __longlong_as_double(kind == 1 ? (long long)gi : (kind == 2 ? kRecMultipole : kRecTable - tile)));
+    X.rin[sl] = make_double2(hypot(x, y), z);
+    X.rtag[sl] = kind == 1 ? kRecGather : (kind == 2 ? kRecMultipole : kRecTable - tile);
   } else if (sl >= 0) {
-    X.rin[sl].w = __longlong_as_double(kRecNone);
+    X.rtag[sl] = kRecNone;
   }
   X.src[gi] = kind ? 1 : 0;
 }
@@ -2449,18 +2454,20 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_bin_field_kernel(Lo
   if ((int)blockIdx.x >= nb_gather) {
     const int64_t i = (int64_t)(blockIdx.x - nb_gather) * T + tid;
     if (i >= tot) return;
-    const double4 r = X.rin[i];
-    const long long w = __double_as_longlong(r.w);
+    const int w = X.rtag[i];
+    if (w >= kRecNone) return;
+    const double2 r = X.rin[i];  // (rho, z): the point at azimuth 0, where E_x = E_rho and E_y = 0
     double qx, qy, qz;
-    if (w == kRecMultipole && multipole_pert(L.T.bem, r.x, r.y, r.z, qx, qy, qz)) {
-      X.rout[i] = make_double4(
-          qx, qy, qz, __longlong_as_double((long long)bin_key_mp(X, L.T.bem, fma(r.x, r.x, fma(r.y, r.y, r.z * r.z)))));
-    } else if (w <= kRecTable && kRecTable - w < (long long)L.T.bem.tl_n) {
-      const int t = (int)(kRecTable - w);  // the packing placed the point in this table-only tile
+    if (w == kRecMultipole && multipole_pert(L.T.bem, r.x, 0.0, r.y, qx, qy, qz)) {
+      X.rout[i] = make_double2(qx, qz);
+      X.rkey[i] = bin_key_mp(X, L.T.bem, fma(r.x, r.x, r.y * r.y));
+    } else if (w <= kRecTable && kRecTable - w < L.T.bem.tl_n) {
+      const int t = kRecTable - w;  // the packing placed the point in this table-only tile
       const double4 c = L.T.bem.tl_cell[t];
       const double h = 0.5 * c.z;
-      tile_point(L.T.bem, t, r.x, r.y, r.z, c.x + h, c.y + h, h, qx, qy, qz);
-      X.rout[i] = make_double4(qx, qy, qz, __longlong_as_double((long long)(X.key_tile0 + t)));
+      tile_point(L.T.bem, t, r.x, 0.0, r.y, c.x + h, c.y + h, h, qx, qy, qz);
+      X.rout[i] = make_double2(qx, qz);
+      X.rkey[i] = X.key_tile0 + t;
     }
     return;
   }
@@ -2472,10 +2479,10 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_bin_field_kernel(Lo
       int sl = -1;
       if (base + tid < ng) {
         sl = X.glist[base + tid];
-        const double4 r = X.rin[sl];
+        const double2 r = X.rin[sl];
         f.px[tid] = r.x;
-        f.py[tid] = r.y;
-        f.pz[tid] = r.z;
+        f.py[tid] = 0.0;
+        f.pz[tid] = r.y;
       }
       f.cidx[tid] = sl;
       f.need[tid] = sl >= 0;
@@ -2483,8 +2490,10 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_bin_field_kernel(Lo
     __syncthreads();
     bem_pert_block<T, 16, SPT_PREC_FP64, true, true>(L.T.bem, 16, f.px, f.py, f.pz, f.ox, f.oy, f.oz, f.bem, f.need,
                                                      okey, X.nsk, X.shift, X.key_tile0);
-    if (tid < 16 && f.need[tid])
-      X.rout[f.cidx[tid]] = make_double4(f.ox[tid], f.oy[tid], f.oz[tid], __longlong_as_double((long long)okey[tid]));
+    if (tid < 16 && f.need[tid]) {
+      X.rout[f.cidx[tid]] = make_double2(f.ox[tid], f.oz[tid]);
+      X.rkey[f.cidx[tid]] = okey[tid];
+    }
     return;
   }
   FieldShared<T, 64>& f = sh.wide;
@@ -2493,22 +2502,22 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_bin_field_kernel(Lo
   const int m = (int)min((int64_t)64, tot - base);
   if (tid < 64) {
     int v = 0;
-    if (tid < m) {
-      const double4 r = X.rin[base + tid];
-      if (__double_as_longlong(r.w) >= 0) {
-        v = 1;
-        f.px[tid] = r.x;
-        f.py[tid] = r.y;
-        f.pz[tid] = r.z;
-      }
+    if (tid < m && X.rtag[base + tid] >= kRecGather) {
+      const double2 r = X.rin[base + tid];
+      v = 1;
+      f.px[tid] = r.x;
+      f.py[tid] = 0.0;
+      f.pz[tid] = r.y;
     }
     f.need[tid] = v;
   }
   if (!__syncthreads_or(tid < 64 && f.need[tid])) return;
   bem_pert_block<T, 64, SPT_PREC_FP64, true, true>(L.T.bem, 64, f.px, f.py, f.pz, f.ox, f.oy, f.oz, f.bem, f.need,
                                                    okey, X.nsk, X.shift, X.key_tile0);
-  if (tid < m && f.need[tid])
-    X.rout[base + tid] = make_double4(f.ox[tid], f.oy[tid], f.oz[tid], __longlong_as_double((long long)okey[tid]));
+  if (tid < m && f.need[tid]) {
+    X.rout[base + tid] = make_double2(f.ox[tid], f.oz[tid]);
+    X.rkey[base + tid] = okey[tid];
+  }
 }
 
 // External field at the stage positions of every active particle of every
@@ -2707,6 +2716,7 @@ __global__ __launch_bounds__(TG) void ls_accel_kernel(LockParams L, int st) {
   double hr = 0.0, mass = 1.0, p0[3] = {0.0, 0.0, 0.0}, v0[3] = {0.0, 0.0, 0.0}, kprev[5][3];
   bool active = false;
   int sl = -1, hint = -1;
+  double fe[3] = {0.0, 0.0, 0.0};  // this stage's field
   if (mine) {
     hr = L.g_hr[g];
     mass = L.T.mass[gi];
@@ -2727,17 +2737,24 @@ __global__ __launch_bounds__(TG) void ls_accel_kernel(LockParams L, int st) {
     act[lt] = active;
     const bool src = binned && L.X.src[gi];
     if (binned) sl = L.X.slot[gi];
-    if (src) {  // the binned gather's output for this stage
-      const double4 r = L.X.rout[sl];
-      double ex, ey, ez;
-      geometry_field_point(L.T.G, sx[lt], sy[lt], sz[lt], r.x, r.y, r.z, ex, ey, ez);
-      LKB(51) = ex;
-      LKB(52) = ey;
-      LKB(53) = ez;
-      ls_keep_field(L, gi, st);
-      const int key = (int)__double_as_longlong(r.w);
+    if (src) {
+      // the binned gather's output for this stage (kept in registers; only
+      // stage 1's and stage 7's fields are stored, as ls_keep_field keeps them)
+      const double2 r = L.X.rout[sl];
+      const double x = sx[lt], y = sy[lt], rho = hypot(x, y);
+      const double qx = rho > 0.0 ? r.x * (x / rho) : 0.0, qy = rho > 0.0 ? r.x * (y / rho) : 0.0;
+      geometry_field_point(L.T.G, x, y, sz[lt], qx, qy, r.y, fe[0], fe[1], fe[2]);
+      if (st == 0) {
+        for (int c = 0; c < 3; ++c) LKB(54 + c) = fe[c];
+        L.fvalid[gi] = 1;
+      } else if (st == 6) {
+        for (int c = 0; c < 3; ++c) LKB(57 + c) = fe[c];
+      }
+      const int key = L.X.rkey[sl];
       L.X.key[gi] = key;
       hint = bin_tile(L.X, key);
+    } else {
+      for (int c = 0; c < 3; ++c) fe[c] = LKB(51 + c);
     }
   }
   __syncthreads();
@@ -2750,7 +2767,7 @@ __global__ __launch_bounds__(TG) void ls_accel_kernel(LockParams L, int st) {
   double kst[3];  // this stage's k (velocity part): the acceleration
   for (int c = 0; c < 3; ++c) {
     const double ac = n == 1 ? 0.0 : Fc[c] / mass;
-    const double a = L.T.q_over_m * LKB(51 + c) + ac + Fi[c] / mass;
+    const double a = L.T.q_over_m * fe[c] + ac + Fi[c] / mass;
     kst[c] = active ? a : 0.0;
     LKB(st * 6 + 3 + c) = kst[c];
   }
@@ -3194,7 +3211,7 @@ struct spt_ctx {
   int32_t* d_bin_key = nullptr;
   int32_t* d_bin_slot = nullptr;
   uint8_t* d_bin_src = nullptr;
-  double4* d_bin_rec = nullptr;   // rin then rout (n each)
+  double2* d_bin_rec = nullptr;   // rin, rout (n double2 each), then rtag, rkey (n int32 each)
   int32_t* d_bin_glist = nullptr; // the compact list of gather records (n)
   size_t bin_cap = 0;
   // surrogate used by spt_trace
@@ -4000,7 +4017,9 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
   if (L.T.n_out > 0) reset_u64_kernel<<<1, 1, 0, s>>>(L.T.snap_count);
   ls_init_kernel<TG><<<n_groups, TG, 0, s>>>(L);
   const bool bin = L.X.key != nullptr;
-  const bool pack4 = TG == 64 && max_size <= 16;  // four groups per accel block
+  // groups of at most 16 share the acceleration and end-of-attempt blocks,
+  // kPackThreads / 16 of them per block (16 lanes each)
+  const bool pack4 = TG == 64 && max_size <= 16;
   const unsigned nb_bin = (unsigned)((n + kBinThreads - 1) / kBinThreads);
   const unsigned nb_slots = (unsigned)((n + 63) / 64);
   // blocks for the records evaluated one per thread (multipole zone, table-only tiles)
@@ -4050,12 +4069,14 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
           if (r) return r;
         }
         if (pack4)
-          ls_accel_kernel<64, APR, 4><<<(unsigned)((n_groups + 3) / 4), 64, 0, s>>>(L, st);
+          ls_accel_kernel<kPackThreads, APR, kPackThreads / 16>
+              <<<(unsigned)((n_groups + kPackThreads / 16 - 1) / (kPackThreads / 16)), kPackThreads, 0, s>>>(L, st);
         else
           ls_accel_kernel<TG, APR><<<n_groups, TG, 0, s>>>(L, st);
       }
       if (pack4)
-        ls_end_kernel<64, 4><<<(unsigned)((n_groups + 3) / 4), 64, 0, s>>>(L);
+        ls_end_kernel<kPackThreads, kPackThreads / 16>
+            <<<(unsigned)((n_groups + kPackThreads / 16 - 1) / (kPackThreads / 16)), kPackThreads, 0, s>>>(L);
       else
         ls_end_kernel<TG><<<n_groups, TG, 0, s>>>(L);
     }
@@ -4116,7 +4137,7 @@ static int ensure_bin_scratch(spt_ctx* c, int64_t n) {
     HIPCHK(c, hipMalloc(&c->d_bin_slot, sizeof(int32_t) * n));
     HIPCHK(c, hipMalloc(&c->d_bin_glist, sizeof(int32_t) * n));
     HIPCHK(c, hipMalloc(&c->d_bin_src, n));
-    HIPCHK(c, hipMalloc(&c->d_bin_rec, sizeof(double4) * 2 * n));
+    HIPCHK(c, hipMalloc(&c->d_bin_rec, (2 * sizeof(double2) + 2 * sizeof(int32_t)) * n));
     c->bin_cap = n;
   }
   return SPT_OK;
@@ -4332,6 +4353,8 @@ static int trace_impl(spt_ctx* c, const spt_trace_args* a, bool no_alloc) {
       L.X.key_tab0 = L.X.key_tile0 + (P.bem.tl_tree ? c->tl_nwin : 0);
       L.X.rin = c->d_bin_rec;
       L.X.rout = c->d_bin_rec + a->n_particles;
+      L.X.rtag = reinterpret_cast<int32_t*>(c->d_bin_rec + 2 * a->n_particles);
+      L.X.rkey = L.X.rtag + a->n_particles;
       L.X.append_only = c->bin_mode == 2;
     }
     memset(&L.V, 0, sizeof(L.V));
