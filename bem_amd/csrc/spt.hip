@@ -760,7 +760,8 @@ __device__ __forceinline__ int bin_key(int bseg, double bd2, double L, int nsk, 
 template <int T, int NMAX, int PR, bool FAR, bool BIN = false>
 __device__ void bem_pert_block(const BemView& B, int m, const double* px, const double* py, const double* pz,
                                double* ox, double* oy, double* oz, BemShared<T, NMAX>& sh, const int* valid = nullptr,
-                               int* okey = nullptr, int key_nsk = 0, int key_shift = 0, int key_tile0 = 0) {
+                               int* okey = nullptr, int key_nsk = 0, int key_shift = 0, int key_tile0 = 0,
+                               const int* ptile = nullptr) {
   static_assert(T % 64 == 0, "whole wavefronts");
   static_assert(!BIN || (FAR && (NMAX == 64 || NMAX == 16)), "the binned gather is a far-rule gather of 64 or 16 points");
   if (m <= 0) return;
@@ -781,7 +782,19 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
   int tleaf = -1, jlo = 0, jhi = B.n_seg;
   double tcx = 0.0, tcz = 0.0, th = 1.0;
   if (tiles && on) {
-    tleaf = tile_find(B, hypot(px[p], py[p]), pz[p], tcx, tcz, th);
+    if (BIN && ptile && ptile[p] < B.tl_n) {
+      // the tile the packing found for this very position (tile_find's
+      // result; the cell as the walk leaves it)
+      tleaf = ptile[p];
+      if (tleaf >= 0) {
+        const double4 c = B.tl_cell[tleaf];
+        th = 0.5 * c.z;
+        tcx = c.x + th;
+        tcz = c.y + th;
+      }
+    } else {
+      tleaf = tile_find(B, hypot(px[p], py[p]), pz[p], tcx, tcz, th);
+    }
     if (tleaf >= 0) {
       const int2 w = B.tl_win[tleaf];
       jlo = w.x;
@@ -1892,6 +1905,7 @@ struct FieldShared {
   double ox[NMAX], oy[NMAX], oz[NMAX];
   int cidx[NMAX];
   int need[NMAX];
+  int tile[NMAX];  // binned gather: each point's tile from the packing (-1: none)
   int m;
   BemShared<T, NMAX> bem;
 };
@@ -2328,7 +2342,8 @@ __global__ __launch_bounds__(kBinThreads) void ls_bin_scan_kernel(LockParams L) 
 // Slot record tags. Records hold (rho, z): the binned gather works in the
 // meridian plane and returns (E_rho, E_z); the acceleration kernel turns
 // them around the axis with the stage position's own x / rho and y / rho.
-constexpr int kRecGather = 0;     // the gather (64 or 16 records per block)
+constexpr int kRecGather = 0;     // kRecGather + 1 + t: the gather (64 or 16 records per block), point in
+                                  // windowed tile t (t = -1: in no tile)
 constexpr int kRecNone = -1;      // no field needed this stage
 constexpr int kRecMultipole = -2; // multipole zone, one record per thread
 constexpr int kRecTable = -3;     // kRecTable - t: table-only tile t, one record per thread
@@ -2399,7 +2414,7 @@ __device__ void ls_bin_pack(const LockParams& L, int64_t gi, int st, double x, d
       X.slot[gi] = sl;
     }
     X.rin[sl] = make_double2(hypot(x, y), z);
-    X.rtag[sl] = kind == 1 ? kRecGather : (kind == 2 ? kRecMultipole : kRecTable - tile);
+    X.rtag[sl] = kind == 1 ? kRecGather + 1 + tile : (kind == 2 ? kRecMultipole : kRecTable - tile);
   } else if (sl >= 0) {
     X.rtag[sl] = kRecNone;
   }
@@ -2483,13 +2498,14 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_bin_field_kernel(Lo
         f.px[tid] = r.x;
         f.py[tid] = 0.0;
         f.pz[tid] = r.y;
+        f.tile[tid] = X.rtag[sl] - kRecGather - 1;
       }
       f.cidx[tid] = sl;
       f.need[tid] = sl >= 0;
     }
     __syncthreads();
     bem_pert_block<T, 16, SPT_PREC_FP64, true, true>(L.T.bem, 16, f.px, f.py, f.pz, f.ox, f.oy, f.oz, f.bem, f.need,
-                                                     okey, X.nsk, X.shift, X.key_tile0);
+                                                     okey, X.nsk, X.shift, X.key_tile0, f.tile);
     if (tid < 16 && f.need[tid]) {
       X.rout[f.cidx[tid]] = make_double2(f.ox[tid], f.oz[tid]);
       X.rkey[f.cidx[tid]] = okey[tid];
@@ -2502,18 +2518,20 @@ __global__ __launch_bounds__(T, SPT_FIELD_MIN_WAVES) void ls_bin_field_kernel(Lo
   const int m = (int)min((int64_t)64, tot - base);
   if (tid < 64) {
     int v = 0;
-    if (tid < m && X.rtag[base + tid] >= kRecGather) {
+    const int tg = tid < m ? X.rtag[base + tid] : kRecNone;
+    if (tg >= kRecGather) {
       const double2 r = X.rin[base + tid];
       v = 1;
       f.px[tid] = r.x;
       f.py[tid] = 0.0;
       f.pz[tid] = r.y;
+      f.tile[tid] = tg - kRecGather - 1;
     }
     f.need[tid] = v;
   }
   if (!__syncthreads_or(tid < 64 && f.need[tid])) return;
   bem_pert_block<T, 64, SPT_PREC_FP64, true, true>(L.T.bem, 64, f.px, f.py, f.pz, f.ox, f.oy, f.oz, f.bem, f.need,
-                                                   okey, X.nsk, X.shift, X.key_tile0);
+                                                   okey, X.nsk, X.shift, X.key_tile0, f.tile);
   if (tid < m && f.need[tid]) {
     X.rout[base + tid] = make_double2(f.ox[tid], f.oz[tid]);
     X.rkey[base + tid] = okey[tid];
