@@ -46,7 +46,7 @@ class FieldBudget:
 
 def fit(geometry, tol=1e-6, far_rule=(2.0, 16.0), near_width=2.0, seg_per_cluster=50, kappa=1.2,
         multipole_terms=128, multipole_tol=None, n_val=1 << 16, use_clusters=True, use_multipole=True,
-        cluster_terms=128, use_tiles=True, tile_alpha=1.0, tile_w_max=5):
+        cluster_terms=128, use_tiles=True, tile_alpha=0.25, tile_w_max=4, zone_factor=2.5):
     """Build and validate a budget for a protrusion BEM geometry (the post or
     the hemispherical tip). Raises ValueError when the validated far-rule
     gather (far rule + near width + clusters + tiles) misses tol.
@@ -62,7 +62,17 @@ def fit(geometry, tol=1e-6, far_rule=(2.0, 16.0), near_width=2.0, seg_per_cluste
     1.15 or 70 .. 100 segments: 88 .. 90 ms; near width 2.5: 6.6e-8; far
     rule (2, 10): 1.7e-7). Tiles cover the quarter plane rho, z >= 0 up to
     the multipole zone's radius (the profile's extent without it), windows
-    of at most tile_w_max segments within tile_alpha cell sides."""
+    of at most tile_w_max segments within tile_alpha cell sides. The zone
+    starts at zone_factor r_ref or at its validated radius, whichever is
+    larger: just beyond r_ref the expansion needs all its terms (q =
+    r_ref / r near 1), where one table per point is cheaper. Tile defaults
+    from a sweep of the same trace (ms per trace): zone 1.083 r_ref (the
+    validated radius), w_max 5, alpha 1: 53.5; zone 2.5 r_ref: 50.2; then
+    w_max 4: 49.5, alpha 0.5: 47.9, alpha 0.25: 47.4, alpha 0.15: 46.9
+    (gather error 1.9e-8 against 7.1e-9); w_max 3 leaves cells without a
+    tile (84 ms); the box size moves the quadtree's cell edges against the
+    cap (zone 2.0 / 4.0 / 8.0 r_ref: 53.5 .. 53.9, 3.0 / 6.0: 51.1 .. 51.3,
+    2.5 / 5.0: 49.4 .. 50.2 at w_max 5, alpha 1)."""
     from ..surrogate import far_rule_error
 
     cl = (_clusters.fit(geometry, seg_per_cluster=seg_per_cluster, n_terms=cluster_terms, kappa=kappa)
@@ -74,6 +84,12 @@ def fit(geometry, tol=1e-6, far_rule=(2.0, 16.0), near_width=2.0, seg_per_cluste
     tl = None
     if use_tiles:
         live_mp = mp is not None and not mp.inert
+        if live_mp:
+            # the expansion stays valid beyond its validated radius
+            mp.report["r_min_validated"] = mp.r_min
+            mp.r_min = max(mp.r_min, zone_factor * mp.r_ref)
+            mp.report["r_min"] = mp.r_min
+            mp.report["r_min_over_r_ref"] = mp.r_min / mp.r_ref
         tl = _tiles.fit(geometry, size=mp.r_min if live_mp else None, r_drop=mp.r_min if live_mp else None,
                         alpha=tile_alpha, w_max=tile_w_max)
     err = far_rule_error(geometry, *far_rule, n=n_val, near_width=near_width, clusters=cl, tiles=tl)

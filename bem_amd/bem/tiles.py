@@ -76,12 +76,15 @@ def _inside(prof, px, pz):
     return bool(np.count_nonzero(cross & (px < xi)) % 2)
 
 
-def build_tree(prof, x0, z0, size, alpha=1.0, w_max=5, s_min=None, r_drop=np.inf):
+def build_tree(prof, x0, z0, size, alpha=1.0, w_max=5, s_min=None, r_drop=np.inf, split=frozenset()):
     """Quadtree over [x0, x0 + size) x [z0, z0 + size), children visited in
     the library's order (low/high rho, then low/high z) so tiles are
     numbered along a Z curve. Returns (tree int32, tiles (n, 5): cell x0,
     z0, side, jlo, jhi). A cell whose window still exceeds w_max at side
-    s_min (default an eighth of the shortest segment) gets no tile."""
+    s_min (default an eighth of the shortest segment) gets no tile; cells
+    listed in split (x0, z0, side tuples, as this walk forms them) are
+    divided even when their window is small enough (a table that missed
+    its tolerance)."""
     prof = np.asarray(prof, dtype=np.float64)
     p0, p1 = prof[:-1], prof[1:]
     L = np.hypot(*(p1 - p0).T)
@@ -101,7 +104,7 @@ def build_tree(prof, x0, z0, size, alpha=1.0, w_max=5, s_min=None, r_drop=np.inf
         # error outside it
         win = np.nonzero(dist < np.maximum(alpha * s, L) + 1e-9 * s)[0]
         lo, hi = (int(win.min()), int(win.max()) + 1) if win.size else (0, 0)
-        if hi - lo <= w_max:
+        if hi - lo <= w_max and (cx0, cz0, s) not in split:
             if closed and dist.min() > 0.0 and _inside(prof, cx0 + 0.5 * s, cz0 + 0.5 * s):
                 tree[k] = -1  # wholly inside the conductor
                 return
@@ -260,19 +263,40 @@ class Tiles:
         return out
 
 
-def fit(geometry, x0=0.0, z0=0.0, size=None, alpha=1.0, w_max=5, r_drop=None, n_check=8, tol=1e-8, seed=3):
+def fit(geometry, x0=0.0, z0=0.0, size=None, alpha=1.0, w_max=5, r_drop=None, n_check=8, tol=1e-8, seed=3,
+        refine=16):
     """Tiles for a BEM geometry's charge over [x0, x0 + size) x [z0, z0 +
     size) (size defaults to 1.05 times the profile's largest coordinate),
     tables from the device (spt_bem_remainder at every tile's Chebyshev
     points), each table checked at n_check random points of its cell
-    against the direct sum; raises ValueError if any misses tol relative
-    to |E_gun|."""
+    against the direct sum. A cell whose table misses tol relative to
+    |E_gun| is split and its children tabulated again, up to refine rounds
+    (large cells away from the surface, where the whole charge is near
+    enough to spoil a 12 x 12 table); raises ValueError if a table still
+    misses."""
     prof, sigma, nsub = geometry.bem_table()
     prof = np.asarray(prof, dtype=np.float64)
     if size is None:
         size = 1.05 * float(np.max(np.abs(prof)))
     r_drop = np.inf if r_drop is None else float(r_drop)
-    tree, tiles = build_tree(prof, x0, z0, size, alpha=alpha, w_max=w_max, r_drop=r_drop)
+    split = set()
+    for rounds in range(refine + 1):
+        out, worst = _fit_tables(geometry, prof, x0, z0, size, alpha, w_max, r_drop, n_check, seed, split)
+        bad = np.nonzero(~(worst <= tol))[0]
+        if bad.size == 0 or rounds == refine:
+            break
+        split |= {tuple(float(v) for v in out.tiles[t, :3]) for t in bad}
+    out.report["refine_rounds"] = rounds
+    out.report["cells_split_for_tables"] = len(split)
+    if not worst.max() <= tol:
+        raise ValueError(f"tile tables miss {tol:g} of |E_gun|: worst {worst.max():.3e} in tile {int(worst.argmax())}")
+    return out
+
+
+def _fit_tables(geometry, prof, x0, z0, size, alpha, w_max, r_drop, n_check, seed, split):
+    """One tree and its tables; returns the Tiles and each table's worst
+    error at its check points (relative to |E_gun|)."""
+    tree, tiles = build_tree(prof, x0, z0, size, alpha=alpha, w_max=w_max, r_drop=r_drop, split=frozenset(split))
     n = tiles.shape[0]
     if n == 0:
         raise ValueError("no tiles: the box holds no cell outside the conductor")
@@ -302,7 +326,5 @@ def fit(geometry, x0=0.0, z0=0.0, size=None, alpha=1.0, w_max=5, r_drop=None, n_
                       box=[float(x0), float(z0), float(size)], r_drop=None if np.isinf(r_drop) else r_drop,
                       window_counts=counts.tolist(), table_err_max=float(worst.max()),
                       table_err_median=float(np.median(worst)), table_bytes=int(coef.nbytes),
-                      smallest_cell=float(tiles[:, 2].min()))
-    if not worst.max() <= tol:
-        raise ValueError(f"tile tables miss {tol:g} of |E_gun|: worst {worst.max():.3e} in tile {int(worst.argmax())}")
-    return out
+                      smallest_cell=float(tiles[:, 2].min()), largest_cell=float(tiles[:, 2].max()))
+    return out, worst
