@@ -655,18 +655,26 @@ __device__ __forceinline__ int tile_only(const BemView& B, double x, double y, d
 }
 
 // Perturbation at a point of a table-only tile t (tile_only), by one
-// thread: every row of the table, then the axis rule of the gather.
+// thread: every row of the table, then the axis rule of the gather. The
+// table comes in by scalar loads where the wavefront's points share it
+// (neighbours in the binned slot order do).
 __device__ __forceinline__ void tile_point(const BemView& B, int t, double x, double y, double z, double cx, double cz,
                                            double h, double& ox, double& oy, double& oz) {
   const double rho = hypot(x, y), ih = 1.0 / h;
   RingAcc acc;
   acc.zero();
-  tile_rows(B.tl_coef + (size_t)t * kTileCoef, (rho - cx) * ih, (z - cz) * ih, 0, 1, acc);
+  const double u = (rho - cx) * ih, v = (z - cz) * ih;
+  const int t0 = __builtin_amdgcn_readfirstlane(t);
+  if (__ballot(t != t0) == 0ull)
+    tile_rows((cseg_ptr)B.tl_coef + (size_t)t0 * kTileCoef, u, v, 0, 1, acc);
+  else
+    tile_rows(B.tl_coef + (size_t)t * kTileCoef, u, v, 0, 1, acc);
   double er, ez;
   ring_finish(acc, rho, er, ez);
-  const double phi = atan2(y, x);
-  ox = rho == 0.0 ? 0.0 : er * cos(phi);
-  oy = rho == 0.0 ? 0.0 : er * sin(phi);
+  // the azimuth's cosine and sine as x / rho and y / rho (exactly 1 and 0
+  // for a point given as (rho, 0, z))
+  ox = rho == 0.0 ? 0.0 : er * (x / rho);
+  oy = rho == 0.0 ? 0.0 : er * (y / rho);
   oz = ez;
 }
 
@@ -876,9 +884,14 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
     sh.re[p] = re;
     sh.ze[p] = z + off * g[S_NZ];
     sh.rho[p] = rho;
-    const double phi = atan2(y, x);
-    sh.cph[p] = cos(phi);
-    sh.sph[p] = sin(phi);
+    if (FAR) {  // x / rho and y / rho: exactly 1 and 0 for the binned gather's (rho, 0, z) points
+      sh.cph[p] = rho == 0.0 ? 1.0 : x / rho;
+      sh.sph[p] = rho == 0.0 ? 0.0 : y / rho;
+    } else {
+      const double phi = atan2(y, x);
+      sh.cph[p] = cos(phi);
+      sh.sph[p] = sin(phi);
+    }
   }
   __syncthreads();
   RingAcc acc;
