@@ -44,7 +44,7 @@ class FieldBudget:
                     multipole=None if self.multipole is None or self.multipole.inert else self.multipole)
 
 
-def fit(geometry, tol=1e-6, far_rule=(2.0, 16.0), near_width=2.0, seg_per_cluster=50, kappa=1.2,
+def fit(geometry, tol=1e-6, far_rule=(2.0, 16.0), near_width=2.5, seg_per_cluster=50, kappa=1.2,
         multipole_terms=128, multipole_tol=None, n_val=1 << 16, use_clusters=True, use_multipole=True,
         cluster_terms=128, use_tiles=True, tile_alpha=0.25, tile_w_max=4, zone_factor=2.5):
     """Build and validate a budget for a protrusion BEM geometry (the post or
@@ -56,8 +56,9 @@ def fit(geometry, tol=1e-6, far_rule=(2.0, 16.0), near_width=2.0, seg_per_cluste
     expansion length against the validated error and the trace time):
     clusters of up to 50 segments within each straight or turning run of
     the profile (the cap is one), each about its smallest enclosing sphere,
-    expanded beyond 1.2 r_c with up to 128 terms; near width 2; far rule
-    (2, 16): field error 1.8e-8 of |E|, 86 ms per trace without tiles (35
+    expanded beyond 1.2 r_c with up to 128 terms; near width 2 (2.5 with
+    the tiles, below); far rule (2, 16): field error 1.8e-8 of |E|, 86 ms
+    per trace without tiles (35
     segments beyond 1.35 r_c about their z-midpoints: 92 ms; kappa 1.1 ..
     1.15 or 70 .. 100 segments: 88 .. 90 ms; near width 2.5: 6.6e-8; far
     rule (2, 10): 1.7e-7). Tiles cover the quarter plane rho, z >= 0 up to
@@ -72,7 +73,10 @@ def fit(geometry, tol=1e-6, far_rule=(2.0, 16.0), near_width=2.0, seg_per_cluste
     (gather error 1.9e-8 against 7.1e-9); w_max 3 leaves cells without a
     tile (84 ms); the box size moves the quadtree's cell edges against the
     cap (zone 2.0 / 4.0 / 8.0 r_ref: 53.5 .. 53.9, 3.0 / 6.0: 51.1 .. 51.3,
-    2.5 / 5.0: 49.4 .. 50.2 at w_max 5, alpha 1)."""
+    2.5 / 5.0: 49.4 .. 50.2 at w_max 5, alpha 1). With these tiles, near
+    width 2.5 against 2: 44.7 against 46.7 ms at a validated gather error
+    of 6.6e-8 against 7.1e-9 (width 3: 44.0 ms at 4.8e-7, half the
+    metric's tolerance; not taken)."""
     from ..surrogate import far_rule_error
 
     cl = (_clusters.fit(geometry, seg_per_cluster=seg_per_cluster, n_terms=cluster_terms, kappa=kappa)
