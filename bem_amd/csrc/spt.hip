@@ -2722,7 +2722,12 @@ __global__ __launch_bounds__(64) void ls_surrogate_kernel(LockParams L, int st) 
 // most TG / GPB particles per block (GPB > 1 packs small groups so a
 // wavefront's lanes are not mostly idle; each group's arithmetic is the
 // same as with GPB = 1).
-template <int TG, int PR, int GPB = 1>
+template <int TG, int GPB>
+__device__ void ls_end_group(const LockParams& L, int g, int flags, int tid, double* red);
+
+// LAST: stage 7 (st = 6), followed by the end of the attempt in the same
+// kernel (ls_end_group) instead of a launch of ls_end_kernel.
+template <int TG, int PR, int GPB = 1, bool LAST = false>
 __global__ __launch_bounds__(TG) void ls_accel_kernel(LockParams L, int st) {
   constexpr int W = TG / GPB;  // lanes per group
   __shared__ double sx[TG], sy[TG], sz[TG], sq[TG];
@@ -2789,76 +2794,76 @@ __global__ __launch_bounds__(TG) void ls_accel_kernel(LockParams L, int st) {
     }
   }
   __syncthreads();
-  if (!mine) return;
-  double Fc[3], Fi[3];
-  const int b = gs * W;
-  pair_forces_p<PR>(L.T.G, tid, n, sx + b, sy + b, sz + b, sq + b, act + b, L.T.a2, Fc, Fi);
-  if (L.imgF)
-    for (int c = 0; c < 3; ++c) Fi[c] = L.imgF[3 * gi + c];
-  double kst[3];  // this stage's k (velocity part): the acceleration
-  for (int c = 0; c < 3; ++c) {
-    const double ac = n == 1 ? 0.0 : Fc[c] / mass;
-    const double a = L.T.q_over_m * fe[c] + ac + Fi[c] / mass;
-    kst[c] = active ? a : 0.0;
-    LKB(st * 6 + 3 + c) = kst[c];
-  }
-  if (st < 6) {
-    // The position part of stage j's k is stage j's velocity, not stored:
-    // formed again from the stored accelerations with the same products in
-    // the same order as when stage j was made (so bit for bit the same).
-    double ps[3];  // the next stage's position
+  if (!LAST && !mine) return;
+  if (mine) {
+    double Fc[3], Fi[3];
+    const int b = gs * W;
+    pair_forces_p<PR>(L.T.G, tid, n, sx + b, sy + b, sz + b, sq + b, act + b, L.T.a2, Fc, Fi);
+    if (L.imgF)
+      for (int c = 0; c < 3; ++c) Fi[c] = L.imgF[3 * gi + c];
+    double kst[3];  // this stage's k (velocity part): the acceleration
     for (int c = 0; c < 3; ++c) {
-      const double p0c = p0[c], v0c = v0[c];
-      double kv[6], kp[6];
-#pragma unroll
-      for (int j = 0; j < 6; ++j) {
-        kv[j] = j < st ? kprev[j < 5 ? j : 4][c] : (j == st ? kst[c] : 0.0);
-        double v = v0c;
-#pragma unroll
-        for (int i = 0; i < j; ++i) v = v + (hr * c_A[j][i]) * kv[i];
-        kp[j] = active ? v : 0.0;
-      }
-      if (st < 5) {
-        const int ns = st + 1;
-        double pc = p0c;
-#pragma unroll
-        for (int j = 0; j < 5; ++j) {
-          if (j < ns) pc = pc + (hr * c_A[ns][j]) * kp[j];
-        }
-        ps[c] = pc;
-        LKB(48 + c) = pc;
-      } else {
-        double sp = c_B[0] * kp[0], sv = c_B[0] * kv[0];
-#pragma unroll
-        for (int k = 1; k < 6; ++k) {
-          sp = sp + c_B[k] * kp[k];
-          sv = sv + c_B[k] * kv[k];
-        }
-        const double pn = p0c + hr * sp;
-        const double vn = v0c + hr * sv;
-        LKB(42 + c) = pn;
-        LKB(45 + c) = vn;
-        LKB(48 + c) = pn;
-        ps[c] = pn;
-      }
+      const double ac = n == 1 ? 0.0 : Fc[c] / mass;
+      const double a = L.T.q_over_m * fe[c] + ac + Fi[c] / mass;
+      kst[c] = active ? a : 0.0;
+      LKB(st * 6 + 3 + c) = kst[c];
     }
-    if (binned) ls_bin_pack(L, gi, st + 1, ps[0], ps[1], ps[2], sl, active, hint);
+    if (st < 6) {
+      // The position part of stage j's k is stage j's velocity, not stored:
+      // formed again from the stored accelerations with the same products in
+      // the same order as when stage j was made (so bit for bit the same).
+      double ps[3];  // the next stage's position
+      for (int c = 0; c < 3; ++c) {
+        const double p0c = p0[c], v0c = v0[c];
+        double kv[6], kp[6];
+  #pragma unroll
+        for (int j = 0; j < 6; ++j) {
+          kv[j] = j < st ? kprev[j < 5 ? j : 4][c] : (j == st ? kst[c] : 0.0);
+          double v = v0c;
+  #pragma unroll
+          for (int i = 0; i < j; ++i) v = v + (hr * c_A[j][i]) * kv[i];
+          kp[j] = active ? v : 0.0;
+        }
+        if (st < 5) {
+          const int ns = st + 1;
+          double pc = p0c;
+  #pragma unroll
+          for (int j = 0; j < 5; ++j) {
+            if (j < ns) pc = pc + (hr * c_A[ns][j]) * kp[j];
+          }
+          ps[c] = pc;
+          LKB(48 + c) = pc;
+        } else {
+          double sp = c_B[0] * kp[0], sv = c_B[0] * kv[0];
+  #pragma unroll
+          for (int k = 1; k < 6; ++k) {
+            sp = sp + c_B[k] * kp[k];
+            sv = sv + c_B[k] * kv[k];
+          }
+          const double pn = p0c + hr * sp;
+          const double vn = v0c + hr * sv;
+          LKB(42 + c) = pn;
+          LKB(45 + c) = vn;
+          LKB(48 + c) = pn;
+          ps[c] = pn;
+        }
+      }
+      if (binned) ls_bin_pack(L, gi, st + 1, ps[0], ps[1], ps[2], sl, active, hint);
+    }
   }
+  // every lane of a running group, together: the end's reductions are
+  // shuffles over the group's lanes
+  if (LAST && run) ls_end_group<TG, GPB>(L, g, flags, tid, sx);
 }
 
 // Error norm, accept/reject, commit, birth reset, kill, screens (reference
 // batched_rk45.py:250-317, 372-399), then the group's next attempt.
-template <int TG, int GPB = 1>
-__global__ __launch_bounds__(TG) void ls_end_kernel(LockParams L) {
-  constexpr int W = TG / GPB;  // lanes per group
-  __shared__ double red[TG];
-  const int gs = threadIdx.x / W;
-  const int g = blockIdx.x * GPB + gs;
-  const int flags = g < L.T.n_groups ? L.g_flags[g] : 0;
+// The end of group g's attempt, on the group's lanes (tid: the lane's
+// member index). With GPB > 1 every lane of a running group must be here
+// (its reductions are 16-lane shuffles); with GPB = 1 the whole block.
+template <int TG, int GPB>
+__device__ void ls_end_group(const LockParams& L, int g, int flags, int tid, double* red) {
   const bool live = flags & LS_RUNNING;
-  if (GPB == 1 && !live) return;  // block-uniform
-  if (GPB > 1 && !__syncthreads_or(live)) return;
-  const int tid = threadIdx.x - gs * W;
   const int64_t off = live ? L.T.offsets[g] : 0;
   const int n = live ? (int)(L.T.offsets[g + 1] - off) : 1;
   const bool mine = live && tid < n;
@@ -2956,6 +2961,19 @@ __global__ __launch_bounds__(TG) void ls_end_kernel(LockParams L) {
   }
   if (!running) return;
   ls_begin<TG, GPB>(L, g, n, tid, t, h, batch + 1, red);
+}
+
+template <int TG, int GPB = 1>
+__global__ __launch_bounds__(TG) void ls_end_kernel(LockParams L) {
+  constexpr int W = TG / GPB;  // lanes per group
+  __shared__ double red[TG];
+  const int gs = threadIdx.x / W;
+  const int g = blockIdx.x * GPB + gs;
+  const int flags = g < L.T.n_groups ? L.g_flags[g] : 0;
+  const bool live = flags & LS_RUNNING;
+  if (GPB == 1 && !live) return;  // block-uniform
+  if (GPB > 1 && !__syncthreads_or(live)) return;
+  ls_end_group<TG, GPB>(L, g, flags, threadIdx.x - gs * W, red);
 }
 #undef LKB
 
@@ -4099,17 +4117,16 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
                                            L.g_flags, LS_RUNNING, n_groups, L.T.a2, (double*)L.imgF, s);
           if (r) return r;
         }
-        if (pack4)
-          ls_accel_kernel<kPackThreads, APR, kPackThreads / 16>
-              <<<(unsigned)((n_groups + kPackThreads / 16 - 1) / (kPackThreads / 16)), kPackThreads, 0, s>>>(L, st);
+        // packed groups: stage 7's kernel also ends the attempt
+        const unsigned nb_pack = (unsigned)((n_groups + kPackThreads / 16 - 1) / (kPackThreads / 16));
+        if (pack4 && st == 6)
+          ls_accel_kernel<kPackThreads, APR, kPackThreads / 16, true><<<nb_pack, kPackThreads, 0, s>>>(L, st);
+        else if (pack4)
+          ls_accel_kernel<kPackThreads, APR, kPackThreads / 16><<<nb_pack, kPackThreads, 0, s>>>(L, st);
         else
           ls_accel_kernel<TG, APR><<<n_groups, TG, 0, s>>>(L, st);
       }
-      if (pack4)
-        ls_end_kernel<kPackThreads, kPackThreads / 16>
-            <<<(unsigned)((n_groups + kPackThreads / 16 - 1) / (kPackThreads / 16)), kPackThreads, 0, s>>>(L);
-      else
-        ls_end_kernel<TG><<<n_groups, TG, 0, s>>>(L);
+      if (!pack4) ls_end_kernel<TG><<<n_groups, TG, 0, s>>>(L);
     }
     return SPT_OK;
   };
