@@ -149,3 +149,24 @@ def test_table_of_distant_rings_converges(tree, post):
         dr, dz = field(x0 + h + h * u, z0 + h + h * v)
         err = np.hypot(bt.cheb_eval(c[0], u, v) - dr, bt.cheb_eval(c[1], u, v) - dz).max() / 1e8
         assert err <= 1e-8, (t, err)
+
+
+def test_split_divides_the_listed_cells(tree, post):
+    """A leaf named in split= becomes four children covering its cell
+    (the refinement tiles.fit applies to cells whose table missed its
+    tolerance); every other leaf stays as it was."""
+    prof = post[0]
+    tr, tiles, size = tree
+    wins = tiles[:, 4] - tiles[:, 3]
+    pick = int(np.nonzero(wins == 0)[0][len(np.nonzero(wins == 0)[0]) // 2])  # a table-only cell
+    cell = tuple(float(v) for v in tiles[pick, :3])
+    tr2, tiles2 = bt.build_tree(prof, 0.0, 0.0, size, alpha=1.0, w_max=5, r_drop=size, split=frozenset([cell]))
+    before = {tuple(float(v) for v in t[:3]) for t in tiles}
+    after = {tuple(float(v) for v in t[:3]) for t in tiles2}
+    assert cell in before and cell not in after
+    kids = after - before
+    x0, z0, s = cell
+    assert len(kids) == 4
+    assert {(k[0], k[1]) for k in kids} == {(x0, z0), (x0 + s / 2, z0), (x0, z0 + s / 2), (x0 + s / 2, z0 + s / 2)}
+    assert all(k[2] == s / 2 for k in kids)
+    assert before - {cell} == after - kids
