@@ -353,7 +353,7 @@ def cpu_baseline(wl, seconds=15.0, n_groups=64, workers=None):
     """The oracle on the first n_groups groups of the workload, spread over
     `workers` forked processes (the reference's own process-level group
     partitioning; default: every host core this process may use, at most
-    16, the GPU box's share), each group traced for seconds * workers /
+    16, one GPU's share of a shared host), each group traced for seconds * workers /
     n_groups of work, so the whole sample takes ~`seconds`. Rate = all
     particle-stages / wall time of the pool. Must run before anything
     touches the GPU (the workers are forked)."""
@@ -381,8 +381,8 @@ def cpu_baseline(wl, seconds=15.0, n_groups=64, workers=None):
                 cpu_model=cpu.get("Model name"), sockets=cpu.get("Socket(s)"),
                 cores_per_socket=cpu.get("Core(s) per socket"), threads_per_core=cpu.get("Thread(s) per core"),
                 host_cpus_online=cpu.get("CPU(s)"), cpus_available_to_process=avail,
-                cap=f"{workers} worker processes: every CPU this process may use, capped at 16 (the GPU box's "
-                    "share of its host)",
+                cap=f"{workers} worker processes: every CPU this process may use, capped at 16 (one GPU's "
+                    "share of a shared host)",
                 sample=f"oracle (NumPy FP64, one BLAS thread per process) on groups 0..{n_groups - 1} "
                        f"({wl['group']} electrons each) of the workload, each traced from birth for "
                        f"{_CPU['seconds']:.2f} s of work (the first, near-surface attempts); {wall:.1f} s wall, "
