@@ -241,8 +241,8 @@ __device__ __forceinline__ NearRule near_rule(const double* g, double proj, doub
 // near_rule for far-rule gathers: the same substitution and interval
 // rule, with asinh x = sign(x) ln(|x| + sqrt(x^2 + 1)) from the LDS log
 // table (absolute error ~2e-16 in v, so the integration limits land within
-// ~2e-16 |s - s0| of the segment's ends) and a reciprocal of d instead of
-// two divisions.
+// ~2e-16 |s - s0| of the segment's ends) and reciprocals (rcp64, ~1 ulp)
+// instead of divisions.
 __device__ __forceinline__ NearRule near_rule_fast(const double* g, double proj, double dist, double width,
                                                    const double2* lt) {
   NearRule r;
@@ -256,9 +256,9 @@ __device__ __forceinline__ NearRule near_rule_fast(const double* g, double proj,
   };
   r.lo = ash(-r.s0 * id);
   const double hi = ash((L - r.s0) * id);
-  r.count = max(1, (int)ceil((hi - r.lo) / width));
-  r.step = (hi - r.lo) / r.count;
-  r.half = 0.5 * (hi - r.lo) / r.count;
+  r.count = max(1, (int)ceil((hi - r.lo) * rcp64(width)));
+  r.step = (hi - r.lo) * rcp64((double)r.count);
+  r.half = 0.5 * r.step;
   return r;
 }
 
@@ -838,7 +838,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
     // all lanes together (tile_lane_segments reduces over the wavefront)
     double rho = 0.0, z = 0.0;
     if (on) {
-      rho = hypot(px[p], py[p]);
+      rho = BIN ? px[p] : hypot(px[p], py[p]);  // the binned gather's points are (rho, 0, z)
       z = pz[p];
     }
     double bd2 = INFINITY;
@@ -897,7 +897,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
     const double bd2 = sh.u.a.bd2[tid];
     const int bseg = sh.u.a.bseg[tid];
     const double x = px[p], y = py[p], z = pz[p];
-    const double rho = hypot(x, y);
+    const double rho = BIN ? x : hypot(x, y);
     const double* g = B.segs + (size_t)bseg * SEG_STRIDE;
     const double L = g[S_L];
     const double thr = 1e-9 * L;
