@@ -987,7 +987,51 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
           }
         }
   };
-  if (tiles && B.n_seg >= 4 * S) {
+  if (tiles && B.n_seg >= 4 * S && BIN && NMAX == 16) {
+    // Few-records form. Lanes outside a tile: every segment outside their
+    // far clusters. Tile lanes: the far nodes of the window's segments are
+    // numbered consecutively (window order, then node order) and node u
+    // goes to slice u mod S, so each slice runs about one node per segment
+    // instead of one slice running a segment's whole rule while the
+    // point's other slices wait (a lone block's latency is this chain); a
+    // near segment j goes to the cooperative pass from slice j mod S, as
+    // below.
+    tile_lane_segments(B, on && !tl, false, 0, B.n_seg, farm, su, S, false, seg);
+    const int wn = (on && tl) ? jhi - jlo : 0;
+    const int wmax = wave_max(wn);
+    int u0 = 0;
+    for (int k = 0; k < wmax; ++k) {
+      if (k < wn) {
+        const int j = jlo + k;
+        const double* g = B.segs + (size_t)j * SEG_STRIDE;
+        const double L = g[S_L];
+        double proj, dist;
+        segment_proj(re, ze, g, proj, dist);
+        if (dist < L) {
+          if (j % S == s) {
+            if (n_near < kNearSlots)
+              sh.u.n.seg[n_near++][tid] = j;
+            else
+              acc.add(near_segment(re, re2, ze, g, B.sigma[j], B.sigma[j + 1], proj, dist, sh.logt, nw));
+          }
+        } else {
+          const int rl = dist >= B.k2 * L ? 2 : (dist >= B.k4 * L ? 1 : 0);
+          const int nq = 8 >> rl, q0 = rl == 0 ? 6 : (rl == 1 ? 0 : 4);  // Gauss-8, -4, -2 (coarse_segment)
+          const bool axw =
+              __ballot(ring_axis_possible(re2, ze, g[S_P0R], g[S_P0Z], g[S_ABR], g[S_ABZ])) != 0ull;
+          const double4* cn = B.coarse + (size_t)j * kCoarsePer + q0;
+          for (int q = ((s - u0) % S + S) % S; q < nq; q += S) {
+            const double4 v = cn[q];
+            if (axw)
+              ring_accum(re, re2, ze - v.y, v.x, v.z, acc, sh.logt);
+            else
+              ring_accum_main(re, re2, ze - v.y, v.x, v.w, v.z, acc, sh.logt);
+          }
+          u0 += nq;
+        }
+      }
+    }
+  } else if (tiles && B.n_seg >= 4 * S) {
     // tile lanes: their window; the others: outside their far clusters
     tile_lane_segments(B, on, tl, jlo, jhi, farm, su, S, wave_major, seg);
   } else if (on) {
