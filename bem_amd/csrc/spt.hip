@@ -637,8 +637,36 @@ __device__ __forceinline__ void tile_rows(C c, double u, double v, int s, int S,
   tv[1] = v;
 #pragma unroll
   for (int b = 2; b < kTileP; ++b) tv[b] = fma(2.0 * v, tv[b - 1], -tv[b - 2]);
-  double ta = 1.0, tam = u;  // T_a(u) and T_{a-1}(u) (T_{-1} = T_1)
   double er = 0.0, ez = 0.0;
+  if (S > 1) {
+    // Several slices: each runs only its own rows, so a wavefront whose
+    // lanes hold different rows waits on ceil(kTileP / S) rows of loads
+    // instead of all kTileP. T_a(u) comes from the same recurrence (same
+    // values, same order of the sums: the results are unchanged).
+    double tu[kTileP];
+    tu[0] = 1.0;
+    tu[1] = u;
+#pragma unroll
+    for (int a = 2; a < kTileP; ++a) tu[a] = fma(2.0 * u, tu[a - 1], -tu[a - 2]);
+#pragma unroll 1
+    for (int a = s; a < kTileP; a += S) {
+      double ta = tu[0];
+#pragma unroll
+      for (int i = 1; i < kTileP; ++i) ta = a == i ? tu[i] : ta;
+      double rr = 0.0, rz = 0.0;
+#pragma unroll
+      for (int b = 0; b < kTileP; ++b) {
+        rr = fma(c[a * kTileP + b], tv[b], rr);
+        rz = fma(c[kTileP * kTileP + a * kTileP + b], tv[b], rz);
+      }
+      er = fma(ta, rr, er);
+      ez = fma(ta, rz, ez);
+    }
+    acc.er_ax += er;
+    acc.ez_ax += ez;
+    return;
+  }
+  double ta = 1.0, tam = u;  // T_a(u) and T_{a-1}(u) (T_{-1} = T_1)
 #pragma unroll 1
   for (int a = 0; a < kTileP; ++a) {
     if (a >= s && (a - s) % S == 0) {
@@ -1111,7 +1139,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
   if (tl) {
     // the tile's table at the (possibly offset) evaluation point
     const double ih = 1.0 / th;
-    tile_eval(B, tleaf, (re - tcx) * ih, (ze - tcz) * ih, s, S, acc);
+    tile_eval(B, tleaf, (re - tcx) * ih, (ze - tcz) * ih, su, S, acc);
   }
   sh.u.n.cnt[tid] = n_near;
   {
