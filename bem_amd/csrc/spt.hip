@@ -824,6 +824,10 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
                                const int* ptile = nullptr) {
   static_assert(T % 64 == 0, "whole wavefronts");
   static_assert(!BIN || (FAR && (NMAX == 64 || NMAX == 16)), "the binned gather is a far-rule gather of 64 or 16 points");
+  // The few-records form's slice reductions by shuffles: only where a
+  // point's T / 16 slices are aligned lanes of one wavefront (not T = 192,
+  // whose 12 slices per point straddle wavefronts).
+  constexpr bool kFewShfl = BIN && NMAX == 16 && 64 % (T / 16) == 0;
   if (m <= 0) return;
   fill_log_table<T>(sh.logt);  // read after the barrier that ends the closest-segment pass
   const int tid = threadIdx.x;
@@ -879,6 +883,18 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
         bseg = j;
       }
     });
+    if (kFewShfl) {
+      // few-records form: a point's slices are aligned lanes of one
+      // wavefront, so the minimum is taken by shuffles, without barriers
+      for (int h = 1; h < S; h <<= 1) {
+        const double d = __shfl_xor(bd2, h);
+        const int sg = __shfl_xor(bseg, h);
+        if (d < bd2 || (d == bd2 && sg < bseg)) {
+          bd2 = d;
+          bseg = sg;
+        }
+      }
+    }
     if (on) {
       sh.u.a.bd2[tid] = bd2;
       sh.u.a.bseg[tid] = bseg;
@@ -908,8 +924,10 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
   }
   // Closest segment over the point's slices: a fixed-pairing tree (slot(s)
   // is this thread; the (distance, index) minimum does not depend on the
-  // pairing).
-  for (int h = 1; h < S; h <<= 1) {
+  // pairing). The few-records form with tiles has it already (shuffles);
+  // there slice 0 reads back only its own LDS slot below.
+  const bool shfl = kFewShfl && tiles;
+  for (int h = 1; h < S && !shfl; h <<= 1) {
     __syncthreads();
     if (on && (s & (2 * h - 1)) == 0 && s + h < S) {
       const double d = sh.u.a.bd2[slot(s + h)];
@@ -920,7 +938,7 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
       }
     }
   }
-  __syncthreads();
+  if (!shfl) __syncthreads();
   if (on && s == 0) {
     const double bd2 = sh.u.a.bd2[tid];
     const int bseg = sh.u.a.bseg[tid];
@@ -1195,6 +1213,27 @@ __device__ void bem_pert_block(const BemView& B, int m, const double* px, const 
       acc.er_main += acc32.er.value();
       acc.ez_main += acc32.ez.value();
     }
+  }
+  if (kFewShfl) {
+    // few-records form: the slices are aligned lanes of one wavefront;
+    // the tree below by shuffles (lane s takes s ^ h, i.e. s + h where the
+    // tree adds: the same pairing and order, so the same sums)
+    double v[4] = {acc.er_main, acc.ez_main, acc.er_ax, acc.ez_ax};
+    for (int h = 1; h < S; h <<= 1) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v[c] += __shfl_xor(v[c], h);
+    }
+    if (on && s == 0) {
+      const RingAcc tot = {v[0], v[1], v[2], v[3]};
+      double er, ez;
+      ring_finish(tot, sh.re[p], er, ez);
+      const bool axis = sh.rho[p] == 0.0;
+      ox[p] = axis ? 0.0 : er * sh.cph[p];
+      oy[p] = axis ? 0.0 : er * sh.sph[p];
+      oz[p] = ez;
+    }
+    __syncthreads();
+    return;
   }
   __syncthreads();  // near-list reads done before acc (aliased) is written
   sh.u.acc[0][tid] = acc.er_main;
