@@ -56,7 +56,7 @@ TRAFFIC_BYTES = 19.8e6
 TRAFFIC_SOURCE = ("FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB copy), fixed rule at (256, 64), "
                   "profiles/bem_field_gather_post400_262k.md")
 FAST_TRAFFIC_SHAPE = (128, 64)
-FAST_TRAFFIC_BYTES = 14.2e6
+FAST_TRAFFIC_BYTES = 14.0e6
 FAST_TRAFFIC_SOURCE = ("FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB copy), fast-form gather with "
                        "tiles at (128, 64) on the points in bin order, "
                        "profiles/bem_field_gather_fast_form_post400_262k.md")
