@@ -2861,9 +2861,13 @@ template <int TG, int GPB>
 __device__ void ls_end_group(const LockParams& L, int g, int flags, int tid, double* red);
 
 // LAST: stage 7 (st = 6), followed by the end of the attempt in the same
-// kernel (ls_end_group) instead of a launch of ls_end_kernel.
-template <int TG, int PR, int GPB = 1, bool LAST = false>
-__global__ __launch_bounds__(TG) void ls_accel_kernel(LockParams L, int st) {
+// kernel (ls_end_group) instead of a launch of ls_end_kernel. ST >= 0: the
+// stage as a compile-time constant (st is then ignored), so the stage
+// combination below keeps only the terms that stage uses; the values are
+// the same as with the stage given at run time.
+template <int TG, int PR, int GPB = 1, bool LAST = false, int ST = -1>
+__global__ __launch_bounds__(TG) void ls_accel_kernel(LockParams L, int st_arg) {
+  const int st = ST >= 0 ? ST : st_arg;
   constexpr int W = TG / GPB;  // lanes per group
   __shared__ double sx[TG], sy[TG], sz[TG], sq[TG];
   __shared__ int act[TG];
@@ -4254,10 +4258,21 @@ static int run_lockstep(spt_ctx* c, LockParams L, int n_groups, int64_t n, Field
         }
         // packed groups: stage 7's kernel also ends the attempt
         const unsigned nb_pack = (unsigned)((n_groups + kPackThreads / 16 - 1) / (kPackThreads / 16));
+        constexpr int GP = kPackThreads / 16;
         if (pack4 && st == 6)
-          ls_accel_kernel<kPackThreads, APR, kPackThreads / 16, true><<<nb_pack, kPackThreads, 0, s>>>(L, st);
+          ls_accel_kernel<kPackThreads, APR, GP, true, 6><<<nb_pack, kPackThreads, 0, s>>>(L, st);
+        else if (pack4 && st == 0)
+          ls_accel_kernel<kPackThreads, APR, GP, false, 0><<<nb_pack, kPackThreads, 0, s>>>(L, st);
+        else if (pack4 && st == 1)
+          ls_accel_kernel<kPackThreads, APR, GP, false, 1><<<nb_pack, kPackThreads, 0, s>>>(L, st);
+        else if (pack4 && st == 2)
+          ls_accel_kernel<kPackThreads, APR, GP, false, 2><<<nb_pack, kPackThreads, 0, s>>>(L, st);
+        else if (pack4 && st == 3)
+          ls_accel_kernel<kPackThreads, APR, GP, false, 3><<<nb_pack, kPackThreads, 0, s>>>(L, st);
+        else if (pack4 && st == 4)
+          ls_accel_kernel<kPackThreads, APR, GP, false, 4><<<nb_pack, kPackThreads, 0, s>>>(L, st);
         else if (pack4)
-          ls_accel_kernel<kPackThreads, APR, kPackThreads / 16><<<nb_pack, kPackThreads, 0, s>>>(L, st);
+          ls_accel_kernel<kPackThreads, APR, GP, false, 5><<<nb_pack, kPackThreads, 0, s>>>(L, st);
         else
           ls_accel_kernel<TG, APR><<<n_groups, TG, 0, s>>>(L, st);
       }
