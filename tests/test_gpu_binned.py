@@ -14,7 +14,9 @@ other points of its block (each lane picks its own rule). So:
   takes the append path (slots in atomic order, mode 2);
 * against the unbinned trace (particle order, whose partial blocks sum with
   another slice count) it keeps every crossing, times within 1e-7
-  relative."""
+  relative, at the default launch shape and at 64, 128 and 256 threads
+  (the few-records form's slice reductions are wavefront shuffles there,
+  LDS trees at 192)."""
 import sys
 
 import numpy as np
@@ -66,8 +68,7 @@ def test_far_gather_is_order_independent(post, shape):
 
 
 @pytest.fixture(scope="module")
-def traces(post):
-    from bem_amd import SpecificParticleTracer
+def workload(post):
     from bem_amd.bem import budget
     from bem_amd.emission import cap_source
     from bem_amd.particles import ParticleGroup
@@ -75,12 +76,34 @@ def traces(post):
     n = 16384
     src = cap_source(n, post.R, post.H, theta_max_deg=60.0, mean_ke_ev=0.2, seed=0)
     pg = ParticleGroup(dict(src, weight=np.full(n, 1.602176634e-19), status=np.ones(n), species="electron"))
-    bud = budget.fit(post, tol=1e-6, n_val=1 << 12)
+    return pg, budget.fit(post, tol=1e-6, n_val=1 << 12)
+
+
+def _trace(post, workload, binned):
+    from bem_amd import SpecificParticleTracer
+
+    pg, bud = workload
+    tr = SpecificParticleTracer(pg, 16, post, [90e-6], rtol=1e-8, atol=1e-12, z_max=100e-6, mode="lockstep",
+                                precision="pairs32", binned=binned, **bud.tracer_kwargs())
+    return tr.trace()
+
+
+def _same_crossings(a, b, what):
+    ka = np.argsort(a["events"]["group"].astype(np.int64) * 16 + a["events"]["particle"])
+    kb = np.argsort(b["events"]["group"].astype(np.int64) * 16 + b["events"]["particle"])
+    np.testing.assert_array_equal(a["events"]["group"][ka], b["events"]["group"][kb])
+    np.testing.assert_array_equal(a["events"]["particle"][ka], b["events"]["particle"][kb])
+    dt = np.abs(a["events"]["t"][ka] - b["events"]["t"][kb]) / b["events"]["t"][kb]
+    _say(f"{what}: crossing time rel diff max {dt.max():.2e}, attempts differ in "
+         f"{int(np.sum(a['attempts'] != b['attempts']))} of {len(a['attempts'])} groups")
+    assert dt.max() <= 1e-7
+
+
+@pytest.fixture(scope="module")
+def traces(post, workload):
     runs = {}
     for name, binned in (("binned", 1), ("binned_again", 1), ("append", 2), ("unbinned", 0)):
-        tr = SpecificParticleTracer(pg, 16, post, [90e-6], rtol=1e-8, atol=1e-12, z_max=100e-6, mode="lockstep",
-                                    precision="pairs32", binned=binned, **bud.tracer_kwargs())
-        runs[name] = o = tr.trace()
+        runs[name] = o = _trace(post, workload, binned)
         _say(f"16,384 e- [{name}]: {o['device_ms']:.1f} ms, attempts mean {o['attempts'].mean():.2f}, "
              f"{len(o['events'])} events")
     return runs
@@ -98,12 +121,19 @@ def test_binned_trace_is_reproducible_and_slot_order_free(traces):
 
 
 def test_binned_trace_matches_the_unbinned_one(traces):
-    a, b = traces["binned"], traces["unbinned"]
-    ka = np.argsort(a["events"]["group"].astype(np.int64) * 16 + a["events"]["particle"])
-    kb = np.argsort(b["events"]["group"].astype(np.int64) * 16 + b["events"]["particle"])
-    np.testing.assert_array_equal(a["events"]["group"][ka], b["events"]["group"][kb])
-    np.testing.assert_array_equal(a["events"]["particle"][ka], b["events"]["particle"][kb])
-    dt = np.abs(a["events"]["t"][ka] - b["events"]["t"][kb]) / b["events"]["t"][kb]
-    _say(f"binned vs unbinned: crossing time rel diff max {dt.max():.2e}, attempts differ in "
-         f"{int(np.sum(a['attempts'] != b['attempts']))} of {len(a['attempts'])} groups")
-    assert dt.max() <= 1e-7
+    _same_crossings(traces["binned"], traces["unbinned"], "binned vs unbinned")
+
+
+@pytest.mark.parametrize("threads", [64, 128, 256])
+def test_binned_trace_at_each_shuffle_shape(post, workload, traces, threads):
+    """At T = 64, 128 and 256 the few-records form (T / 16 = 4, 8, 16 slices
+    per point) reduces over a point's slices by wavefront shuffles; the
+    binned trace there keeps the unbinned trace's crossings."""
+    ctx = post._ctx()
+    ctx.check(ctx.lib.spt_set_field_shape(ctx.h, threads, 64), "spt_set_field_shape")
+    try:
+        a = _trace(post, workload, 1)
+    finally:
+        ctx.lib.spt_set_field_shape(ctx.h, 0, 0)
+    assert np.all(a["status"] == 0) and len(a["events"]) == 16384
+    _same_crossings(a, traces["unbinned"], f"binned at ({threads}, 64) vs unbinned")
